@@ -1,0 +1,299 @@
+/*
+ * fsi_pbf.h — C-ABI of the MI355X (gfx950) position-based-fluids substep.
+ *
+ * This is the drop-in boundary for the fluid hot path of Realtime-FSI-Engine.
+ * The reference drives the substep from the C++ class `PBFluids`
+ * (include/fluid/PBFluids.h:88-268), which dispatches GLSL compute shaders over
+ * OpenGL SSBOs (src/fluid/PBFluids.cpp:323-517).  Every entry point below
+ * replaces one member of that class (cited per function); the C++ mirror class
+ * `PBFluids` in realtime-fsi-engine_amd/host/PBFluids.h is written on top of it.
+ *
+ * Conventions
+ *   - Plain pointers and sizes only; host pointers unless the name says `dev`.
+ *   - Every call returns an int status: FSI_OK (0) or a negative FSI_ERR_*.
+ *     fsi_last_error() returns the text of the last failure on this thread.
+ *   - One handle owns one HIP stream on the device that was current when it was
+ *     created.  A handle is not thread-safe; distinct handles are independent.
+ *   - `step` is asynchronous (enqueued on the handle's stream); every `read_*`
+ *     call synchronises that stream before copying out.
+ *   - Particle records are the reference's 48-byte std430 layout
+ *     (include/fluid/Particle.h:8-12): pos.xyz, pad, predPos.xyz, pad, vel.xyz, pad.
+ */
+#ifndef FSI_PBF_H
+#define FSI_PBF_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FSI_OK 0
+#define FSI_ERR_INVALID_ARG (-1) /* bad pointer, size or parameter value          */
+#define FSI_ERR_HIP (-2)         /* a HIP runtime call failed                      */
+#define FSI_ERR_STATE (-3)       /* call not valid in the handle's current state   */
+#define FSI_ERR_CAPACITY (-4)    /* caller buffer too small                        */
+#define FSI_ERR_NO_DEVICE (-5)   /* no usable gfx950 device                        */
+
+#define FSI_PBF_ABI_VERSION 1
+
+/* ------------------------------------------------------------------------- */
+/* Types                                                                      */
+/* ------------------------------------------------------------------------- */
+
+/* One particle, 48 bytes (include/fluid/Particle.h:8-12). */
+typedef struct fsi_particle {
+    float pos[3];
+    float pad0;
+    float pred_pos[3]; /* never written by the solver: kept only for the layout */
+    float pad1;
+    float vel[3];
+    float pad2;
+} fsi_particle;
+
+/* Contact candidate, 32 bytes (include/fluid/PBFluids.h:67-70). */
+typedef struct fsi_contact_candidate {
+    float pos[4]; /* xyz world position, w = 0              */
+    float vel[4]; /* xyz velocity, w = the particle's pad2  */
+} fsi_contact_candidate;
+
+/* Solver parameters, field for field the reference's FluidConfig
+ * (include/core/config.h:9-63), same order; bools are uint32 (0/1). */
+typedef struct fsi_fluid_config {
+    float dt;
+    float gravity[3];
+    float h;
+    float rho0;
+    uint32_t solver_iterations;
+    uint32_t substep_iterations;
+    float eps;
+    uint32_t particle_count;
+    float spawn_min[3];
+    float spawn_max[3];
+    uint32_t spawn_random;
+    float spacing;
+    float initial_velocity[3];
+    float bounds_min[3];
+    float bounds_max[3];
+    float bound_damping;
+    uint32_t hash_size;
+    uint32_t enable_scorr;
+    float k_corr;
+    float n_corr;
+    float delta_q;
+    float cohesion_strength;
+    uint32_t enable_viscosity;
+    float viscosity;
+    float interaction_radius;
+    float interaction_strength;
+    uint32_t enable_vorticity;
+    float vorticity_epsilon;
+    uint32_t enable_apbf;
+    uint32_t min_lod;
+    uint32_t max_lod;
+    float lod_max_dist;
+} fsi_fluid_config;
+
+/* The packed per-step constant block, byte for byte the reference's std140
+ * FluidConfigUBO (include/fluid/PBFluids.h:21-61), 144 bytes.  Exposed so a
+ * caller can check the coefficient precompute of PBFluids::setParams
+ * (src/fluid/PBFluids.cpp:39-109). */
+typedef struct fsi_pbf_constants {
+    float bounds_min[4];
+    float bounds_max[4];
+    float gravity_dt[4]; /* xyz gravity, w = dt / substeps */
+    float h, rho0, eps, wq;
+    float k_corr, n_corr, viscosity, bound_damping;
+    uint32_t hash_size, particle_count, enable_scorr, enable_viscosity;
+    float cohesion_strength, interaction_radius, interaction_strength, w0_self;
+    uint32_t hash_mask;
+    float poly6_coeff, spiky_coeff, inv_rho0;
+    uint32_t min_lod, max_lod;
+    float lod_max_dist;
+    uint32_t enable_apbf;
+} fsi_pbf_constants;
+
+/* Per-pass device timing (HIP events on the handle's stream), accumulated
+ * since the last fsi_pbf_timing_reset. */
+enum fsi_pbf_pass {
+    FSI_PASS_PREDICT = 0,   /* predict + hash                               */
+    FSI_PASS_SORT = 1,      /* all radix passes of one substep              */
+    FSI_PASS_TABLES = 2,    /* cell start/end tables + reorder              */
+    FSI_PASS_LAMBDA = 3,    /* density / lambda                             */
+    FSI_PASS_DELTAP = 4,    /* position correction + apply + SDF projection */
+    FSI_PASS_INTEGRATE = 5, /* XSPH, curl, collisions, velocity             */
+    FSI_PASS_VORTICITY = 6, /* vorticity confinement, once per frame        */
+    FSI_PASS_LOD = 7,       /* APBF level of detail                         */
+    FSI_PASS_COUNT = 8
+};
+
+typedef struct fsi_pbf_timing {
+    double ms[FSI_PASS_COUNT];       /* summed kernel time per pass           */
+    uint64_t launches[FSI_PASS_COUNT]; /* number of timed launches per pass   */
+    uint64_t substeps;
+    uint64_t frames;
+} fsi_pbf_timing;
+
+typedef struct fsi_pbf fsi_pbf; /* opaque handle */
+
+/* ------------------------------------------------------------------------- */
+/* Library                                                                    */
+/* ------------------------------------------------------------------------- */
+
+/* Text of the last error raised on the calling thread ("" if none). */
+const char* fsi_last_error(void);
+
+/* ABI version of the loaded library (FSI_PBF_ABI_VERSION). */
+int fsi_abi_version(void);
+
+/* The reference's FluidConfig defaults (include/core/config.h:9-63). */
+int fsi_fluid_config_default(fsi_fluid_config* out);
+
+/* Number of visible HIP devices, without touching any of them. */
+int fsi_device_count(int* count);
+
+/* ------------------------------------------------------------------------- */
+/* Solver handle — replaces class PBFluids (include/fluid/PBFluids.h:88-268)  */
+/* ------------------------------------------------------------------------- */
+
+/* PBFluids::PBFluids(const FluidConfig&)  (src/fluid/PBFluids.cpp:12-37).
+ * Binds the current HIP device, creates the stream, applies setParams. */
+int fsi_pbf_create(const fsi_fluid_config* cfg, fsi_pbf** out);
+
+/* ~PBFluids (RAII release of every buffer, src/opengl/SSBO.cpp:18-21). */
+int fsi_pbf_destroy(fsi_pbf* h);
+
+/* PBFluids::setParams  (src/fluid/PBFluids.cpp:39-109).  Resets the bounds
+ * from cfg (setBounds with the current collision padding), recomputes the
+ * packed constants.  hash_size must be a power of two >= 2; a changed
+ * hash_size reallocates the cell table.  The particle count stays the one
+ * given to fsi_pbf_set_particles (cfg->particle_count is ignored after that). */
+int fsi_pbf_set_params(fsi_pbf* h, const fsi_fluid_config* cfg);
+
+/* PBFluids::params()  (include/fluid/PBFluids.h:97). */
+int fsi_pbf_get_params(fsi_pbf* h, fsi_fluid_config* out);
+
+/* The packed constants of the last setParams/setBounds/setParticles. */
+int fsi_pbf_get_constants(fsi_pbf* h, fsi_pbf_constants* out);
+
+/* PBFluids::setParticles  (src/fluid/PBFluids.cpp:111-152): copies n records
+ * to the device and (re)allocates every per-particle buffer. n may be 0. */
+int fsi_pbf_set_particles(fsi_pbf* h, const fsi_particle* particles, uint32_t n);
+
+/* Number of particles currently held. */
+int fsi_pbf_particle_count(fsi_pbf* h, uint32_t* n);
+
+/* PBFluids::readbackParticles  (src/fluid/PBFluids.cpp:154-191).  Copies the
+ * current state (all 48 bytes per record) into out[0..n); cap must be >= n. */
+int fsi_pbf_read_particles(fsi_pbf* h, fsi_particle* out, uint32_t cap, uint32_t* n);
+
+/* PBFluids::setCollisionPadding  (include/fluid/PBFluids.h:108).  Like the
+ * reference it takes effect at the next set_bounds / set_params. */
+int fsi_pbf_set_collision_padding(fsi_pbf* h, float padding);
+
+/* PBFluids::setBounds  (src/fluid/PBFluids.cpp:299-306): bounds shrunk by the
+ * collision padding, damping clamped to [0,1]. */
+int fsi_pbf_set_bounds(fsi_pbf* h, const float bounds_min[3], const float bounds_max[3],
+                       float damping);
+
+/* PBFluids::setSDFBoundary + SDFBoundary::uploadToGPU
+ * (include/fluid/PBFluids.h:120; src/fluid/SDFBoundary.cpp:580-598).
+ * grid holds dims[0]*dims[1]*dims[2] signed distances, x fastest (the texture
+ * order of the reference), voxel (i,j,k) centred at origin + (i,j,k)*cell.
+ * grid == NULL detaches the boundary.  The data is copied. */
+int fsi_pbf_set_sdf(fsi_pbf* h, const float* grid, const int32_t dims[3],
+                    const float origin[3], float cell);
+
+/* PBFluids::setSDFPadding / setSDFContainment  (include/fluid/PBFluids.h:124,130). */
+int fsi_pbf_set_sdf_padding(fsi_pbf* h, float padding);
+int fsi_pbf_set_sdf_containment(fsi_pbf* h, int inside);
+
+/* PBFluids::setInteraction  (src/fluid/PBFluids.cpp:312-317). */
+int fsi_pbf_set_interaction(fsi_pbf* h, int active, const float pos[3]);
+
+/* PBFluids::setCameraPos  (include/fluid/PBFluids.h:100). */
+int fsi_pbf_set_camera(fsi_pbf* h, const float pos[3]);
+
+/* PBFluids::step  (src/fluid/PBFluids.cpp:323-517): substep_iterations
+ * substeps and one vorticity pass, enqueued on the handle's stream. */
+int fsi_pbf_step(fsi_pbf* h);
+
+/* Block until all work enqueued on the handle has finished. */
+int fsi_pbf_sync(fsi_pbf* h);
+
+/* PBFluids::filterContactCandidates  (src/fluid/PBFluids.cpp:193-237):
+ * particles with |sdf(pos)| < band are compacted, in particle-id order. */
+int fsi_pbf_filter_contacts(fsi_pbf* h, float band);
+
+/* PBFluids::readbackContactCandidates  (src/fluid/PBFluids.cpp:239-284).
+ * *count receives the number of candidates (clamped to the particle count);
+ * at most cap records are copied. */
+int fsi_pbf_read_contacts(fsi_pbf* h, fsi_contact_candidate* out, uint32_t cap,
+                          uint32_t* count);
+
+/* ------------------------------------------------------------------------- */
+/* Inspection taps (parity checks; the reference exposes these buffers only   */
+/* as SSBOs: hashGrid, offsets, SolverBuffer — PBFluids.h:216-233)            */
+/* ------------------------------------------------------------------------- */
+
+/* Sorted (key, id) pairs and the cell table of the last substep.
+ * keys/ids: n entries each; cells: 2*hash_size int32 (start, end), -1 = empty.
+ * Any pointer may be NULL to skip it. */
+int fsi_pbf_read_grid(fsi_pbf* h, uint32_t* keys, uint32_t* ids, int32_t* cells);
+
+/* SolverBuffer in particle-id order: predPos_lambda and deltaP_rho as 4 floats
+ * each (CS_ComputeLambdas.glsl:50-53).  Either pointer may be NULL. */
+int fsi_pbf_read_solver(fsi_pbf* h, float* pred_pos_lambda, float* delta_p_rho);
+
+/* Enable the diagnostic taps (neighbour counts of every density pass).
+ * Off by default; the taps cost one extra store per particle per pass. */
+int fsi_pbf_set_taps(fsi_pbf* h, int enable);
+
+/* Per-particle counts from the last density pass, in particle-id order:
+ * candidates = bucket entries visited other than the particle itself,
+ * neighbours = those with |x_i - x_j|^2 <= h^2 (aliased buckets count twice,
+ * as in CS_ComputeLambdas.glsl:126-150).  Requires taps. */
+int fsi_pbf_read_counts(fsi_pbf* h, uint32_t* candidates, uint32_t* neighbours);
+
+/* Per-pass device timing. */
+int fsi_pbf_set_timing(fsi_pbf* h, int enable);
+int fsi_pbf_timing_reset(fsi_pbf* h);
+int fsi_pbf_read_timing(fsi_pbf* h, fsi_pbf_timing* out);
+
+/* Device pointer of the HIP stream the handle enqueues on (hipStream_t). */
+int fsi_pbf_get_stream(fsi_pbf* h, void** stream);
+
+/* ------------------------------------------------------------------------- */
+/* Host-side helpers for scenes and boundaries                                */
+/* ------------------------------------------------------------------------- */
+
+/* Lattice block spawn (the grid branch of spawnParticles, src/main.cpp:160-188,
+ * with a seeded jitter): nx*ny*nz particles at
+ * origin + (i + 0.5)*spacing + U(-jitter, +jitter) per axis, ids x-fastest,
+ * then y, then z; velocity v0; all pads 0.  The jitter draws come from an
+ * MT19937 generator seeded with `seed`, three draws per particle (x, y, z),
+ * u = (draw >> 8) * 2^-24, offset = (2u - 1) * jitter. */
+int fsi_scene_lattice(uint32_t nx, uint32_t ny, uint32_t nz, const float origin[3],
+                      float spacing, float jitter, uint32_t seed, const float v0[3],
+                      fsi_particle* out);
+
+/* Icosphere: subdivisions of a unit icosahedron projected on the sphere
+ * (centre c, radius r).  Sizes: V = 10*4^s + 2, F = 20*4^s. */
+int fsi_mesh_icosphere(uint32_t subdivisions, const double centre[3], double radius,
+                       double* verts /* 3V */, int32_t* faces /* 3F */);
+
+/* Signed-distance grid of a closed triangle mesh, as
+ * SDFBoundary::buildFromMesh (src/fluid/SDFBoundary.cpp:278-436): FP64 BVH
+ * closest point, pseudo-normal sign, origin = bbox - 5*cell, dims =
+ * ceil(span / cell).  Two calls: with grid == NULL only origin/dims are
+ * returned; then grid (dims product floats) is filled in x-fastest order,
+ * ready for fsi_pbf_set_sdf. */
+int fsi_sdf_build(const double* verts, uint32_t nverts, const int32_t* faces, uint32_t nfaces,
+                  double cell, double origin[3], int32_t dims[3], float* grid);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FSI_PBF_H */

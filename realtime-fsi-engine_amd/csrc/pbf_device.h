@@ -1,0 +1,181 @@
+// pbf_device.h — device-side constants and per-pair arithmetic of the PBF substep.
+//
+// Every expression keeps the operand order of the reference shaders
+// (resources/shaders/compute/CS_*.glsl); the library is compiled with
+// -ffp-contract=off and IEEE-rounded division / square root, so each FP32
+// operation rounds exactly once, as in the shader source read literally.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fsi {
+
+// Constants of one step: the reference's FluidConfigUBO
+// (include/fluid/PBFluids.h:21-61) plus the per-dispatch uniforms that
+// PBFluids::step sets (src/fluid/PBFluids.cpp:432-504).  Passed by value.
+struct StepConst {
+    float bmin[3], bmax[3], g[3], sub_dt;
+    float h, rho0, eps, wq;
+    float k_corr, n_corr, viscosity, damping;
+    float cohesion, irad, istr, w0_self;
+    float poly6, spiky, inv_rho0, lod_max_dist;
+    uint32_t hash_mask, n, enable_scorr, enable_visc;
+    uint32_t min_lod, max_lod, enable_apbf, enable_vort;
+    uint32_t interacting;
+    float ipos[3], cam[3], vort_eps;
+    // SDF boundary (R32F grid, x fastest, voxel centres at origin + i*cell)
+    uint32_t enable_sdf, sdf_mode;
+    int32_t dims[3];
+    float sdf_origin[3], sdf_cell, sdf_pad;
+    const float* sdf_grid;
+};
+
+struct V3 {
+    float x, y, z;
+};
+
+__device__ __forceinline__ V3 mk(float x, float y, float z) { return V3{x, y, z}; }
+__device__ __forceinline__ V3 add(V3 a, V3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ V3 sub(V3 a, V3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ V3 scl(V3 a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ V3 dvs(V3 a, float s) { return mk(a.x / s, a.y / s, a.z / s); }
+__device__ __forceinline__ float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ float len(V3 a) { return sqrtf(dot(a, a)); }
+__device__ __forceinline__ V3 cross(V3 a, V3 b) {
+    return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ V3 xyz(float4 v) { return mk(v.x, v.y, v.z); }
+
+// CS_PredictAndHash.glsl:77-83 — wrapping int32 products, XOR, mask.
+__device__ __forceinline__ uint32_t cell_hash(int32_t cx, int32_t cy, int32_t cz, uint32_t mask) {
+    const uint32_t n = ((uint32_t)cx * 73856093u) ^ ((uint32_t)cy * 19349663u) ^ ((uint32_t)cz * 83492791u);
+    return n & mask;
+}
+
+// ivec3(floor(x / h)) — a true division (CS_PredictAndHash.glsl:125).
+__device__ __forceinline__ void cell_of(V3 x, float h, int32_t& cx, int32_t& cy, int32_t& cz) {
+    cx = (int32_t)floorf(x.x / h);
+    cy = (int32_t)floorf(x.y / h);
+    cz = (int32_t)floorf(x.z / h);
+}
+
+// Poly6 (CS_ComputeLambdas.glsl:74-81) given r2.
+__device__ __forceinline__ float poly6_r2(const StepConst& c, float r2) {
+    const float h2 = c.h * c.h;
+    if (r2 > h2) return 0.0f;
+    const float diff = h2 - r2;
+    return c.poly6 * diff * diff * diff;
+}
+
+// Spiky gradient (CS_ComputeLambdas.glsl:83-92).
+__device__ __forceinline__ V3 spiky(const StepConst& c, V3 r, float r2) {
+    const float h2 = c.h * c.h;
+    if (r2 > h2 || r2 < 1e-10f) return mk(0.0f, 0.0f, 0.0f);
+    const float dist = sqrtf(r2);
+    const float diff = c.h - dist;
+    const float scalar = -c.spiky * diff * diff / dist;
+    return scl(r, scalar);
+}
+
+// x^n: binary powering for integer n in [0, 64], powf otherwise.
+__device__ __forceinline__ float powr(float x, float n) {
+    if (n >= 0.0f && n <= 64.0f && n == floorf(n)) {
+        uint32_t e = (uint32_t)n;
+        float result = 1.0f;
+        float base = x;
+        while (e) {
+            if (e & 1u) result = result * base;
+            e >>= 1u;
+            if (e) base = base * base;
+        }
+        return result;
+    }
+    return powf(x, n);
+}
+
+// s_corr (CS_ComputeDeltaP.glsl:94-104) given r2.
+__device__ __forceinline__ float scorr_r2(const StepConst& c, float r2) {
+    if (c.enable_scorr == 0u) return 0.0f;
+    const float w = poly6_r2(c, r2);
+    if (c.wq <= 1e-12f || w <= 0.0f) return 0.0f;
+    const float ratio = w / c.wq;
+    if (ratio <= 0.0f) return 0.0f;
+    return -c.k_corr * powr(ratio, c.n_corr);
+}
+
+// ---- SDF: GL_LINEAR + CLAMP_TO_EDGE emulation (Texture3D.cpp:25-32) ----
+
+__device__ __forceinline__ void sdf_taps(float tc, int32_t size, int32_t& i0, int32_t& i1, float& a) {
+    const float uu = tc * (float)size - 0.5f;
+    float f = floorf(uu);
+    a = uu - f;
+    if (!(f >= -1.0f)) f = -1.0f;
+    if (f > (float)size) f = (float)size;
+    int32_t lo = (int32_t)f, hi = lo + 1;
+    lo = lo < 0 ? 0 : (lo > size - 1 ? size - 1 : lo);
+    hi = hi < 0 ? 0 : (hi > size - 1 ? size - 1 : hi);
+    i0 = lo;
+    i1 = hi;
+}
+
+__device__ __forceinline__ float lerp1(float t0, float t1, float a) { return (1.0f - a) * t0 + a * t1; }
+
+// querySDF (CS_Integrate.glsl:86-89).
+__device__ __forceinline__ float sdf_query(const StepConst& c, V3 p) {
+    const float tx = ((p.x - c.sdf_origin[0]) / c.sdf_cell + 0.5f) / (float)c.dims[0];
+    const float ty = ((p.y - c.sdf_origin[1]) / c.sdf_cell + 0.5f) / (float)c.dims[1];
+    const float tz = ((p.z - c.sdf_origin[2]) / c.sdf_cell + 0.5f) / (float)c.dims[2];
+    int32_t x0, x1, y0, y1, z0, z1;
+    float ax, ay, az;
+    sdf_taps(tx, c.dims[0], x0, x1, ax);
+    sdf_taps(ty, c.dims[1], y0, y1, ay);
+    sdf_taps(tz, c.dims[2], z0, z1, az);
+    const size_t W = (size_t)c.dims[0], WH = (size_t)c.dims[0] * (size_t)c.dims[1];
+    const float* g = c.sdf_grid;
+    const size_t r00 = (size_t)y0 * W + (size_t)z0 * WH, r10 = (size_t)y1 * W + (size_t)z0 * WH;
+    const size_t r01 = (size_t)y0 * W + (size_t)z1 * WH, r11 = (size_t)y1 * W + (size_t)z1 * WH;
+    const float c00 = lerp1(g[r00 + x0], g[r00 + x1], ax);
+    const float c10 = lerp1(g[r10 + x0], g[r10 + x1], ax);
+    const float c01 = lerp1(g[r01 + x0], g[r01 + x1], ax);
+    const float c11 = lerp1(g[r11 + x0], g[r11 + x1], ax);
+    const float c0 = lerp1(c00, c10, ay);
+    const float c1 = lerp1(c01, c11, ay);
+    return lerp1(c0, c1, az);
+}
+
+// sdfGradient (CS_ApplyDeltaP.glsl:58-65).
+__device__ __forceinline__ V3 sdf_gradient(const StepConst& c, V3 p) {
+    const float e = c.sdf_cell;
+    const float z = 0.0f;
+    const float gx = sdf_query(c, mk(p.x + e, p.y + z, p.z + z)) - sdf_query(c, mk(p.x - e, p.y - z, p.z - z));
+    const float gy = sdf_query(c, mk(p.x + z, p.y + e, p.z + z)) - sdf_query(c, mk(p.x - z, p.y - e, p.z - z));
+    const float gz = sdf_query(c, mk(p.x + z, p.y + z, p.z + e)) - sdf_query(c, mk(p.x - z, p.y - z, p.z - e));
+    return dvs(mk(gx, gy, gz), 2.0f * e);
+}
+
+// SDF projection shared by CS_ApplyDeltaP.glsl:108-121 and CS_Integrate.glsl:216-240.
+__device__ __forceinline__ bool sdf_project(const StepConst& c, V3& p, V3& nrm) {
+    const float pad = c.sdf_pad;
+    const float d = sdf_query(c, p);
+    const V3 g = sdf_gradient(c, p);
+    const float l = len(g);
+    if (l > 1e-6f) {
+        if (c.sdf_mode == 1u) {
+            if (d > -pad) {
+                nrm = dvs(mk(-g.x, -g.y, -g.z), l);
+                p = add(p, scl(nrm, d + pad));
+                return true;
+            }
+        } else {
+            if (d < pad) {
+                nrm = dvs(g, l);
+                p = add(p, scl(nrm, pad - d));
+                return true;
+            }
+        }
+    }
+    return false;
+}
+
+}  // namespace fsi

@@ -1,0 +1,72 @@
+// pbf_launch.h — internal launch interface between the solver handle
+// (pbf_capi.hip) and the kernels (pbf_kernels.hip, radix_sort.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "pbf_device.h"
+
+namespace fsi {
+
+// Keys per radix tile = 256 threads x kSortIPT.
+constexpr int kSortIPT = 8;
+// Widest radix digit the solver plans with (LDS: 44 KiB per workgroup at 10).
+constexpr int kSortMaxDigitBits = 10;
+
+struct SortScratch {
+    uint32_t* keys[2];  // ping-pong key buffers, n each
+    uint32_t* ids[2];   // ping-pong id buffers, n each
+    uint32_t* hist;     // sort_hist_entries(n, kSortMaxDigitBits)
+    uint32_t* partial;  // scan_partials_needed(hist entries)
+};
+
+size_t sort_hist_entries(uint32_t n, int max_digit_bits);
+size_t scan_partials_needed(size_t n);
+int sort_plan(int key_bits, int max_digit_bits, int digit_bits[8]);
+
+// Stable sort of (keys_in[i], i) by the low key_bits bits of the key.  The
+// result lands in one of the scratch ping-pong buffers (returned pointers).
+hipError_t radix_sort_pairs(const uint32_t* keys_in, uint32_t n, int key_bits, int max_digit_bits,
+                            SortScratch& s, uint32_t** keys_out, uint32_t** ids_out, hipStream_t st);
+
+// out[i] = sum(in[0..i)), in == out allowed.
+hipError_t scan_exclusive_u32(const uint32_t* in, uint32_t* out, size_t n, uint32_t* partial, hipStream_t st);
+
+// Device buffers of one solver handle (particle-id order unless "sorted").
+struct DeviceState {
+    uint32_t n = 0, hash_size = 0;
+    float4* pos = nullptr;     // pos.xyz, pad0
+    float4* vel = nullptr;     // vel.xyz, pad2
+    float4* xstar = nullptr;   // predicted position, id order
+    uint32_t* keys = nullptr;  // unsorted keys, id order
+    uint32_t* skeys = nullptr; // sorted keys (points into sort scratch)
+    uint32_t* perm = nullptr;  // sorted slot -> particle id (points into sort scratch)
+    int2* cells = nullptr;     // hash_size (start, end)
+    float4* P = nullptr;       // sorted: x* being corrected
+    float4* PL = nullptr;      // sorted: x* and lambda
+    float4* vs = nullptr;      // sorted: post-predict velocity (v*, pad2)
+    float4* curl = nullptr;    // sorted: curl / rho0 of the last Integrate
+    float4* posS = nullptr;    // sorted: committed positions of the last substep
+    float* rho = nullptr;      // sorted: density of the last lambda pass
+    uint32_t* lod = nullptr;   // sorted: APBF level of detail
+    float4* dp_tap = nullptr;  // sorted: last delta-p (taps only)
+    uint32_t* cand = nullptr;  // sorted: candidates of the last lambda pass (taps only)
+    uint32_t* nbr = nullptr;   // sorted: neighbours of the last lambda pass (taps only)
+    uint32_t* flags = nullptr; // n: contact flags / scan
+    SortScratch sort{};
+};
+
+hipError_t launch_predict(const StepConst& c, DeviceState& d, hipStream_t st);
+hipError_t launch_cells(const StepConst& c, DeviceState& d, hipStream_t st);
+hipError_t launch_reorder(const StepConst& c, DeviceState& d, hipStream_t st);
+hipError_t launch_lod(const StepConst& c, DeviceState& d, hipStream_t st);
+hipError_t launch_lambda(const StepConst& c, DeviceState& d, uint32_t iter, bool taps, hipStream_t st);
+hipError_t launch_deltap(const StepConst& c, DeviceState& d, uint32_t iter, bool taps, hipStream_t st);
+hipError_t launch_integrate(const StepConst& c, DeviceState& d, hipStream_t st);
+hipError_t launch_vorticity(const StepConst& c, DeviceState& d, hipStream_t st);
+hipError_t launch_contacts(const StepConst& c, DeviceState& d, float band, float4* out, uint32_t* count,
+                           hipStream_t st);
+
+}  // namespace fsi

@@ -1,0 +1,380 @@
+"""ctypes binding of the gfx950 PBF substep library (include/fsi_pbf.h).
+
+The binding mirrors the reference's ``PBFluids`` solver interface
+(include/fluid/PBFluids.h:88-268): ``Solver`` has ``set_params``,
+``set_particles``, ``set_bounds``, ``set_sdf``, ``step``, ``read_particles``,
+``filter_contacts`` / ``read_contacts`` and the inspection taps.  Particle
+arrays are ``(n, 12)`` float32 numpy arrays in the reference's 48-byte record
+layout (include/fluid/Particle.h:8-12): pos.xyz, pad0, predPos.xyz, pad1,
+vel.xyz, pad2.
+
+The library must be built (``make -C realtime-fsi-engine_amd``); loading fails
+loudly otherwise.  There is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field, fields
+from typing import Optional
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libfsi_pbf.so")
+
+FSI_OK = 0
+PASS_NAMES = ["predict", "sort", "tables", "lambda", "deltap", "integrate", "vorticity", "lod"]
+
+
+class FsiError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"fsi error {code}: {msg}")
+        self.code = code
+
+
+class CFluidConfig(C.Structure):
+    _fields_ = [
+        ("dt", C.c_float), ("gravity", C.c_float * 3), ("h", C.c_float), ("rho0", C.c_float),
+        ("solver_iterations", C.c_uint32), ("substep_iterations", C.c_uint32), ("eps", C.c_float),
+        ("particle_count", C.c_uint32), ("spawn_min", C.c_float * 3), ("spawn_max", C.c_float * 3),
+        ("spawn_random", C.c_uint32), ("spacing", C.c_float), ("initial_velocity", C.c_float * 3),
+        ("bounds_min", C.c_float * 3), ("bounds_max", C.c_float * 3), ("bound_damping", C.c_float),
+        ("hash_size", C.c_uint32), ("enable_scorr", C.c_uint32), ("k_corr", C.c_float), ("n_corr", C.c_float),
+        ("delta_q", C.c_float), ("cohesion_strength", C.c_float), ("enable_viscosity", C.c_uint32),
+        ("viscosity", C.c_float), ("interaction_radius", C.c_float), ("interaction_strength", C.c_float),
+        ("enable_vorticity", C.c_uint32), ("vorticity_epsilon", C.c_float), ("enable_apbf", C.c_uint32),
+        ("min_lod", C.c_uint32), ("max_lod", C.c_uint32), ("lod_max_dist", C.c_float),
+    ]
+
+
+class CConstants(C.Structure):
+    _fields_ = [
+        ("bounds_min", C.c_float * 4), ("bounds_max", C.c_float * 4), ("gravity_dt", C.c_float * 4),
+        ("h", C.c_float), ("rho0", C.c_float), ("eps", C.c_float), ("wq", C.c_float),
+        ("k_corr", C.c_float), ("n_corr", C.c_float), ("viscosity", C.c_float), ("bound_damping", C.c_float),
+        ("hash_size", C.c_uint32), ("particle_count", C.c_uint32), ("enable_scorr", C.c_uint32),
+        ("enable_viscosity", C.c_uint32), ("cohesion_strength", C.c_float), ("interaction_radius", C.c_float),
+        ("interaction_strength", C.c_float), ("w0_self", C.c_float), ("hash_mask", C.c_uint32),
+        ("poly6_coeff", C.c_float), ("spiky_coeff", C.c_float), ("inv_rho0", C.c_float),
+        ("min_lod", C.c_uint32), ("max_lod", C.c_uint32), ("lod_max_dist", C.c_float),
+        ("enable_apbf", C.c_uint32),
+    ]
+
+
+class CTiming(C.Structure):
+    _fields_ = [("ms", C.c_double * 8), ("launches", C.c_uint64 * 8), ("substeps", C.c_uint64),
+                ("frames", C.c_uint64)]
+
+
+@dataclass
+class FluidConfig:
+    """The reference's FluidConfig (include/core/config.h:9-63), same order and defaults."""
+    dt: float = 1.0 / 60.0
+    gravity: tuple = (0.0, -3.81, 0.0)
+    h: float = 0.15
+    rho0: float = 1200.0
+    solver_iterations: int = 3
+    substep_iterations: int = 2
+    eps: float = 0.0001
+    particle_count: int = 1 << 12
+    spawn_min: tuple = (2.0, 0.2, 2.0)
+    spawn_max: tuple = (5.0, 5.0, 5.0)
+    spawn_random: bool = False
+    spacing: float = 0.1
+    initial_velocity: tuple = (0.0, 0.0, 0.0)
+    bounds_min: tuple = (0.0, 0.0, 0.0)
+    bounds_max: tuple = (6.0, 6.0, 6.0)
+    bound_damping: float = 0.5
+    hash_size: int = 1 << 17
+    enable_scorr: bool = True
+    k_corr: float = 0.001
+    n_corr: float = 4.0
+    delta_q: float = 0.3
+    cohesion_strength: float = 0.001
+    enable_viscosity: bool = True
+    viscosity: float = 0.1
+    interaction_radius: float = 1.5
+    interaction_strength: float = 25.0
+    enable_vorticity: bool = True
+    vorticity_epsilon: float = 0.05
+    enable_apbf: bool = False
+    min_lod: int = 2
+    max_lod: int = 7
+    lod_max_dist: float = 15.0
+
+    def to_c(self) -> CFluidConfig:
+        c = CFluidConfig()
+        for f in fields(self):
+            v = getattr(self, f.name)
+            if isinstance(v, (tuple, list)):
+                arr = getattr(c, f.name)
+                for i, x in enumerate(v):
+                    arr[i] = float(np.float32(x))
+            elif isinstance(v, bool):
+                setattr(c, f.name, 1 if v else 0)
+            else:
+                setattr(c, f.name, v)
+        return c
+
+    def f32(self, name: str) -> float:
+        """The FP32 value the library sees for a float field."""
+        return float(np.float32(getattr(self, name)))
+
+
+_lib: Optional[C.CDLL] = None
+
+
+def load() -> C.CDLL:
+    """Load lib/libfsi_pbf.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FileNotFoundError(
+            f"{LIB_PATH} is missing: build it with `make -C {HERE}` (or __graft_entry__.build())")
+    lib = C.CDLL(LIB_PATH)
+    P = C.c_void_p
+    u32p = C.POINTER(C.c_uint32)
+    i32p = C.POINTER(C.c_int32)
+    f32p = C.POINTER(C.c_float)
+    f64p = C.POINTER(C.c_double)
+    sig = {
+        "fsi_last_error": ([], C.c_char_p),
+        "fsi_abi_version": ([], C.c_int),
+        "fsi_fluid_config_default": ([C.POINTER(CFluidConfig)], C.c_int),
+        "fsi_device_count": ([C.POINTER(C.c_int)], C.c_int),
+        "fsi_pbf_create": ([C.POINTER(CFluidConfig), C.POINTER(P)], C.c_int),
+        "fsi_pbf_destroy": ([P], C.c_int),
+        "fsi_pbf_set_params": ([P, C.POINTER(CFluidConfig)], C.c_int),
+        "fsi_pbf_get_params": ([P, C.POINTER(CFluidConfig)], C.c_int),
+        "fsi_pbf_get_constants": ([P, C.POINTER(CConstants)], C.c_int),
+        "fsi_pbf_set_particles": ([P, P, C.c_uint32], C.c_int),
+        "fsi_pbf_particle_count": ([P, u32p], C.c_int),
+        "fsi_pbf_read_particles": ([P, P, C.c_uint32, u32p], C.c_int),
+        "fsi_pbf_set_collision_padding": ([P, C.c_float], C.c_int),
+        "fsi_pbf_set_bounds": ([P, f32p, f32p, C.c_float], C.c_int),
+        "fsi_pbf_set_sdf": ([P, f32p, i32p, f32p, C.c_float], C.c_int),
+        "fsi_pbf_set_sdf_padding": ([P, C.c_float], C.c_int),
+        "fsi_pbf_set_sdf_containment": ([P, C.c_int], C.c_int),
+        "fsi_pbf_set_interaction": ([P, C.c_int, f32p], C.c_int),
+        "fsi_pbf_set_camera": ([P, f32p], C.c_int),
+        "fsi_pbf_step": ([P], C.c_int),
+        "fsi_pbf_sync": ([P], C.c_int),
+        "fsi_pbf_filter_contacts": ([P, C.c_float], C.c_int),
+        "fsi_pbf_read_contacts": ([P, P, C.c_uint32, u32p], C.c_int),
+        "fsi_pbf_read_grid": ([P, u32p, u32p, i32p], C.c_int),
+        "fsi_pbf_read_solver": ([P, f32p, f32p], C.c_int),
+        "fsi_pbf_set_taps": ([P, C.c_int], C.c_int),
+        "fsi_pbf_read_counts": ([P, u32p, u32p], C.c_int),
+        "fsi_pbf_set_timing": ([P, C.c_int], C.c_int),
+        "fsi_pbf_timing_reset": ([P], C.c_int),
+        "fsi_pbf_read_timing": ([P, C.POINTER(CTiming)], C.c_int),
+        "fsi_pbf_get_stream": ([P, C.POINTER(P)], C.c_int),
+        "fsi_scene_lattice": ([C.c_uint32, C.c_uint32, C.c_uint32, f32p, C.c_float, C.c_float, C.c_uint32, f32p, P],
+                              C.c_int),
+        "fsi_mesh_icosphere": ([C.c_uint32, f64p, C.c_double, f64p, i32p], C.c_int),
+        "fsi_sdf_build": ([f64p, C.c_uint32, i32p, C.c_uint32, C.c_double, f64p, i32p, f32p], C.c_int),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = lib
+    return lib
+
+
+def _check(rc: int):
+    if rc != FSI_OK:
+        raise FsiError(rc, load().fsi_last_error().decode())
+
+
+def _ptr(a: np.ndarray, ctype):
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+def _f3(v) -> C.Array:
+    return (C.c_float * 3)(*[float(np.float32(x)) for x in v])
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    _check(load().fsi_device_count(C.byref(n)))
+    return n.value
+
+
+# ---------------------------------------------------------------- scenes
+
+def lattice(nx: int, ny: int, nz: int, origin, spacing: float, jitter: float, seed: int = 1337,
+            v0=(0.0, 0.0, 0.0)) -> np.ndarray:
+    """Lattice block of nx*ny*nz particles (ids x-fastest) with seeded jitter."""
+    out = np.zeros((nx * ny * nz, 12), dtype=np.float32)
+    _check(load().fsi_scene_lattice(nx, ny, nz, _f3(origin), float(np.float32(spacing)), float(np.float32(jitter)),
+                                    seed, _f3(v0), out.ctypes.data_as(C.c_void_p)))
+    return out
+
+
+def icosphere(subdivisions: int, centre, radius: float):
+    V = 10 * 4 ** subdivisions + 2
+    F = 20 * 4 ** subdivisions
+    verts = np.zeros((V, 3), dtype=np.float64)
+    faces = np.zeros((F, 3), dtype=np.int32)
+    c = (C.c_double * 3)(*centre)
+    _check(load().fsi_mesh_icosphere(subdivisions, c, float(radius), _ptr(verts, C.c_double), _ptr(faces, C.c_int32)))
+    return verts, faces
+
+
+@dataclass
+class SdfGrid:
+    grid: np.ndarray      # (nz, ny, nx) float32, x fastest in memory
+    origin: np.ndarray    # (3,) float64
+    cell: float
+    dims: tuple           # (nx, ny, nz)
+
+
+def sdf_build(verts: np.ndarray, faces: np.ndarray, cell: float) -> SdfGrid:
+    verts = np.ascontiguousarray(verts, dtype=np.float64)
+    faces = np.ascontiguousarray(faces, dtype=np.int32)
+    lib = load()
+    origin = np.zeros(3, dtype=np.float64)
+    dims = np.zeros(3, dtype=np.int32)
+    args = (_ptr(verts, C.c_double), len(verts), _ptr(faces, C.c_int32), len(faces), float(cell),
+            _ptr(origin, C.c_double), _ptr(dims, C.c_int32))
+    _check(lib.fsi_sdf_build(*args, None))
+    grid = np.zeros((int(dims[2]), int(dims[1]), int(dims[0])), dtype=np.float32)
+    _check(lib.fsi_sdf_build(*args, _ptr(grid, C.c_float)))
+    return SdfGrid(grid=grid, origin=origin, cell=float(cell), dims=(int(dims[0]), int(dims[1]), int(dims[2])))
+
+
+# ---------------------------------------------------------------- solver
+
+class Solver:
+    """One solver handle (the reference's PBFluids)."""
+
+    def __init__(self, cfg: FluidConfig):
+        self.lib = load()
+        self.cfg = cfg
+        self._c = cfg.to_c()
+        h = C.c_void_p()
+        _check(self.lib.fsi_pbf_create(C.byref(self._c), C.byref(h)))
+        self.h = h
+        self.n = 0
+
+    def close(self):
+        if self.h:
+            self.lib.fsi_pbf_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_params(self, cfg: FluidConfig):
+        self.cfg = cfg
+        self._c = cfg.to_c()
+        _check(self.lib.fsi_pbf_set_params(self.h, C.byref(self._c)))
+
+    def constants(self) -> CConstants:
+        k = CConstants()
+        _check(self.lib.fsi_pbf_get_constants(self.h, C.byref(k)))
+        return k
+
+    def set_particles(self, particles: np.ndarray):
+        p = np.ascontiguousarray(particles, dtype=np.float32).reshape(-1, 12)
+        self.n = p.shape[0]
+        _check(self.lib.fsi_pbf_set_particles(self.h, p.ctypes.data_as(C.c_void_p), self.n))
+
+    def read_particles(self) -> np.ndarray:
+        out = np.zeros((self.n, 12), dtype=np.float32)
+        n = C.c_uint32(0)
+        _check(self.lib.fsi_pbf_read_particles(self.h, out.ctypes.data_as(C.c_void_p), self.n, C.byref(n)))
+        return out
+
+    def set_collision_padding(self, p: float):
+        _check(self.lib.fsi_pbf_set_collision_padding(self.h, float(p)))
+
+    def set_bounds(self, mn, mx, damping: float = 0.0):
+        _check(self.lib.fsi_pbf_set_bounds(self.h, _f3(mn), _f3(mx), float(np.float32(damping))))
+
+    def set_sdf(self, sdf: Optional[SdfGrid], padding: float = 0.0, containment: bool = False):
+        if sdf is None:
+            _check(self.lib.fsi_pbf_set_sdf(self.h, None, None, None, 0.0))
+            return
+        g = np.ascontiguousarray(sdf.grid, dtype=np.float32)
+        dims = (C.c_int32 * 3)(*sdf.dims)
+        _check(self.lib.fsi_pbf_set_sdf(self.h, _ptr(g, C.c_float), dims, _f3(sdf.origin),
+                                        float(np.float32(sdf.cell))))
+        _check(self.lib.fsi_pbf_set_sdf_padding(self.h, float(np.float32(padding))))
+        _check(self.lib.fsi_pbf_set_sdf_containment(self.h, 1 if containment else 0))
+
+    def set_interaction(self, active: bool, pos=(0.0, 0.0, 0.0)):
+        _check(self.lib.fsi_pbf_set_interaction(self.h, 1 if active else 0, _f3(pos)))
+
+    def set_camera(self, pos):
+        _check(self.lib.fsi_pbf_set_camera(self.h, _f3(pos)))
+
+    def step(self, frames: int = 1):
+        for _ in range(frames):
+            _check(self.lib.fsi_pbf_step(self.h))
+
+    def sync(self):
+        _check(self.lib.fsi_pbf_sync(self.h))
+
+    def filter_contacts(self, band: float):
+        _check(self.lib.fsi_pbf_filter_contacts(self.h, float(np.float32(band))))
+
+    def read_contacts(self) -> np.ndarray:
+        count = C.c_uint32(0)
+        _check(self.lib.fsi_pbf_read_contacts(self.h, None, 0, C.byref(count)))
+        out = np.zeros((count.value, 8), dtype=np.float32)
+        if count.value:
+            _check(self.lib.fsi_pbf_read_contacts(self.h, out.ctypes.data_as(C.c_void_p), count.value,
+                                                  C.byref(count)))
+        return out
+
+    def read_grid(self):
+        keys = np.zeros(self.n, dtype=np.uint32)
+        ids = np.zeros(self.n, dtype=np.uint32)
+        cells = np.zeros((self.cfg.hash_size, 2), dtype=np.int32)
+        _check(self.lib.fsi_pbf_read_grid(self.h, _ptr(keys, C.c_uint32), _ptr(ids, C.c_uint32),
+                                          _ptr(cells, C.c_int32)))
+        return keys, ids, cells
+
+    def read_solver(self):
+        a = np.zeros((self.n, 4), dtype=np.float32)
+        b = np.zeros((self.n, 4), dtype=np.float32)
+        _check(self.lib.fsi_pbf_read_solver(self.h, _ptr(a, C.c_float), _ptr(b, C.c_float)))
+        return a, b
+
+    def set_taps(self, on: bool):
+        _check(self.lib.fsi_pbf_set_taps(self.h, 1 if on else 0))
+
+    def read_counts(self):
+        c = np.zeros(self.n, dtype=np.uint32)
+        nb = np.zeros(self.n, dtype=np.uint32)
+        _check(self.lib.fsi_pbf_read_counts(self.h, _ptr(c, C.c_uint32), _ptr(nb, C.c_uint32)))
+        return c, nb
+
+    def set_timing(self, on: bool):
+        _check(self.lib.fsi_pbf_set_timing(self.h, 1 if on else 0))
+
+    def timing_reset(self):
+        _check(self.lib.fsi_pbf_timing_reset(self.h))
+
+    def read_timing(self) -> dict:
+        t = CTiming()
+        _check(self.lib.fsi_pbf_read_timing(self.h, C.byref(t)))
+        return {
+            "ms": {PASS_NAMES[i]: t.ms[i] for i in range(8)},
+            "launches": {PASS_NAMES[i]: int(t.launches[i]) for i in range(8)},
+            "substeps": int(t.substeps),
+            "frames": int(t.frames),
+        }

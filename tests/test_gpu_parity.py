@@ -1,0 +1,264 @@
+"""Parity of the gfx950 substep against the CPU oracle, through the C-ABI.
+
+Bars (both sides run the same seeded inputs):
+  * packed constants, sorted hash keys, the sorted permutation, the cell
+    start/end table and the per-particle candidate / neighbour counts are
+    bit-exact;
+  * positions after one substep agree within 1e-5*h (the kernels keep the
+    oracle's FP32 operation order, so they are expected to be equal);
+  * velocities within 1e-5*h / dt, the same bound expressed per unit time.
+"""
+import numpy as np
+import pytest
+
+from conftest import dam_break_config
+
+pytestmark = pytest.mark.gpu
+
+
+def _fsi():
+    import fsi_pbf
+
+    return fsi_pbf
+
+
+def _oracle():
+    import pyoracle
+
+    return pyoracle
+
+
+def run_pair(cfg, parts, frames=1, taps=True, sdf=None, sdf_padding=0.0, containment=False, interaction=None,
+             camera=(0.0, 0.0, 0.0), collision_padding=0.0):
+    fsi = _fsi()
+    s = fsi.Solver(cfg)
+    if collision_padding:
+        s.set_collision_padding(collision_padding)
+        s.set_params(cfg)
+    s.set_taps(taps)
+    s.set_particles(parts)
+    if sdf is not None:
+        s.set_sdf(sdf, sdf_padding, containment)
+    if interaction is not None:
+        s.set_interaction(True, interaction)
+    s.set_camera(camera)
+    s.step(frames)
+    ref = _oracle().OracleSim(cfg, parts, taps=taps, sdf=sdf, sdf_padding=sdf_padding, containment=containment,
+                              interaction=interaction, camera=camera, collision_padding=collision_padding)
+    ref.step(frames)
+    return s, ref
+
+
+def assert_state_close(s, ref, cfg, frames=1, label=""):
+    got = s.read_particles()
+    exp = ref.particles()
+    dx = float(np.abs(got[:, 0:3] - exp[:, 0:3]).max()) if len(got) else 0.0
+    dv = float(np.abs(got[:, 8:11] - exp[:, 8:11]).max()) if len(got) else 0.0
+    tol_x = 1e-5 * cfg.h
+    tol_v = tol_x / (cfg.dt / cfg.substep_iterations)
+    print(f"{label} n={len(got)} frames={frames}: max|dx|={dx:.3e} (tol {tol_x:.1e}) max|dv|={dv:.3e} "
+          f"exact_x={np.array_equal(got[:, 0:3], exp[:, 0:3])}")
+    assert dx <= tol_x
+    assert dv <= tol_v
+    # pads and predPos are carried through untouched
+    assert np.array_equal(got[:, 3], exp[:, 3]) and np.array_equal(got[:, 4:8], exp[:, 4:8])
+    assert np.array_equal(got[:, 11], exp[:, 11])
+
+
+def assert_grid_equal(s, ref):
+    keys, ids, cells = s.read_grid()
+    assert np.array_equal(keys, ref.keys), "sorted keys"
+    assert np.array_equal(ids, ref.ids), "sorted permutation"
+    assert np.array_equal(cells, ref.cells), "cell table"
+
+
+@pytest.mark.parametrize("case", ["default", "dam", "apbf"])
+def test_constants_bit_exact(gpu, case):
+    fsi = _fsi()
+    if case == "default":
+        cfg = fsi.FluidConfig()
+    elif case == "dam":
+        cfg = dam_break_config()
+    else:
+        cfg = dam_break_config(h=0.137, rho0=999.0, enable_apbf=True, delta_q=0.2, substep_iterations=3)
+    parts = fsi.lattice(4, 4, 4, (1.0, 1.0, 1.0), cfg.h / 2, 0.0)
+    with fsi.Solver(cfg) as s:
+        s.set_collision_padding(0.01)
+        s.set_params(cfg)
+        s.set_particles(parts)
+        k = s.constants()
+    u = _oracle().pack_ubo(cfg, len(parts), "f32", collision_padding=0.01)
+    f = np.float32
+    pairs = [(k.h, u.h), (k.rho0, u.rho0), (k.eps, u.eps), (k.wq, u.wq), (k.w0_self, u.w0_self),
+             (k.poly6_coeff, u.poly6_coeff), (k.spiky_coeff, u.spiky_coeff), (k.inv_rho0, u.inv_rho0),
+             (k.gravity_dt[3], u.sub_dt), (k.bound_damping, u.bound_damping)]
+    pairs += [(k.bounds_min[i], u.bounds_min[i]) for i in range(3)] + [(k.bounds_max[i], u.bounds_max[i]) for i in range(3)]
+    for a, b in pairs:
+        assert f(a).tobytes() == f(b).tobytes(), (a, b)
+    assert k.hash_mask == u.hash_mask and k.min_lod == u.min_lod and k.max_lod == u.max_lod
+    assert k.particle_count == len(parts)
+
+
+SCENES = {
+    # name: (lattice dims, hash size, iterations)
+    "c1_block_32k": ((32, 32, 32), 1 << 17, 4),
+    "ragged_4999": ((17, 21, 14), 1 << 12, 3),
+    "tiny_hash": ((10, 10, 10), 1 << 3, 2),
+    "big_hash": ((24, 20, 16), 1 << 20, 2),
+}
+
+
+@pytest.mark.parametrize("name", list(SCENES))
+def test_one_substep_grid_counts_positions(gpu, name):
+    fsi = _fsi()
+    dims, H, iters = SCENES[name]
+    cfg = dam_break_config(n_iter=iters, substeps=1, hash_size=H)
+    parts = fsi.lattice(*dims, (0.0, 0.0, 0.0), 0.05, 0.001, 1337)
+    if name == "ragged_4999":
+        parts = parts[:4999].copy()
+    s, ref = run_pair(cfg, parts)
+    assert_grid_equal(s, ref)
+    cand, nbr = s.read_counts()
+    assert np.array_equal(cand, ref.cand), "candidate counts"
+    assert np.array_equal(nbr, ref.nbr), "neighbour counts"
+    ppl, dpr = s.read_solver()
+    assert np.abs(ppl[:, 0:3] - ref.solver[:, 0:3]).max() <= 1e-5 * cfg.h
+    assert np.all(ppl[:, 3] == 0.0) and np.all(ref.solver[:, 3] == 0.0)
+    rel = np.abs(dpr[:, 3] - ref.solver[:, 7]).max() / np.abs(ref.solver[:, 7]).max()
+    assert rel <= 1e-6, rel
+    assert_state_close(s, ref, cfg, label=name)
+    s.close()
+
+
+def test_full_step_two_substeps_vorticity(gpu):
+    fsi = _fsi()
+    cfg = dam_break_config(n_iter=4, substeps=2)
+    parts = fsi.lattice(32, 32, 32, (0.0, 0.0, 0.0), 0.05, 0.001, 1337)
+    s, ref = run_pair(cfg, parts, frames=1)
+    assert_grid_equal(s, ref)
+    assert_state_close(s, ref, cfg, label="c1 one frame")
+    _, dpr = s.read_solver()
+    assert np.abs(dpr[:, 0:3] - ref.solver[:, 4:7]).max() <= 1e-5 * max(1.0, np.abs(ref.solver[:, 4:7]).max())
+    s.close()
+
+
+def test_c1_ten_frames(gpu):
+    """The 32,768-particle dam break over ten frames (I=4, S=2)."""
+    fsi = _fsi()
+    cfg = dam_break_config(n_iter=4, substeps=2)
+    parts = fsi.lattice(32, 32, 32, (0.0, 0.0, 0.0), 0.05, 0.001, 1337)
+    s, ref = run_pair(cfg, parts, frames=10, taps=False)
+    assert_state_close(s, ref, cfg, frames=10, label="c1 ten frames")
+    s.close()
+
+
+def test_sdf_obstacle_and_container(gpu):
+    fsi = _fsi()
+    cfg = dam_break_config(n_iter=3, substeps=2, box=(3.2, 3.2, 3.2))
+    verts, faces = fsi.icosphere(3, (1.6, 0.9, 1.6), 0.6)
+    sdf = fsi.sdf_build(verts, faces, 0.05)
+    parts = fsi.lattice(40, 30, 40, (0.6, 0.5, 0.6), 0.05, 0.001, 7)
+    for containment in (False, True):
+        s, ref = run_pair(cfg, parts, frames=2, sdf=sdf, sdf_padding=0.025, containment=containment)
+        assert_grid_equal(s, ref)
+        assert_state_close(s, ref, cfg, frames=2, label=f"sdf containment={containment}")
+        # contact filter: same candidates, same (id) order
+        s.filter_contacts(0.1)
+        got = s.read_contacts()
+        exp = ref.contacts(0.1)
+        assert got.shape == exp.shape and got.shape[0] > 0
+        assert np.array_equal(got, exp)
+        s.close()
+
+
+def test_apbf_interaction_padding(gpu):
+    fsi = _fsi()
+    cfg = dam_break_config(n_iter=3, substeps=2, enable_apbf=True, min_lod=1, max_lod=5, lod_max_dist=2.0)
+    parts = fsi.lattice(20, 20, 20, (0.5, 0.5, 0.5), 0.05, 0.001, 11)
+    s, ref = run_pair(cfg, parts, frames=2, camera=(0.5, 0.5, 0.5), interaction=(1.0, 1.2, 1.0),
+                      collision_padding=0.02)
+    assert_grid_equal(s, ref)
+    assert_state_close(s, ref, cfg, frames=2, label="apbf+interaction")
+    s.close()
+
+
+def test_walls_and_damping(gpu):
+    """Particles placed exactly on the walls hit the <= / >= clamps."""
+    fsi = _fsi()
+    cfg = dam_break_config(n_iter=2, substeps=1, box=(1.0, 1.0, 1.0), bound_damping=0.7)
+    parts = np.zeros((6, 12), dtype=np.float32)
+    parts[:, 0:3] = [[0.0, 0.5, 0.5], [1.0, 0.5, 0.5], [0.5, 0.0, 0.5], [0.5, 1.0, 0.5], [0.5, 0.5, 0.0],
+                     [0.5, 0.5, 1.0]]
+    parts[:, 8:11] = [[-1, 0, 0], [1, 0, 0], [0, -1, 0], [0, 1, 0], [0, 0, -1], [0, 0, 1]]
+    parts[:, 4:7] = parts[:, 0:3]
+    parts[:, 3] = 7.0  # pads survive
+    parts[:, 11] = -3.0
+    s, ref = run_pair(cfg, parts, frames=3)
+    assert_state_close(s, ref, cfg, frames=3, label="walls")
+    s.close()
+
+
+def test_empty_single_and_reads_before_step(gpu):
+    fsi = _fsi()
+    cfg = dam_break_config()
+    with fsi.Solver(cfg) as s:
+        s.set_particles(np.zeros((0, 12), dtype=np.float32))
+        s.step(2)
+        assert s.read_particles().shape == (0, 12)
+        one = fsi.lattice(1, 1, 1, (1.0, 1.0, 1.0), 0.05, 0.0)
+        s.set_particles(one)
+        keys, ids, cells = s.read_grid()
+        assert np.all(cells == -1)
+        s.step(1)
+    s, ref = run_pair(cfg, one, frames=3)
+    assert_state_close(s, ref, cfg, frames=3, label="single")
+    s.close()
+
+
+def test_error_paths(gpu):
+    fsi = _fsi()
+    cfg = dam_break_config()
+    with fsi.Solver(cfg) as s:
+        bad = dam_break_config(hash_size=1000)
+        with pytest.raises(fsi.FsiError) as e:
+            s.set_params(bad)
+        assert e.value.code == -1
+        parts = fsi.lattice(4, 4, 4, (1.0, 1.0, 1.0), 0.05, 0.0)
+        s.set_particles(parts)
+        s.step(1)
+        with pytest.raises(fsi.FsiError) as e:  # taps are off
+            s.read_counts()
+        assert e.value.code == -3
+        # a changed hash size reallocates the cell table
+        s.set_params(dam_break_config(hash_size=1 << 10))
+        s.step(1)
+        keys, ids, cells = s.read_grid()
+        assert cells.shape == (1 << 10, 2)
+
+
+def test_full_size_properties(gpu):
+    """1,048,576 particles (128x64x128 lattice): one frame, then size-independent
+    properties of the sort and the cell table, and the grid against the oracle's
+    stable sort of the same keys."""
+    fsi = _fsi()
+    cfg = dam_break_config(n_iter=6, substeps=2, box=(12.8, 6.4, 12.8), hash_size=1 << 20)
+    parts = fsi.lattice(128, 64, 128, (0.0, 0.0, 0.0), 0.05, 0.001, 1337)
+    with fsi.Solver(cfg) as s:
+        s.set_particles(parts)
+        s.step(1)
+        keys, ids, cells = s.read_grid()
+        got = s.read_particles()
+    n = len(parts)
+    assert np.all(np.diff(keys.astype(np.int64)) >= 0), "keys sorted"
+    assert np.array_equal(np.sort(ids), np.arange(n, dtype=np.uint32)), "ids form a permutation"
+    # stable: equal keys keep ascending ids
+    same = np.diff(keys.astype(np.int64)) == 0
+    assert np.all(np.diff(ids.astype(np.int64))[same] > 0)
+    occ = cells[:, 0] >= 0
+    assert np.array_equal(occ, cells[:, 1] >= 0)
+    assert int((cells[occ, 1] - cells[occ, 0]).sum()) == n
+    assert np.array_equal(np.unique(keys), np.nonzero(occ)[0])
+    assert np.all(np.isfinite(got[:, 0:3]))
+    # the keys are those of the committed predicted positions: re-sort them stably
+    order = np.argsort(keys[np.argsort(ids)], kind="stable")
+    assert np.array_equal(order.astype(np.uint32), ids)
