@@ -261,6 +261,18 @@ int fsi_pbf_set_timing(fsi_pbf* h, int enable);
 int fsi_pbf_timing_reset(fsi_pbf* h);
 int fsi_pbf_read_timing(fsi_pbf* h, fsi_pbf_timing* out);
 
+/* Launch-shape knobs picked by the autotuner (no counterpart in the
+ * reference, whose shaders hard-code local_size_x = 256 and four 8-bit radix
+ * passes, src/fluid/PBFluids.cpp:330,355-361).  Every setting computes the
+ * same results; only speed differs. */
+enum fsi_pbf_knob {
+    FSI_TUNE_GATHER = 0,          /* neighbour-gather form: 0 single loop, 1 two-phase list  */
+    FSI_TUNE_SORT_DIGIT_BITS = 1, /* widest radix digit, 1..10                                */
+    FSI_TUNE_COUNT = 2
+};
+int fsi_pbf_set_tuning(fsi_pbf* h, int knob, int value);
+int fsi_pbf_get_tuning(fsi_pbf* h, int knob, int* value);
+
 /* Device pointer of the HIP stream the handle enqueues on (hipStream_t). */
 int fsi_pbf_get_stream(fsi_pbf* h, void** stream);
 

@@ -94,6 +94,7 @@ struct fsi_pbf {
     std::vector<float> host_pred;  // predPos.xyz + pad1 per particle (never touched on device)
     bool stepped = false;
     bool taps = false;
+    int sort_digit_bits = kSortMaxDigitBits;
     // contacts
     float4* contacts = nullptr;
     uint32_t* contact_count = nullptr;
@@ -218,6 +219,22 @@ StepConst step_const(const fsi_pbf* h) {
     c.sdf_cell = h->sdf_cell;
     c.sdf_pad = h->sdf_padding;
     c.sdf_grid = h->sdf_grid;
+    // Morton binning of the processing order: cover the bounds (plus two
+    // cells) with at most 128 bins per axis
+    auto cell_clamped = [&](float x) {
+        const double v = std::floor((double)x / (double)u.h);
+        return (int64_t)std::max(-1.0e9, std::min(1.0e9, std::isfinite(v) ? v : 0.0));
+    };
+    int64_t span = 1;
+    for (int a = 0; a < 3; ++a) {
+        const int64_t lo = cell_clamped(u.bounds_min[a]) - 2;
+        const int64_t hi = cell_clamped(u.bounds_max[a]) + 2;
+        c.dcmin[a] = (int32_t)lo;
+        span = std::max<int64_t>(span, hi - lo + 1);
+    }
+    uint32_t shift = 0;
+    while ((span >> shift) > 128 && shift < 31) ++shift;
+    c.dshift = shift;
     return c;
 }
 
@@ -265,13 +282,20 @@ hipError_t alloc_particles(fsi_pbf* h, uint32_t n) {
     ALLOC(d.cand, n);
     ALLOC(d.nbr, n);
     ALLOC(d.flags, n);
+    ALLOC(d.bins, kOrderBins);
+    ALLOC(d.bin_of, n);
+    ALLOC(d.bin_rank, n);
+    ALLOC(d.order, n);
+    ALLOC(d.list_count, n);
+    ALLOC(d.list, (size_t)kNbrCap * n);
     ALLOC(d.sort.keys[0], n);
     ALLOC(d.sort.keys[1], n);
     ALLOC(d.sort.ids[0], n);
     ALLOC(d.sort.ids[1], n);
     const size_t hist = sort_hist_entries(n, kSortMaxDigitBits);
     ALLOC(d.sort.hist, hist);
-    const size_t parts = std::max(scan_partials_needed(hist), scan_partials_needed(n));
+    const size_t parts = std::max({scan_partials_needed(hist), scan_partials_needed(n),
+                                   scan_partials_needed(kOrderBins)});
     ALLOC(d.sort.partial, parts);
     ALLOC(h->contacts, 2 * (size_t)n);
     ALLOC(h->contact_count, 1);
@@ -298,6 +322,12 @@ void free_all(fsi_pbf* h) {
     dev_free(&d.cand);
     dev_free(&d.nbr);
     dev_free(&d.flags);
+    dev_free(&d.bins);
+    dev_free(&d.bin_of);
+    dev_free(&d.bin_rank);
+    dev_free(&d.order);
+    dev_free(&d.list_count);
+    dev_free(&d.list);
     dev_free(&d.sort.keys[0]);
     dev_free(&d.sort.keys[1]);
     dev_free(&d.sort.ids[0]);
@@ -662,7 +692,7 @@ int fsi_pbf_step(fsi_pbf* h) {
         }
         {
             PassTimer t(h, FSI_PASS_SORT);
-            HIP_TRY(radix_sort_pairs(d.keys, d.n, key_bits, kSortMaxDigitBits, d.sort, &d.skeys, &d.perm, st));
+            HIP_TRY(radix_sort_pairs(d.keys, d.n, key_bits, h->sort_digit_bits, d.sort, &d.skeys, &d.perm, st));
         }
         {
             PassTimer t(h, FSI_PASS_TABLES);
@@ -832,6 +862,31 @@ int fsi_pbf_read_timing(fsi_pbf* h, fsi_pbf_timing* out) {
     if ((rc = drain_timing(h))) return rc;
     *out = h->acc;
     return FSI_OK;
+}
+
+int fsi_pbf_set_tuning(fsi_pbf* h, int knob, int value) {
+    if (!h) return fail(FSI_ERR_INVALID_ARG, "handle is NULL");
+    switch (knob) {
+        case FSI_TUNE_GATHER:
+            if (value < 0 || value > 1) return fail(FSI_ERR_INVALID_ARG, "gather form must be 0 or 1");
+            h->d.variant = value;
+            return FSI_OK;
+        case FSI_TUNE_SORT_DIGIT_BITS:
+            if (value < 1 || value > kSortMaxDigitBits) return fail(FSI_ERR_INVALID_ARG, "digit bits must be 1..10");
+            h->sort_digit_bits = value;
+            return FSI_OK;
+        default:
+            return fail(FSI_ERR_INVALID_ARG, "unknown knob");
+    }
+}
+
+int fsi_pbf_get_tuning(fsi_pbf* h, int knob, int* value) {
+    if (!h || !value) return fail(FSI_ERR_INVALID_ARG, "NULL argument");
+    switch (knob) {
+        case FSI_TUNE_GATHER: *value = h->d.variant; return FSI_OK;
+        case FSI_TUNE_SORT_DIGIT_BITS: *value = h->sort_digit_bits; return FSI_OK;
+        default: return fail(FSI_ERR_INVALID_ARG, "unknown knob");
+    }
 }
 
 int fsi_pbf_get_stream(fsi_pbf* h, void** stream) {

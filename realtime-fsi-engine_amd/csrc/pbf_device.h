@@ -24,6 +24,10 @@ struct StepConst {
     uint32_t min_lod, max_lod, enable_apbf, enable_vort;
     uint32_t interacting;
     float ipos[3], cam[3], vort_eps;
+    // processing order: cells binned into a 128^3 Morton grid of cells shifted
+    // by dshift bits, starting at cell dcmin
+    int32_t dcmin[3];
+    uint32_t dshift;
     // SDF boundary (R32F grid, x fastest, voxel centres at origin + i*cell)
     uint32_t enable_sdf, sdf_mode;
     int32_t dims[3];
@@ -59,6 +63,27 @@ __device__ __forceinline__ void cell_of(V3 x, float h, int32_t& cx, int32_t& cy,
     cy = (int32_t)floorf(x.y / h);
     cz = (int32_t)floorf(x.z / h);
 }
+
+// 21-bit Morton code of a cell, on a 128^3 grid of (cell - dcmin) >> dshift
+// (clamped).  Used only to order the work; results do not depend on it.
+__device__ __forceinline__ uint32_t spread7(uint32_t v) {
+    v &= 0x7fu;
+    v = (v | (v << 8)) & 0x0300f00fu;
+    v = (v | (v << 4)) & 0x030c30c3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+__device__ __forceinline__ uint32_t clamp_bin(int32_t c, int32_t c0, uint32_t shift) {
+    int32_t d = c - c0;
+    d = d < 0 ? 0 : d;
+    d >>= shift;
+    return (uint32_t)(d > 127 ? 127 : d);
+}
+__device__ __forceinline__ uint32_t morton_bin(const StepConst& c, int32_t cx, int32_t cy, int32_t cz) {
+    return spread7(clamp_bin(cx, c.dcmin[0], c.dshift)) | (spread7(clamp_bin(cy, c.dcmin[1], c.dshift)) << 1) |
+           (spread7(clamp_bin(cz, c.dcmin[2], c.dshift)) << 2);
+}
+constexpr uint32_t kOrderBins = 1u << 21;
 
 // Poly6 (CS_ComputeLambdas.glsl:74-81) given r2.
 __device__ __forceinline__ float poly6_r2(const StepConst& c, float r2) {

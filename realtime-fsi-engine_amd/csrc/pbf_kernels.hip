@@ -26,24 +26,137 @@ constexpr int kBlock = 256;
 
 __device__ __forceinline__ uint32_t gid() { return blockIdx.x * kBlock + threadIdx.x; }
 
+// Workgroups are handed to the 8 XCDs of the MI355X round-robin (block b runs
+// on XCD b % 8).  Renumber them so that each XCD walks one contiguous range
+// of the processing order, i.e. one compact region of space whose buckets and
+// cell-table lines stay in that XCD's own L2.
+__device__ __forceinline__ uint32_t xcd_gid() {
+    const uint32_t b = blockIdx.x, nb = gridDim.x;
+    const uint32_t q = nb >> 3, r = nb & 7u, x = b & 7u, l = b >> 3;
+    const uint32_t lb = (x < r) ? x * (q + 1) + l : r * (q + 1) + (x - r) * q + l;
+    return lb * kBlock + threadIdx.x;
+}
+
 inline dim3 grid_for(uint32_t n) { return dim3((n + kBlock - 1) / kBlock); }
 
-// Visit the candidate slots of the 27 buckets around cell (cx,cy,cz), in the
-// order of CS_ComputeLambdas.glsl:126-150, skipping the particle's own slot.
+// Visit the candidates of the 27 buckets around x in the reference order
+// (CS_ComputeLambdas.glsl:126-150: bucket offsets dz, dy, dx with dx fastest,
+// slots in sorted order, the particle's own slot skipped, an aliased bucket
+// visited once per occurrence).  f(k, xk, r, r2) sees every candidate, with
+// r = x - xk.  The next bucket's range is loaded while the current bucket is
+// scanned, and candidates are loaded four at a time (indexes clamped to the
+// bucket, so every load stays inside it): up to five loads are in flight per
+// lane instead of one exposed latency per candidate.
 template <class F>
-__device__ __forceinline__ void for_each_candidate(int32_t cx, int32_t cy, int32_t cz, uint32_t mask,
-                                                   const int2* __restrict__ cells, uint32_t self, F&& f) {
-    for (int dz = -1; dz <= 1; ++dz)
-        for (int dy = -1; dy <= 1; ++dy)
-            for (int dx = -1; dx <= 1; ++dx) {
-                const uint32_t b = cell_hash(cx + dx, cy + dy, cz + dz, mask);
-                const int2 se = cells[b];
-                if (se.x == -1) continue;
-                for (int32_t k = se.x; k < se.y; ++k) {
-                    if ((uint32_t)k == self) continue;
-                    f((uint32_t)k);
-                }
+__device__ __forceinline__ void scan_buckets(V3 x, float h, uint32_t mask, const int2* __restrict__ cells,
+                                             const float4* __restrict__ X, uint32_t self, F&& f) {
+    int32_t cx, cy, cz;
+    cell_of(x, h, cx, cy, cz);
+    int2 se = cells[cell_hash(cx - 1, cy - 1, cz - 1, mask)];
+    auto visit = [&](int32_t k, float4 xk) {
+        if ((uint32_t)k == self) return;
+        const V3 r = sub(x, xyz(xk));
+        f((uint32_t)k, xk, r, dot(r, r));
+    };
+#pragma unroll 1
+    for (int q = 0; q < 27; ++q) {
+        int2 nxt = make_int2(-1, -1);
+        if (q < 26) {
+            const int qn = q + 1;
+            nxt = cells[cell_hash(cx + qn % 3 - 1, cy + (qn / 3) % 3 - 1, cz + qn / 9 - 1, mask)];
+        }
+        // an empty bucket is (-1, -1): no iterations
+        const int32_t last = se.y - 1;
+        for (int32_t k = se.x; k < se.y; k += 4) {
+            const int32_t k1 = min(k + 1, last), k2 = min(k + 2, last), k3 = min(k + 3, last);
+            const float4 a0 = X[k], a1 = X[k1], a2 = X[k2], a3 = X[k3];
+            visit(k, a0);
+            if (k + 1 <= last) visit(k + 1, a1);
+            if (k + 2 <= last) visit(k + 2, a2);
+            if (k + 3 <= last) visit(k + 3, a3);
+        }
+        se = nxt;
+    }
+}
+
+// Phase 1 of a two-phase gather: append, in visit order, the slots of the
+// candidates inside the support (r^2 <= h^2) to this lane's column of an LDS
+// list (col[t * kBlock], at most kNbrCap entries).  Returns the neighbour count.
+template <bool TAPS>
+__device__ __forceinline__ uint32_t collect(V3 xi, float h, uint32_t mask, const int2* __restrict__ cells,
+                                            const float4* __restrict__ X, uint32_t self, uint32_t* col,
+                                            uint32_t& ncand) {
+    const float h2 = h * h;
+    uint32_t cnt = 0;
+    scan_buckets(xi, h, mask, cells, X, self, [&](uint32_t k, float4, V3, float r2) {
+        if (TAPS) ++ncand;
+        if (r2 <= h2) {
+            if (cnt < (uint32_t)kNbrCap) col[cnt * kBlock] = k;
+            ++cnt;
+        }
+    });
+    return cnt;
+}
+
+// Run body over the slots idx(0..cnt) in order, fetching four neighbours
+// ahead: the index loads and then the position loads of a group of four are
+// issued together before the four bodies run.
+template <class Idx, class Body>
+__device__ __forceinline__ void walk_list(uint32_t cnt, const float4* __restrict__ X, Idx&& idx, Body&& body) {
+    uint32_t r = 0;
+#pragma unroll 1
+    for (; r + 4 <= cnt; r += 4) {
+        const uint32_t j0 = idx(r), j1 = idx(r + 1), j2 = idx(r + 2), j3 = idx(r + 3);
+        const float4 a0 = X[j0], a1 = X[j1], a2 = X[j2], a3 = X[j3];
+        body(j0, a0);
+        body(j1, a1);
+        body(j2, a2);
+        body(j3, a3);
+    }
+#pragma unroll 1
+    for (; r < cnt; ++r) {
+        const uint32_t j = idx(r);
+        body(j, X[j]);
+    }
+}
+
+// Neighbour gather, bit-identical to the reference's loop in both forms:
+//   V = 0  single loop: body(j, xj, r, r2) for each candidate in the support.
+//   V = 1  two phases: collect() the neighbours, then run body over the list,
+//          so the lanes of a wave execute the expensive arithmetic (square
+//          root, divisions) together on real neighbours instead of under
+//          divergence on the ~85% of candidates outside the support.  A
+//          particle with more than kNbrCap neighbours re-runs the single loop
+//          (same slots, same order).
+// Pairs with r^2 > h^2 are skipped: in the reference they add exact zeros.
+template <int V, bool TAPS, class Body>
+__device__ __forceinline__ void gather(V3 xi, float h, uint32_t mask, const int2* __restrict__ cells,
+                                       const float4* __restrict__ X, uint32_t self, uint32_t* col, Body&& body,
+                                       uint32_t& ncand, uint32_t& nnbr) {
+    const float h2 = h * h;
+    auto single = [&](bool count) {
+        scan_buckets(xi, h, mask, cells, X, self, [&](uint32_t k, float4 xk, V3 r, float r2) {
+            if (TAPS && count) ++ncand;
+            if (r2 <= h2) {
+                if (TAPS && count) ++nnbr;
+                body(k, xk, r, r2);
             }
+        });
+    };
+    if constexpr (V == 0) {
+        single(true);
+    } else {
+        const uint32_t cnt = collect<TAPS>(xi, h, mask, cells, X, self, col, ncand);
+        nnbr = cnt;
+        if (cnt <= (uint32_t)kNbrCap) {
+            walk_list(cnt, X, [&](uint32_t t) { return col[t * kBlock]; }, [&](uint32_t j, float4 xj) {
+                const V3 r = sub(xi, xyz(xj));
+                body(j, xj, r, dot(r, r));
+            });
+        } else {
+            single(false);
+        }
+    }
 }
 
 // CS_PredictAndHash.glsl:88-130
@@ -91,19 +204,40 @@ __global__ __launch_bounds__(kBlock) void k_build_cells(const uint32_t* __restri
     if (i == n - 1 || skeys[i + 1] != k) cells[k].y = (int32_t)(i + 1);
 }
 
-// Gather the predicted state into key-sorted order; lambda starts at 0
-// (CS_PredictAndHash.glsl:121).
-__global__ __launch_bounds__(kBlock) void k_reorder(uint32_t n, const uint32_t* __restrict__ perm,
+// Gather the predicted state into key-sorted order (lambda starts at 0,
+// CS_PredictAndHash.glsl:121) and bin every slot by the Morton code of its
+// cell; the bins give the spatial processing order of the gather passes.
+__global__ __launch_bounds__(kBlock) void k_reorder(StepConst c, const uint32_t* __restrict__ perm,
                                                     const float4* __restrict__ xstar,
                                                     const float4* __restrict__ vel, float4* __restrict__ P,
-                                                    float4* __restrict__ PL, float4* __restrict__ vs) {
+                                                    float4* __restrict__ PL, float4* __restrict__ vs,
+                                                    uint32_t* __restrict__ bins, uint32_t* __restrict__ bin_of,
+                                                    uint32_t* __restrict__ bin_rank) {
     const uint32_t k = gid();
-    if (k >= n) return;
+    if (k >= c.n) return;
     const uint32_t id = perm[k];
     const float4 x = xstar[id];
     P[k] = x;
     PL[k] = x;
     vs[k] = vel[id];
+    int32_t cx, cy, cz;
+    cell_of(xyz(x), c.h, cx, cy, cz);
+    const uint32_t b = morton_bin(c, cx, cy, cz);
+    bin_of[k] = b;
+    bin_rank[k] = atomicAdd(&bins[b], 1u);
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill_u32(uint32_t* __restrict__ p, uint32_t n, uint32_t v) {
+    const uint32_t i = gid();
+    if (i < n) p[i] = v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_order(uint32_t n, const uint32_t* __restrict__ base,
+                                                  const uint32_t* __restrict__ bin_of,
+                                                  const uint32_t* __restrict__ bin_rank, uint32_t* __restrict__ order) {
+    const uint32_t k = gid();
+    if (k >= n) return;
+    order[base[bin_of[k]] + bin_rank[k]] = k;
 }
 
 // CS_ComputeLOD.glsl:66-79
@@ -121,15 +255,20 @@ __global__ __launch_bounds__(kBlock) void k_lod(StepConst c, const float4* __res
     lod[k] = l;
 }
 
-// CS_ComputeLambdas.glsl:108-162
-template <bool TAPS>
-__global__ __launch_bounds__(kBlock) void k_lambda(StepConst c, const float4* __restrict__ P,
-                                                   float4* __restrict__ PL, float* __restrict__ rho_out,
-                                                   const uint32_t* __restrict__ lod, const int2* __restrict__ cells,
-                                                   uint32_t iter, uint32_t* __restrict__ cand_out,
-                                                   uint32_t* __restrict__ nbr_out) {
-    const uint32_t k = gid();
-    if (k >= c.n) return;
+// CS_ComputeLambdas.glsl:108-162.  In the two-phase form the neighbour list
+// is also written to HBM, row-major by thread (row r of thread t at
+// list[r * n + t]), for the delta-p pass of the same iteration, which sees
+// the same positions.
+template <int V, bool TAPS>
+__global__ __launch_bounds__(kBlock) void k_lambda(StepConst c, const uint32_t* __restrict__ order,
+                                                   const float4* __restrict__ P, float4* __restrict__ PL,
+                                                   float* __restrict__ rho_out, const uint32_t* __restrict__ lod,
+                                                   const int2* __restrict__ cells, uint32_t iter,
+                                                   uint32_t* __restrict__ list_count, uint32_t* __restrict__ list,
+                                                   uint32_t* __restrict__ cand_out, uint32_t* __restrict__ nbr_out) {
+    const uint32_t t = xcd_gid();
+    if (t >= c.n) return;
+    const uint32_t k = order[t];
     const V3 xi = xyz(P[k]);
     if (c.enable_apbf != 0u && lod[k] < iter) {
         // inactive: keep the last lambda, refresh the position
@@ -137,26 +276,40 @@ __global__ __launch_bounds__(kBlock) void k_lambda(StepConst c, const float4* __
         return;
     }
     const float inv = c.inv_rho0;
-    const float h2 = c.h * c.h;
     float rho = c.w0_self;
     float sum_g2 = 0.0f;
     V3 gi = mk(0.0f, 0.0f, 0.0f);
     uint32_t ncand = 0, nnbr = 0;
-    int32_t cx, cy, cz;
-    cell_of(xi, c.h, cx, cy, cz);
-    for_each_candidate(cx, cy, cz, c.hash_mask, cells, k, [&](uint32_t j) {
-        const V3 r = sub(xi, xyz(P[j]));
-        const float r2 = dot(r, r);
-        if (TAPS) ++ncand;
-        if (r2 <= h2) {
-            if (TAPS) ++nnbr;
-            rho = rho + poly6_r2(c, r2);
-            const V3 gw = spiky(c, r, r2);
-            const float gj2 = dot(gw, gw);
-            sum_g2 = sum_g2 + gj2 * (inv * inv);
-            gi = add(gi, gw);
+    auto body = [&](uint32_t, float4, V3 r, float r2) {
+        rho = rho + poly6_r2(c, r2);
+        const V3 gw = spiky(c, r, r2);
+        const float gj2 = dot(gw, gw);
+        sum_g2 = sum_g2 + gj2 * (inv * inv);
+        gi = add(gi, gw);
+    };
+    if constexpr (V == 0) {
+        gather<0, TAPS>(xi, c.h, c.hash_mask, cells, P, k, nullptr, body, ncand, nnbr);
+    } else {
+        __shared__ uint32_t lst[kNbrCap * kBlock];
+        uint32_t* col = &lst[threadIdx.x];
+        const uint32_t cnt = collect<TAPS>(xi, c.h, c.hash_mask, cells, P, k, col, ncand);
+        nnbr = cnt;
+        list_count[t] = cnt;
+        const uint32_t m = cnt < (uint32_t)kNbrCap ? cnt : (uint32_t)kNbrCap;
+#pragma unroll 1
+        for (uint32_t r = 0; r < m; ++r) list[(size_t)r * c.n + t] = col[r * kBlock];
+        if (cnt <= (uint32_t)kNbrCap) {
+            walk_list(cnt, P, [&](uint32_t r) { return col[r * kBlock]; }, [&](uint32_t j, float4 xj) {
+                const V3 d = sub(xi, xyz(xj));
+                body(j, xj, d, dot(d, d));
+            });
+        } else {
+            const float h2 = c.h * c.h;
+            scan_buckets(xi, c.h, c.hash_mask, cells, P, k, [&](uint32_t j, float4 xj, V3 d, float r2) {
+                if (r2 <= h2) body(j, xj, d, r2);
+            });
         }
-    });
+    }
     gi = scl(gi, inv);
     float C = rho / c.rho0 - 1.0f;
     const float floorc = -c.cohesion;
@@ -172,32 +325,47 @@ __global__ __launch_bounds__(kBlock) void k_lambda(StepConst c, const float4* __
 }
 
 // CS_ComputeDeltaP.glsl:120-166 fused with CS_ApplyDeltaP.glsl:80-124: reads
-// (x, lambda) from PL, writes the corrected position to P.
-template <bool TAPS>
-__global__ __launch_bounds__(kBlock) void k_deltap(StepConst c, const float4* __restrict__ PL,
-                                                   float4* __restrict__ P, const uint32_t* __restrict__ lod,
-                                                   const int2* __restrict__ cells, uint32_t iter,
-                                                   float4* __restrict__ dp_tap) {
-    const uint32_t k = gid();
-    if (k >= c.n) return;
+// (x, lambda) from PL, writes the corrected position to P.  In the two-phase
+// form the neighbours come from the list the lambda pass of this iteration
+// wrote (same positions, same order).
+template <int V, bool TAPS>
+__global__ __launch_bounds__(kBlock) void k_deltap(StepConst c, const uint32_t* __restrict__ order,
+                                                   const float4* __restrict__ PL, float4* __restrict__ P,
+                                                   const uint32_t* __restrict__ lod, const int2* __restrict__ cells,
+                                                   uint32_t iter, const uint32_t* __restrict__ list_count,
+                                                   const uint32_t* __restrict__ list, float4* __restrict__ dp_tap) {
+    const uint32_t t = xcd_gid();
+    if (t >= c.n) return;
+    const uint32_t k = order[t];
     if (c.enable_apbf != 0u && lod[k] < iter) return;  // P[k] already holds x
     const float4 xi4 = PL[k];
     const V3 xi = xyz(xi4);
     const float li = xi4.w;
     const float h2 = c.h * c.h;
     V3 dp = mk(0.0f, 0.0f, 0.0f);
-    int32_t cx, cy, cz;
-    cell_of(xi, c.h, cx, cy, cz);
-    for_each_candidate(cx, cy, cz, c.hash_mask, cells, k, [&](uint32_t j) {
-        const float4 xj4 = PL[j];
-        const V3 r = sub(xi, xyz(xj4));
-        const float r2 = dot(r, r);
-        if (r2 <= h2) {
-            const V3 gw = spiky(c, r, r2);
-            const float sc = scorr_r2(c, r2);
-            dp = add(dp, scl(gw, li + xj4.w + sc));
+    auto body = [&](float lj, V3 r, float r2) {
+        const V3 gw = spiky(c, r, r2);
+        const float sc = scorr_r2(c, r2);
+        dp = add(dp, scl(gw, li + lj + sc));
+    };
+    auto single = [&]() {
+        scan_buckets(xi, c.h, c.hash_mask, cells, PL, k, [&](uint32_t, float4 xj, V3 r, float r2) {
+            if (r2 <= h2) body(xj.w, r, r2);
+        });
+    };
+    if constexpr (V == 0) {
+        single();
+    } else {
+        const uint32_t cnt = list_count[t];
+        if (cnt <= (uint32_t)kNbrCap) {
+            walk_list(cnt, PL, [&](uint32_t r) { return list[(size_t)r * c.n + t]; }, [&](uint32_t, float4 xj) {
+                const V3 d = sub(xi, xyz(xj));
+                body(xj.w, d, dot(d, d));
+            });
+        } else {
+            single();
         }
-    });
+    }
     dp = scl(dp, c.inv_rho0);
     if (TAPS) dp_tap[k] = make_float4(dp.x, dp.y, dp.z, 0.0f);
     // apply: clamp |dp| <= h/2, x += dp, SDF projection
@@ -218,14 +386,17 @@ __global__ __launch_bounds__(kBlock) void k_deltap(StepConst c, const float4* __
 
 // CS_Integrate.glsl:135-280.  Neighbour velocities come from the sorted
 // post-predict snapshot vs (the Jacobi form of the racy read at :171).
-__global__ __launch_bounds__(kBlock) void k_integrate(StepConst c, const float4* __restrict__ P,
+template <int V>
+__global__ __launch_bounds__(kBlock) void k_integrate(StepConst c, const uint32_t* __restrict__ order, const float4* __restrict__ P,
                                                       const float4* __restrict__ vs,
                                                       const uint32_t* __restrict__ perm,
                                                       float4* __restrict__ pos, float4* __restrict__ vel,
                                                       const int2* __restrict__ cells, float4* __restrict__ curl_out,
                                                       float4* __restrict__ posS) {
-    const uint32_t k = gid();
-    if (k >= c.n) return;
+    const uint32_t t = xcd_gid();
+    if (t >= c.n) return;
+    const uint32_t k = order[t];
+    
     const uint32_t id = perm[k];
     const float4 op4 = pos[id];
     const V3 old_pos = xyz(op4);
@@ -237,26 +408,21 @@ __global__ __launch_bounds__(kBlock) void k_integrate(StepConst c, const float4*
     const bool do_vort = c.enable_vort == 1u;
     V3 visc_delta = mk(0.0f, 0.0f, 0.0f);
     if (do_visc || do_vort) {
-        const float h2 = c.h * c.h;
         V3 visc = mk(0.0f, 0.0f, 0.0f);
         V3 curl = mk(0.0f, 0.0f, 0.0f);
-        int32_t cx, cy, cz;
-        cell_of(xp, c.h, cx, cy, cz);
-        for_each_candidate(cx, cy, cz, c.hash_mask, cells, k, [&](uint32_t j) {
-            const V3 r = sub(xp, xyz(P[j]));
-            const float r2 = dot(r, r);
-            if (r2 <= h2) {
-                const V3 vij = sub(xyz(vs[j]), old_vel);
-                if (do_visc) {
-                    const float w = poly6_r2(c, r2);
-                    visc = add(visc, scl(scl(vij, w), inv));
-                }
-                if (do_vort) {
-                    const V3 gw = spiky(c, r, r2);
-                    curl = add(curl, cross(vij, gw));
-                }
+        uint32_t ncand = 0, nnbr = 0;
+        __shared__ uint32_t lst[kNbrCap * kBlock];
+        gather<V, false>(xp, c.h, c.hash_mask, cells, P, k, &lst[threadIdx.x], [&](uint32_t j, float4, V3 r, float r2) {
+            const V3 vij = sub(xyz(vs[j]), old_vel);
+            if (do_visc) {
+                const float w = poly6_r2(c, r2);
+                visc = add(visc, scl(scl(vij, w), inv));
             }
-        });
+            if (do_vort) {
+                const V3 gw = spiky(c, r, r2);
+                curl = add(curl, cross(vij, gw));
+            }
+        }, ncand, nnbr);
         if (do_visc) visc_delta = scl(visc, c.viscosity);
         if (do_vort) {
             const V3 cw = scl(curl, inv);
@@ -291,26 +457,24 @@ __global__ __launch_bounds__(kBlock) void k_integrate(StepConst c, const float4*
 }
 
 // CS_VorticityConfinement.glsl:133-167 (pass 1)
-__global__ __launch_bounds__(kBlock) void k_vorticity(StepConst c, const float4* __restrict__ posS,
+template <int V>
+__global__ __launch_bounds__(kBlock) void k_vorticity(StepConst c, const uint32_t* __restrict__ order, const float4* __restrict__ posS,
                                                       const float4* __restrict__ curl,
                                                       const uint32_t* __restrict__ perm,
                                                       float4* __restrict__ vel, const int2* __restrict__ cells) {
-    const uint32_t k = gid();
-    if (k >= c.n) return;
+    const uint32_t t = xcd_gid();
+    if (t >= c.n) return;
+    const uint32_t k = order[t];
+    
     const V3 pi = xyz(posS[k]);
-    const float h2 = c.h * c.h;
-    int32_t cx, cy, cz;
-    cell_of(pi, c.h, cx, cy, cz);
     V3 eta = mk(0.0f, 0.0f, 0.0f);
-    for_each_candidate(cx, cy, cz, c.hash_mask, cells, k, [&](uint32_t j) {
-        const V3 r = sub(pi, xyz(posS[j]));
-        const float r2 = dot(r, r);
-        if (r2 <= h2) {
-            const V3 gw = spiky(c, r, r2);
-            const float om = len(xyz(curl[j]));
-            eta = add(eta, scl(gw, om));
-        }
-    });
+    uint32_t ncand = 0, nnbr = 0;
+    __shared__ uint32_t lst[kNbrCap * kBlock];
+    gather<V, false>(pi, c.h, c.hash_mask, cells, posS, k, &lst[threadIdx.x], [&](uint32_t j, float4, V3 r, float r2) {
+        const V3 gw = spiky(c, r, r2);
+        const float om = len(xyz(curl[j]));
+        eta = add(eta, scl(gw, om));
+    }, ncand, nnbr);
     eta = scl(eta, c.inv_rho0);
     const float el = len(eta);
     if (el < 1e-6f) return;
@@ -363,7 +527,12 @@ hipError_t launch_cells(const StepConst& c, DeviceState& d, hipStream_t st) {
 }
 
 hipError_t launch_reorder(const StepConst& c, DeviceState& d, hipStream_t st) {
-    hipLaunchKernelGGL(k_reorder, grid_for(c.n), dim3(kBlock), 0, st, c.n, d.perm, d.xstar, d.vel, d.P, d.PL, d.vs);
+    hipLaunchKernelGGL(k_fill_u32, grid_for(kOrderBins), dim3(kBlock), 0, st, d.bins, kOrderBins, 0u);
+    hipLaunchKernelGGL(k_reorder, grid_for(c.n), dim3(kBlock), 0, st, c, d.perm, d.xstar, d.vel, d.P, d.PL, d.vs,
+                       d.bins, d.bin_of, d.bin_rank);
+    hipError_t e = scan_exclusive_u32(d.bins, d.bins, kOrderBins, d.sort.partial, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_order, grid_for(c.n), dim3(kBlock), 0, st, c.n, d.bins, d.bin_of, d.bin_rank, d.order);
     return hipGetLastError();
 }
 
@@ -372,34 +541,52 @@ hipError_t launch_lod(const StepConst& c, DeviceState& d, hipStream_t st) {
     return hipGetLastError();
 }
 
+#define FSI_VARIANT_SWITCH(v, CALL) \
+    switch (v) {                        \
+        case 1: CALL(1); break;         \
+        default: CALL(0); break;        \
+    }
+
 hipError_t launch_lambda(const StepConst& c, DeviceState& d, uint32_t iter, bool taps, hipStream_t st) {
-    if (taps)
-        hipLaunchKernelGGL(k_lambda<true>, grid_for(c.n), dim3(kBlock), 0, st, c, d.P, d.PL, d.rho, d.lod, d.cells,
-                           iter, d.cand, d.nbr);
-    else
-        hipLaunchKernelGGL(k_lambda<false>, grid_for(c.n), dim3(kBlock), 0, st, c, d.P, d.PL, d.rho, d.lod, d.cells,
-                           iter, d.cand, d.nbr);
+#define L(V)                                                                                                   \
+    if (taps)                                                                                                  \
+        hipLaunchKernelGGL((k_lambda<V, true>), grid_for(c.n), dim3(kBlock), 0, st, c, d.order, d.P, d.PL, d.rho, d.lod, \
+                           d.cells, iter, d.list_count, d.list, d.cand, d.nbr);                                 \
+    else                                                                                                       \
+        hipLaunchKernelGGL((k_lambda<V, false>), grid_for(c.n), dim3(kBlock), 0, st, c, d.order, d.P, d.PL, d.rho,      \
+                           d.lod, d.cells, iter, d.list_count, d.list, d.cand, d.nbr);
+    FSI_VARIANT_SWITCH(d.variant, L)
+#undef L
     return hipGetLastError();
 }
 
 hipError_t launch_deltap(const StepConst& c, DeviceState& d, uint32_t iter, bool taps, hipStream_t st) {
-    if (taps)
-        hipLaunchKernelGGL(k_deltap<true>, grid_for(c.n), dim3(kBlock), 0, st, c, d.PL, d.P, d.lod, d.cells, iter,
-                           d.dp_tap);
-    else
-        hipLaunchKernelGGL(k_deltap<false>, grid_for(c.n), dim3(kBlock), 0, st, c, d.PL, d.P, d.lod, d.cells, iter,
-                           d.dp_tap);
+#define L(V)                                                                                                    \
+    if (taps)                                                                                                   \
+        hipLaunchKernelGGL((k_deltap<V, true>), grid_for(c.n), dim3(kBlock), 0, st, c, d.order, d.PL, d.P, d.lod, d.cells, \
+                           iter, d.list_count, d.list, d.dp_tap);                                               \
+    else                                                                                                        \
+        hipLaunchKernelGGL((k_deltap<V, false>), grid_for(c.n), dim3(kBlock), 0, st, c, d.order, d.PL, d.P, d.lod,        \
+                           d.cells, iter, d.list_count, d.list, d.dp_tap);
+    FSI_VARIANT_SWITCH(d.variant, L)
+#undef L
     return hipGetLastError();
 }
 
 hipError_t launch_integrate(const StepConst& c, DeviceState& d, hipStream_t st) {
-    hipLaunchKernelGGL(k_integrate, grid_for(c.n), dim3(kBlock), 0, st, c, d.P, d.vs, d.perm, d.pos, d.vel, d.cells,
-                       d.curl, d.posS);
+#define L(V)                                                                                                     \
+    hipLaunchKernelGGL((k_integrate<V>), grid_for(c.n), dim3(kBlock), 0, st, c, d.order, d.P, d.vs, d.perm, d.pos, d.vel, \
+                       d.cells, d.curl, d.posS);
+    FSI_VARIANT_SWITCH(d.variant, L)
+#undef L
     return hipGetLastError();
 }
 
 hipError_t launch_vorticity(const StepConst& c, DeviceState& d, hipStream_t st) {
-    hipLaunchKernelGGL(k_vorticity, grid_for(c.n), dim3(kBlock), 0, st, c, d.posS, d.curl, d.perm, d.vel, d.cells);
+#define L(V) \
+    hipLaunchKernelGGL((k_vorticity<V>), grid_for(c.n), dim3(kBlock), 0, st, c, d.order, d.posS, d.curl, d.perm, d.vel, d.cells);
+    FSI_VARIANT_SWITCH(d.variant, L)
+#undef L
     return hipGetLastError();
 }
 

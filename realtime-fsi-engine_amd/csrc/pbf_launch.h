@@ -12,6 +12,9 @@ namespace fsi {
 
 // Keys per radix tile = 256 threads x kSortIPT.
 constexpr int kSortIPT = 8;
+// Neighbour-list capacity: rows of the lambda -> delta-p list in HBM, and the
+// per-lane LDS list of the other gather passes.
+constexpr int kNbrCap = 32;
 // Widest radix digit the solver plans with (LDS: 44 KiB per workgroup at 10).
 constexpr int kSortMaxDigitBits = 10;
 
@@ -55,7 +58,14 @@ struct DeviceState {
     uint32_t* cand = nullptr;  // sorted: candidates of the last lambda pass (taps only)
     uint32_t* nbr = nullptr;   // sorted: neighbours of the last lambda pass (taps only)
     uint32_t* flags = nullptr; // n: contact flags / scan
+    uint32_t* bins = nullptr;  // kOrderBins: Morton-bin counts, then their exclusive scan
+    uint32_t* bin_of = nullptr;  // sorted slot -> Morton bin
+    uint32_t* bin_rank = nullptr;  // sorted slot -> rank inside its bin
+    uint32_t* order = nullptr; // processing order: thread t handles sorted slot order[t]
+    uint32_t* list_count = nullptr;  // by thread: neighbours found by the last lambda pass
+    uint32_t* list = nullptr;  // kNbrCap x n: neighbour slots, row-major by thread
     SortScratch sort{};
+    int variant = 0;           // gather form (tuning knob, see gather() in pbf_kernels.hip)
 };
 
 hipError_t launch_predict(const StepConst& c, DeviceState& d, hipStream_t st);

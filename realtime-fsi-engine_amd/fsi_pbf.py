@@ -24,6 +24,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libfsi_pbf.so")
 
 FSI_OK = 0
+TUNE_GATHER, TUNE_SORT_DIGIT_BITS = 0, 1
 PASS_NAMES = ["predict", "sort", "tables", "lambda", "deltap", "integrate", "vorticity", "lod"]
 
 
@@ -171,6 +172,8 @@ def load() -> C.CDLL:
         "fsi_pbf_timing_reset": ([P], C.c_int),
         "fsi_pbf_read_timing": ([P, C.POINTER(CTiming)], C.c_int),
         "fsi_pbf_get_stream": ([P, C.POINTER(P)], C.c_int),
+        "fsi_pbf_set_tuning": ([P, C.c_int, C.c_int], C.c_int),
+        "fsi_pbf_get_tuning": ([P, C.c_int, C.POINTER(C.c_int)], C.c_int),
         "fsi_scene_lattice": ([C.c_uint32, C.c_uint32, C.c_uint32, f32p, C.c_float, C.c_float, C.c_uint32, f32p, P],
                               C.c_int),
         "fsi_mesh_icosphere": ([C.c_uint32, f64p, C.c_double, f64p, i32p], C.c_int),
@@ -362,6 +365,14 @@ class Solver:
         nb = np.zeros(self.n, dtype=np.uint32)
         _check(self.lib.fsi_pbf_read_counts(self.h, _ptr(c, C.c_uint32), _ptr(nb, C.c_uint32)))
         return c, nb
+
+    def set_tuning(self, knob: int, value: int):
+        _check(self.lib.fsi_pbf_set_tuning(self.h, knob, value))
+
+    def get_tuning(self, knob: int) -> int:
+        v = C.c_int(0)
+        _check(self.lib.fsi_pbf_get_tuning(self.h, knob, C.byref(v)))
+        return v.value
 
     def set_timing(self, on: bool):
         _check(self.lib.fsi_pbf_set_timing(self.h, 1 if on else 0))
