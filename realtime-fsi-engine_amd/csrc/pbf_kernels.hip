@@ -79,23 +79,67 @@ __device__ __forceinline__ void scan_buckets(V3 x, float h, uint32_t mask, const
     }
 }
 
-// Phase 1 of a two-phase gather: append, in visit order, the slots of the
-// candidates inside the support (r^2 <= h^2) to this lane's column of an LDS
-// list (col[t * kBlock], at most kNbrCap entries).  Returns the neighbour count.
-template <bool TAPS>
-__device__ __forceinline__ uint32_t collect(V3 xi, float h, uint32_t mask, const int2* __restrict__ cells,
-                                            const float4* __restrict__ X, uint32_t self, uint32_t* col,
-                                            uint32_t& ncand) {
+// Per-lane LDS list of the two-phase form (entries), and the fill level at
+// which the whole wave flushes.
+constexpr uint32_t kLdsCap = 16;
+constexpr uint32_t kFlushAt = 8;
+
+// Phase 1 of the two-phase gather, with the phase-2 work interleaved in
+// wave-uniform flushes.  The 27 buckets are scanned as in scan_buckets; the
+// slots of candidates inside the support (r^2 <= h^2) are appended, in visit
+// order, to this lane's column of an LDS list (col[e * kBlock]).  After every
+// bucket, if any lane holds kFlushAt entries or more, every lane hands its
+// entries to flush(first, count) and empties its column, so the expensive
+// arithmetic of phase 2 runs with the lanes of the wave together; a lane whose
+// column fills inside a bucket flushes alone.  Entries reach flush in visit
+// order, numbered from `first`.  Returns the lane's neighbour count.
+template <bool TAPS, class Flush>
+__device__ __forceinline__ uint32_t collect_flush(V3 x, float h, uint32_t mask, const int2* __restrict__ cells,
+                                                  const float4* __restrict__ X, uint32_t self, uint32_t* col,
+                                                  Flush&& flush, uint32_t& ncand) {
     const float h2 = h * h;
-    uint32_t cnt = 0;
-    scan_buckets(xi, h, mask, cells, X, self, [&](uint32_t k, float4, V3, float r2) {
+    int32_t cx, cy, cz;
+    cell_of(x, h, cx, cy, cz);
+    int2 se = cells[cell_hash(cx - 1, cy - 1, cz - 1, mask)];
+    uint32_t cnt = 0, total = 0;
+    auto visit = [&](int32_t k, float4 xk) {
+        if ((uint32_t)k == self) return;
         if (TAPS) ++ncand;
-        if (r2 <= h2) {
-            if (cnt < (uint32_t)kNbrCap) col[cnt * kBlock] = k;
-            ++cnt;
+        const V3 r = sub(x, xyz(xk));
+        if (dot(r, r) <= h2) {
+            col[cnt * kBlock] = (uint32_t)k;
+            if (++cnt == kLdsCap) {
+                flush(total, cnt);
+                total += cnt;
+                cnt = 0;
+            }
         }
-    });
-    return cnt;
+    };
+#pragma unroll 1
+    for (int q = 0; q < 27; ++q) {
+        int2 nxt = make_int2(-1, -1);
+        if (q < 26) {
+            const int qn = q + 1;
+            nxt = cells[cell_hash(cx + qn % 3 - 1, cy + (qn / 3) % 3 - 1, cz + qn / 9 - 1, mask)];
+        }
+        const int32_t last = se.y - 1;
+        for (int32_t k = se.x; k < se.y; k += 4) {
+            const int32_t k1 = min(k + 1, last), k2 = min(k + 2, last), k3 = min(k + 3, last);
+            const float4 a0 = X[k], a1 = X[k1], a2 = X[k2], a3 = X[k3];
+            visit(k, a0);
+            if (k + 1 <= last) visit(k + 1, a1);
+            if (k + 2 <= last) visit(k + 2, a2);
+            if (k + 3 <= last) visit(k + 3, a3);
+        }
+        se = nxt;
+        if (__any(cnt >= kFlushAt)) {
+            flush(total, cnt);
+            total += cnt;
+            cnt = 0;
+        }
+    }
+    flush(total, cnt);
+    return total + cnt;
 }
 
 // Run body over the slots idx(0..cnt) in order, fetching four neighbours
@@ -122,40 +166,31 @@ __device__ __forceinline__ void walk_list(uint32_t cnt, const float4* __restrict
 
 // Neighbour gather, bit-identical to the reference's loop in both forms:
 //   V = 0  single loop: body(j, xj, r, r2) for each candidate in the support.
-//   V = 1  two phases: collect() the neighbours, then run body over the list,
-//          so the lanes of a wave execute the expensive arithmetic (square
-//          root, divisions) together on real neighbours instead of under
-//          divergence on the ~85% of candidates outside the support.  A
-//          particle with more than kNbrCap neighbours re-runs the single loop
-//          (same slots, same order).
+//   V = 1  two phases (collect_flush): the lanes of a wave execute the
+//          expensive arithmetic (square root, divisions) together on real
+//          neighbours instead of under divergence on the ~85% of candidates
+//          outside the support.
 // Pairs with r^2 > h^2 are skipped: in the reference they add exact zeros.
 template <int V, bool TAPS, class Body>
 __device__ __forceinline__ void gather(V3 xi, float h, uint32_t mask, const int2* __restrict__ cells,
                                        const float4* __restrict__ X, uint32_t self, uint32_t* col, Body&& body,
                                        uint32_t& ncand, uint32_t& nnbr) {
-    const float h2 = h * h;
-    auto single = [&](bool count) {
+    if constexpr (V == 0) {
+        const float h2 = h * h;
         scan_buckets(xi, h, mask, cells, X, self, [&](uint32_t k, float4 xk, V3 r, float r2) {
-            if (TAPS && count) ++ncand;
+            if (TAPS) ++ncand;
             if (r2 <= h2) {
-                if (TAPS && count) ++nnbr;
+                if (TAPS) ++nnbr;
                 body(k, xk, r, r2);
             }
         });
-    };
-    if constexpr (V == 0) {
-        single(true);
     } else {
-        const uint32_t cnt = collect<TAPS>(xi, h, mask, cells, X, self, col, ncand);
-        nnbr = cnt;
-        if (cnt <= (uint32_t)kNbrCap) {
-            walk_list(cnt, X, [&](uint32_t t) { return col[t * kBlock]; }, [&](uint32_t j, float4 xj) {
+        nnbr = collect_flush<TAPS>(xi, h, mask, cells, X, self, col, [&](uint32_t, uint32_t cnt) {
+            walk_list(cnt, X, [&](uint32_t e) { return col[e * kBlock]; }, [&](uint32_t j, float4 xj) {
                 const V3 r = sub(xi, xyz(xj));
                 body(j, xj, r, dot(r, r));
             });
-        } else {
-            single(false);
-        }
+        }, ncand);
     }
 }
 
@@ -256,9 +291,9 @@ __global__ __launch_bounds__(kBlock) void k_lod(StepConst c, const float4* __res
 }
 
 // CS_ComputeLambdas.glsl:108-162.  In the two-phase form the neighbour list
-// is also written to HBM, row-major by thread (row r of thread t at
-// list[r * n + t]), for the delta-p pass of the same iteration, which sees
-// the same positions.
+// is also written to HBM, one row of kNbrCap slots per thread (entry r of
+// thread t at list[t * kNbrCap + r]), for the delta-p pass of the same
+// iteration, which sees the same positions.
 template <int V, bool TAPS>
 __global__ __launch_bounds__(kBlock) void k_lambda(StepConst c, const uint32_t* __restrict__ order,
                                                    const float4* __restrict__ P, float4* __restrict__ PL,
@@ -290,25 +325,18 @@ __global__ __launch_bounds__(kBlock) void k_lambda(StepConst c, const uint32_t* 
     if constexpr (V == 0) {
         gather<0, TAPS>(xi, c.h, c.hash_mask, cells, P, k, nullptr, body, ncand, nnbr);
     } else {
-        __shared__ uint32_t lst[kNbrCap * kBlock];
+        __shared__ uint32_t lst[kLdsCap * kBlock];
         uint32_t* col = &lst[threadIdx.x];
-        const uint32_t cnt = collect<TAPS>(xi, c.h, c.hash_mask, cells, P, k, col, ncand);
-        nnbr = cnt;
-        list_count[t] = cnt;
-        const uint32_t m = cnt < (uint32_t)kNbrCap ? cnt : (uint32_t)kNbrCap;
+        nnbr = collect_flush<TAPS>(xi, c.h, c.hash_mask, cells, P, k, col, [&](uint32_t first, uint32_t cnt) {
 #pragma unroll 1
-        for (uint32_t r = 0; r < m; ++r) list[(size_t)r * c.n + t] = col[r * kBlock];
-        if (cnt <= (uint32_t)kNbrCap) {
-            walk_list(cnt, P, [&](uint32_t r) { return col[r * kBlock]; }, [&](uint32_t j, float4 xj) {
+            for (uint32_t e = 0; e < cnt; ++e)
+                if (first + e < (uint32_t)kNbrCap) list[(size_t)t * kNbrCap + first + e] = col[e * kBlock];
+            walk_list(cnt, P, [&](uint32_t e) { return col[e * kBlock]; }, [&](uint32_t j, float4 xj) {
                 const V3 d = sub(xi, xyz(xj));
                 body(j, xj, d, dot(d, d));
             });
-        } else {
-            const float h2 = c.h * c.h;
-            scan_buckets(xi, c.h, c.hash_mask, cells, P, k, [&](uint32_t j, float4 xj, V3 d, float r2) {
-                if (r2 <= h2) body(j, xj, d, r2);
-            });
-        }
+        }, ncand);
+        list_count[t] = nnbr;
     }
     gi = scl(gi, inv);
     float C = rho / c.rho0 - 1.0f;
@@ -324,8 +352,9 @@ __global__ __launch_bounds__(kBlock) void k_lambda(StepConst c, const uint32_t* 
     }
 }
 
-// CS_ComputeDeltaP.glsl:120-166 fused with CS_ApplyDeltaP.glsl:80-124: reads
-// (x, lambda) from PL, writes the corrected position to P.  In the two-phase
+// CS_ComputeDeltaP.glsl:120-166 fused with the clamp and add of
+// CS_ApplyDeltaP.glsl:80-106: reads (x, lambda) from PL, writes the corrected
+// position to P (the SDF projection follows in k_apply_sdf).  In the two-phase
 // form the neighbours come from the list the lambda pass of this iteration
 // wrote (same positions, same order).
 template <int V, bool TAPS>
@@ -358,10 +387,28 @@ __global__ __launch_bounds__(kBlock) void k_deltap(StepConst c, const uint32_t* 
     } else {
         const uint32_t cnt = list_count[t];
         if (cnt <= (uint32_t)kNbrCap) {
-            walk_list(cnt, PL, [&](uint32_t r) { return list[(size_t)r * c.n + t]; }, [&](uint32_t, float4 xj) {
-                const V3 d = sub(xi, xyz(xj));
-                body(xj.w, d, dot(d, d));
-            });
+            // this thread's row of the list: 16-byte aligned, four slots per load
+            const uint4* row = reinterpret_cast<const uint4*>(list + (size_t)t * kNbrCap);
+            uint32_t r = 0;
+#pragma unroll 1
+            for (; r + 4 <= cnt; r += 4) {
+                const uint4 j = row[r >> 2];
+                const float4 a0 = PL[j.x], a1 = PL[j.y], a2 = PL[j.z], a3 = PL[j.w];
+                const V3 d0 = sub(xi, xyz(a0));
+                body(a0.w, d0, dot(d0, d0));
+                const V3 d1 = sub(xi, xyz(a1));
+                body(a1.w, d1, dot(d1, d1));
+                const V3 d2 = sub(xi, xyz(a2));
+                body(a2.w, d2, dot(d2, d2));
+                const V3 d3 = sub(xi, xyz(a3));
+                body(a3.w, d3, dot(d3, d3));
+            }
+#pragma unroll 1
+            for (; r < cnt; ++r) {
+                const float4 a = PL[list[(size_t)t * kNbrCap + r]];
+                const V3 d = sub(xi, xyz(a));
+                body(a.w, d, dot(d, d));
+            }
         } else {
             single();
         }
@@ -376,12 +423,23 @@ __global__ __launch_bounds__(kBlock) void k_deltap(StepConst c, const uint32_t* 
         const float l = sqrtf(l2);
         dp = scl(dp, maxdp / l);
     }
-    V3 p = add(xi, dp);
-    if (c.enable_sdf == 1u) {
-        V3 nrm;
-        (void)sdf_project(c, p, nrm);
-    }
+    const V3 p = add(xi, dp);
     P[k] = make_float4(p.x, p.y, p.z, 0.0f);
+}
+
+// The SDF projection of CS_ApplyDeltaP.glsl:108-121 on the corrected
+// positions, in place (a kernel of its own so that the seven trilinear
+// samples do not raise the register count of the delta-p gather).
+__global__ __launch_bounds__(kBlock) void k_apply_sdf(StepConst c, const uint32_t* __restrict__ order,
+                                                      float4* __restrict__ P, const uint32_t* __restrict__ lod,
+                                                      uint32_t iter) {
+    const uint32_t t = xcd_gid();
+    if (t >= c.n) return;
+    const uint32_t k = order[t];
+    if (c.enable_apbf != 0u && lod[k] < iter) return;
+    V3 p = xyz(P[k]);
+    V3 nrm;
+    if (sdf_project(c, p, nrm)) P[k] = make_float4(p.x, p.y, p.z, 0.0f);
 }
 
 // CS_Integrate.glsl:135-280.  Neighbour velocities come from the sorted
@@ -411,7 +469,7 @@ __global__ __launch_bounds__(kBlock) void k_integrate(StepConst c, const uint32_
         V3 visc = mk(0.0f, 0.0f, 0.0f);
         V3 curl = mk(0.0f, 0.0f, 0.0f);
         uint32_t ncand = 0, nnbr = 0;
-        __shared__ uint32_t lst[kNbrCap * kBlock];
+        __shared__ uint32_t lst[kLdsCap * kBlock];
         gather<V, false>(xp, c.h, c.hash_mask, cells, P, k, &lst[threadIdx.x], [&](uint32_t j, float4, V3 r, float r2) {
             const V3 vij = sub(xyz(vs[j]), old_vel);
             if (do_visc) {
@@ -469,7 +527,7 @@ __global__ __launch_bounds__(kBlock) void k_vorticity(StepConst c, const uint32_
     const V3 pi = xyz(posS[k]);
     V3 eta = mk(0.0f, 0.0f, 0.0f);
     uint32_t ncand = 0, nnbr = 0;
-    __shared__ uint32_t lst[kNbrCap * kBlock];
+    __shared__ uint32_t lst[kLdsCap * kBlock];
     gather<V, false>(pi, c.h, c.hash_mask, cells, posS, k, &lst[threadIdx.x], [&](uint32_t j, float4, V3 r, float r2) {
         const V3 gw = spiky(c, r, r2);
         const float om = len(xyz(curl[j]));
@@ -570,6 +628,8 @@ hipError_t launch_deltap(const StepConst& c, DeviceState& d, uint32_t iter, bool
                            d.cells, iter, d.list_count, d.list, d.dp_tap);
     FSI_VARIANT_SWITCH(d.variant, L)
 #undef L
+    if (c.enable_sdf == 1u)
+        hipLaunchKernelGGL(k_apply_sdf, grid_for(c.n), dim3(kBlock), 0, st, c, d.order, d.P, d.lod, iter);
     return hipGetLastError();
 }
 

@@ -63,9 +63,9 @@ struct DeviceState {
     uint32_t* bin_rank = nullptr;  // sorted slot -> rank inside its bin
     uint32_t* order = nullptr; // processing order: thread t handles sorted slot order[t]
     uint32_t* list_count = nullptr;  // by thread: neighbours found by the last lambda pass
-    uint32_t* list = nullptr;  // kNbrCap x n: neighbour slots, row-major by thread
+    uint32_t* list = nullptr;  // n x kNbrCap: neighbour slots, one row per thread
     SortScratch sort{};
-    int variant = 0;           // gather form (tuning knob, see gather() in pbf_kernels.hip)
+    int variant = 1;           // gather form (tuning knob, see gather() in pbf_kernels.hip)
 };
 
 hipError_t launch_predict(const StepConst& c, DeviceState& d, hipStream_t st);
