@@ -175,17 +175,36 @@ void pack_constants(fsi_pbf* h) {
     u.enable_apbf = p.enable_apbf ? 1u : 0u;
 }
 
-StepConst step_const(const fsi_pbf* h) {
-    const fsi_pbf_constants& u = h->k;
+// Morton binning of one scene's processing order: cover its bounds (plus two
+// cells) with at most 2^axis_bits bins per axis.
+void set_binning(Domain& d, const float bmin[3], const float bmax[3], float h, uint32_t axis_bits,
+                 uint32_t bin_base) {
+    auto cell_clamped = [&](float x) {
+        const double v = std::floor((double)x / (double)h);
+        return (int64_t)std::max(-1.0e9, std::min(1.0e9, std::isfinite(v) ? v : 0.0));
+    };
+    int64_t span = 1;
+    for (int a = 0; a < 3; ++a) {
+        const int64_t lo = cell_clamped(bmin[a]) - 2;
+        const int64_t hi = cell_clamped(bmax[a]) + 2;
+        d.dcmin[a] = (int32_t)lo;
+        span = std::max<int64_t>(span, hi - lo + 1);
+    }
+    uint32_t shift = 0;
+    while ((span >> shift) > ((int64_t)1 << axis_bits) && shift < 31) ++shift;
+    d.dshift = shift;
+    d.axis_bits = axis_bits;
+    d.bin_base = bin_base;
+}
+
+// The constants shared by every scene of a step (FluidConfigUBO minus the box).
+StepConst step_const_common(const fsi_fluid_config& cfg, const fsi_pbf_constants& u, uint32_t n, bool interacting,
+                            const float ipos[3], const float cam[3]) {
     StepConst c{};
     for (int a = 0; a < 3; ++a) {
-        c.bmin[a] = u.bounds_min[a];
-        c.bmax[a] = u.bounds_max[a];
         c.g[a] = u.gravity_dt[a];
-        c.ipos[a] = h->ipos[a];
-        c.cam[a] = h->cam[a];
-        c.dims[a] = h->sdf_dims[a];
-        c.sdf_origin[a] = h->sdf_origin[a];
+        c.ipos[a] = ipos[a];
+        c.cam[a] = cam[a];
     }
     c.sub_dt = u.gravity_dt[3];
     c.h = u.h;
@@ -204,37 +223,36 @@ StepConst step_const(const fsi_pbf* h) {
     c.spiky = u.spiky_coeff;
     c.inv_rho0 = u.inv_rho0;
     c.lod_max_dist = u.lod_max_dist;
-    c.hash_mask = u.hash_mask;
-    c.n = h->d.n;
+    c.n = n;
     c.enable_scorr = u.enable_scorr;
     c.enable_visc = u.enable_viscosity;
     c.min_lod = u.min_lod;
     c.max_lod = u.max_lod;
     c.enable_apbf = u.enable_apbf;
-    c.enable_vort = h->cfg.enable_vorticity ? 1u : 0u;
-    c.interacting = h->interacting ? 1u : 0u;
-    c.vort_eps = h->cfg.vorticity_epsilon;
-    c.enable_sdf = h->sdf_valid ? 1u : 0u;
-    c.sdf_mode = h->sdf_containment ? 1u : 0u;
-    c.sdf_cell = h->sdf_cell;
-    c.sdf_pad = h->sdf_padding;
-    c.sdf_grid = h->sdf_grid;
-    // Morton binning of the processing order: cover the bounds (plus two
-    // cells) with at most 128 bins per axis
-    auto cell_clamped = [&](float x) {
-        const double v = std::floor((double)x / (double)u.h);
-        return (int64_t)std::max(-1.0e9, std::min(1.0e9, std::isfinite(v) ? v : 0.0));
-    };
-    int64_t span = 1;
+    c.enable_vort = cfg.enable_vorticity ? 1u : 0u;
+    c.interacting = interacting ? 1u : 0u;
+    c.vort_eps = cfg.vorticity_epsilon;
+    return c;
+}
+
+StepConst step_const(const fsi_pbf* h) {
+    const fsi_pbf_constants& u = h->k;
+    StepConst c = step_const_common(h->cfg, u, h->d.n, h->interacting, h->ipos, h->cam);
+    Domain& d = c.dom;
     for (int a = 0; a < 3; ++a) {
-        const int64_t lo = cell_clamped(u.bounds_min[a]) - 2;
-        const int64_t hi = cell_clamped(u.bounds_max[a]) + 2;
-        c.dcmin[a] = (int32_t)lo;
-        span = std::max<int64_t>(span, hi - lo + 1);
+        d.bmin[a] = u.bounds_min[a];
+        d.bmax[a] = u.bounds_max[a];
+        d.dims[a] = h->sdf_dims[a];
+        d.sdf_origin[a] = h->sdf_origin[a];
     }
-    uint32_t shift = 0;
-    while ((span >> shift) > 128 && shift < 31) ++shift;
-    c.dshift = shift;
+    d.hash_mask = u.hash_mask;
+    d.hash_base = 0;
+    d.enable_sdf = h->sdf_valid ? 1u : 0u;
+    d.sdf_mode = h->sdf_containment ? 1u : 0u;
+    d.sdf_cell = h->sdf_cell;
+    d.sdf_pad = h->sdf_padding;
+    d.sdf_grid = h->sdf_grid;
+    set_binning(d, u.bounds_min, u.bounds_max, u.h, 7, 0);
     return c;
 }
 
@@ -282,6 +300,7 @@ hipError_t alloc_particles(fsi_pbf* h, uint32_t n) {
     ALLOC(d.cand, n);
     ALLOC(d.nbr, n);
     ALLOC(d.flags, n);
+    ALLOC(d.slots, n);
     ALLOC(d.bins, kOrderBins);
     ALLOC(d.bin_of, n);
     ALLOC(d.bin_rank, n);
@@ -322,6 +341,7 @@ void free_all(fsi_pbf* h) {
     dev_free(&d.cand);
     dev_free(&d.nbr);
     dev_free(&d.flags);
+    dev_free(&d.slots);
     dev_free(&d.bins);
     dev_free(&d.bin_of);
     dev_free(&d.bin_rank);
@@ -682,6 +702,7 @@ int fsi_pbf_step(fsi_pbf* h) {
     if (h->d.n == 0) return FSI_OK;  // PBFluids.cpp:325
     const StepConst c = step_const(h);
     DeviceState& d = h->d;
+    d.any_sdf = c.dom.enable_sdf == 1u;
     hipStream_t st = h->stream;
     const int key_bits = log2u(h->cfg.hash_size);
     const uint32_t max_iter = h->cfg.enable_apbf ? h->cfg.max_lod : h->cfg.solver_iterations;

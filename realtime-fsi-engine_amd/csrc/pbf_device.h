@@ -11,29 +11,49 @@
 
 namespace fsi {
 
-// Constants of one step: the reference's FluidConfigUBO
-// (include/fluid/PBFluids.h:21-61) plus the per-dispatch uniforms that
-// PBFluids::step sets (src/fluid/PBFluids.cpp:432-504).  Passed by value.
-struct StepConst {
-    float bmin[3], bmax[3], g[3], sub_dt;
-    float h, rho0, eps, wq;
-    float k_corr, n_corr, viscosity, damping;
-    float cohesion, irad, istr, w0_self;
-    float poly6, spiky, inv_rho0, lod_max_dist;
-    uint32_t hash_mask, n, enable_scorr, enable_visc;
-    uint32_t min_lod, max_lod, enable_apbf, enable_vort;
-    uint32_t interacting;
-    float ipos[3], cam[3], vort_eps;
-    // processing order: cells binned into a 128^3 Morton grid of cells shifted
-    // by dshift bits, starting at cell dcmin
+// Per-scene state of a step: the box, the scene's slice of the cell table,
+// its Morton bins and its SDF boundary.  The plain solver has one; the
+// batched solver one per scene.
+struct Domain {
+    float bmin[3], bmax[3];  // padded AABB (PBFluids::setBounds)
+    uint32_t hash_mask;      // hashSize - 1 (PBFluids.cpp:95)
+    uint32_t hash_base;      // first entry of this scene's cell table
+    // processing order: cells binned into a Morton grid of 2^axis_bits bins per
+    // axis, bin = (cell - dcmin) >> dshift (clamped), numbered from bin_base
     int32_t dcmin[3];
-    uint32_t dshift;
+    uint32_t dshift, axis_bits, bin_base;
     // SDF boundary (R32F grid, x fastest, voxel centres at origin + i*cell)
     uint32_t enable_sdf, sdf_mode;
     int32_t dims[3];
     float sdf_origin[3], sdf_cell, sdf_pad;
     const float* sdf_grid;
 };
+
+// Constants of one step: the reference's FluidConfigUBO
+// (include/fluid/PBFluids.h:21-61) plus the per-dispatch uniforms that
+// PBFluids::step sets (src/fluid/PBFluids.cpp:432-504).  Passed by value.
+struct StepConst {
+    float g[3], sub_dt;
+    float h, rho0, eps, wq;
+    float k_corr, n_corr, viscosity, damping;
+    float cohesion, irad, istr, w0_self;
+    float poly6, spiky, inv_rho0, lod_max_dist;
+    uint32_t n, enable_scorr, enable_visc;
+    uint32_t min_lod, max_lod, enable_apbf, enable_vort;
+    uint32_t interacting;
+    float ipos[3], cam[3], vort_eps;
+    Domain dom;                 // the single scene of the plain solver
+    const Domain* doms;         // batched: one per scene
+    const uint16_t* scene_of;   // batched: scene of particle id i (= of sorted slot i)
+};
+
+// The domain of particle / slot i (the batched solver keeps every scene's
+// particles in one contiguous id range, and its sorted slots in the same one).
+template <bool B>
+__device__ __forceinline__ const Domain& domain_of(const StepConst& c, uint32_t i) {
+    if constexpr (B) return c.doms[c.scene_of[i]];
+    else return c.dom;
+}
 
 struct V3 {
     float x, y, z;
@@ -64,8 +84,9 @@ __device__ __forceinline__ void cell_of(V3 x, float h, int32_t& cx, int32_t& cy,
     cz = (int32_t)floorf(x.z / h);
 }
 
-// 21-bit Morton code of a cell, on a 128^3 grid of (cell - dcmin) >> dshift
-// (clamped).  Used only to order the work; results do not depend on it.
+// Morton bin of a cell: (cell - dcmin) >> dshift per axis, clamped to
+// [0, 2^axis_bits), bits interleaved (axis_bits <= 7).  Used only to order
+// the work; results do not depend on it.
 __device__ __forceinline__ uint32_t spread7(uint32_t v) {
     v &= 0x7fu;
     v = (v | (v << 8)) & 0x0300f00fu;
@@ -73,15 +94,18 @@ __device__ __forceinline__ uint32_t spread7(uint32_t v) {
     v = (v | (v << 2)) & 0x09249249u;
     return v;
 }
-__device__ __forceinline__ uint32_t clamp_bin(int32_t c, int32_t c0, uint32_t shift) {
+__device__ __forceinline__ uint32_t clamp_bin(int32_t c, int32_t c0, uint32_t shift, uint32_t bits) {
     int32_t d = c - c0;
     d = d < 0 ? 0 : d;
     d >>= shift;
-    return (uint32_t)(d > 127 ? 127 : d);
+    const int32_t top = (1 << bits) - 1;
+    return (uint32_t)(d > top ? top : d);
 }
-__device__ __forceinline__ uint32_t morton_bin(const StepConst& c, int32_t cx, int32_t cy, int32_t cz) {
-    return spread7(clamp_bin(cx, c.dcmin[0], c.dshift)) | (spread7(clamp_bin(cy, c.dcmin[1], c.dshift)) << 1) |
-           (spread7(clamp_bin(cz, c.dcmin[2], c.dshift)) << 2);
+__device__ __forceinline__ uint32_t morton_bin(const Domain& d, int32_t cx, int32_t cy, int32_t cz) {
+    const uint32_t b = d.axis_bits;
+    return d.bin_base + (spread7(clamp_bin(cx, d.dcmin[0], d.dshift, b)) |
+                         (spread7(clamp_bin(cy, d.dcmin[1], d.dshift, b)) << 1) |
+                         (spread7(clamp_bin(cz, d.dcmin[2], d.dshift, b)) << 2));
 }
 constexpr uint32_t kOrderBins = 1u << 21;
 
@@ -147,7 +171,7 @@ __device__ __forceinline__ void sdf_taps(float tc, int32_t size, int32_t& i0, in
 __device__ __forceinline__ float lerp1(float t0, float t1, float a) { return (1.0f - a) * t0 + a * t1; }
 
 // querySDF (CS_Integrate.glsl:86-89).
-__device__ __forceinline__ float sdf_query(const StepConst& c, V3 p) {
+__device__ __forceinline__ float sdf_query(const Domain& c, V3 p) {
     const float tx = ((p.x - c.sdf_origin[0]) / c.sdf_cell + 0.5f) / (float)c.dims[0];
     const float ty = ((p.y - c.sdf_origin[1]) / c.sdf_cell + 0.5f) / (float)c.dims[1];
     const float tz = ((p.z - c.sdf_origin[2]) / c.sdf_cell + 0.5f) / (float)c.dims[2];
@@ -170,7 +194,7 @@ __device__ __forceinline__ float sdf_query(const StepConst& c, V3 p) {
 }
 
 // sdfGradient (CS_ApplyDeltaP.glsl:58-65).
-__device__ __forceinline__ V3 sdf_gradient(const StepConst& c, V3 p) {
+__device__ __forceinline__ V3 sdf_gradient(const Domain& c, V3 p) {
     const float e = c.sdf_cell;
     const float z = 0.0f;
     const float gx = sdf_query(c, mk(p.x + e, p.y + z, p.z + z)) - sdf_query(c, mk(p.x - e, p.y - z, p.z - z));
@@ -180,7 +204,7 @@ __device__ __forceinline__ V3 sdf_gradient(const StepConst& c, V3 p) {
 }
 
 // SDF projection shared by CS_ApplyDeltaP.glsl:108-121 and CS_Integrate.glsl:216-240.
-__device__ __forceinline__ bool sdf_project(const StepConst& c, V3& p, V3& nrm) {
+__device__ __forceinline__ bool sdf_project(const Domain& c, V3& p, V3& nrm) {
     const float pad = c.sdf_pad;
     const float d = sdf_query(c, p);
     const V3 g = sdf_gradient(c, p);

@@ -48,11 +48,11 @@ inline dim3 grid_for(uint32_t n) { return dim3((n + kBlock - 1) / kBlock); }
 // bucket, so every load stays inside it): up to five loads are in flight per
 // lane instead of one exposed latency per candidate.
 template <class F>
-__device__ __forceinline__ void scan_buckets(V3 x, float h, uint32_t mask, const int2* __restrict__ cells,
+__device__ __forceinline__ void scan_buckets(V3 x, float h, const Domain& dm, const int2* __restrict__ cells,
                                              const float4* __restrict__ X, uint32_t self, F&& f) {
     int32_t cx, cy, cz;
     cell_of(x, h, cx, cy, cz);
-    int2 se = cells[cell_hash(cx - 1, cy - 1, cz - 1, mask)];
+    int2 se = cells[dm.hash_base + cell_hash(cx - 1, cy - 1, cz - 1, dm.hash_mask)];
     auto visit = [&](int32_t k, float4 xk) {
         if ((uint32_t)k == self) return;
         const V3 r = sub(x, xyz(xk));
@@ -63,7 +63,7 @@ __device__ __forceinline__ void scan_buckets(V3 x, float h, uint32_t mask, const
         int2 nxt = make_int2(-1, -1);
         if (q < 26) {
             const int qn = q + 1;
-            nxt = cells[cell_hash(cx + qn % 3 - 1, cy + (qn / 3) % 3 - 1, cz + qn / 9 - 1, mask)];
+            nxt = cells[dm.hash_base + cell_hash(cx + qn % 3 - 1, cy + (qn / 3) % 3 - 1, cz + qn / 9 - 1, dm.hash_mask)];
         }
         // an empty bucket is (-1, -1): no iterations
         const int32_t last = se.y - 1;
@@ -94,13 +94,13 @@ constexpr uint32_t kFlushAt = 8;
 // column fills inside a bucket flushes alone.  Entries reach flush in visit
 // order, numbered from `first`.  Returns the lane's neighbour count.
 template <bool TAPS, class Flush>
-__device__ __forceinline__ uint32_t collect_flush(V3 x, float h, uint32_t mask, const int2* __restrict__ cells,
+__device__ __forceinline__ uint32_t collect_flush(V3 x, float h, const Domain& dm, const int2* __restrict__ cells,
                                                   const float4* __restrict__ X, uint32_t self, uint32_t* col,
                                                   Flush&& flush, uint32_t& ncand) {
     const float h2 = h * h;
     int32_t cx, cy, cz;
     cell_of(x, h, cx, cy, cz);
-    int2 se = cells[cell_hash(cx - 1, cy - 1, cz - 1, mask)];
+    int2 se = cells[dm.hash_base + cell_hash(cx - 1, cy - 1, cz - 1, dm.hash_mask)];
     uint32_t cnt = 0, total = 0;
     auto visit = [&](int32_t k, float4 xk) {
         if ((uint32_t)k == self) return;
@@ -120,7 +120,7 @@ __device__ __forceinline__ uint32_t collect_flush(V3 x, float h, uint32_t mask, 
         int2 nxt = make_int2(-1, -1);
         if (q < 26) {
             const int qn = q + 1;
-            nxt = cells[cell_hash(cx + qn % 3 - 1, cy + (qn / 3) % 3 - 1, cz + qn / 9 - 1, mask)];
+            nxt = cells[dm.hash_base + cell_hash(cx + qn % 3 - 1, cy + (qn / 3) % 3 - 1, cz + qn / 9 - 1, dm.hash_mask)];
         }
         const int32_t last = se.y - 1;
         for (int32_t k = se.x; k < se.y; k += 4) {
@@ -172,12 +172,12 @@ __device__ __forceinline__ void walk_list(uint32_t cnt, const float4* __restrict
 //          outside the support.
 // Pairs with r^2 > h^2 are skipped: in the reference they add exact zeros.
 template <int V, bool TAPS, class Body>
-__device__ __forceinline__ void gather(V3 xi, float h, uint32_t mask, const int2* __restrict__ cells,
+__device__ __forceinline__ void gather(V3 xi, float h, const Domain& dm, const int2* __restrict__ cells,
                                        const float4* __restrict__ X, uint32_t self, uint32_t* col, Body&& body,
                                        uint32_t& ncand, uint32_t& nnbr) {
     if constexpr (V == 0) {
         const float h2 = h * h;
-        scan_buckets(xi, h, mask, cells, X, self, [&](uint32_t k, float4 xk, V3 r, float r2) {
+        scan_buckets(xi, h, dm, cells, X, self, [&](uint32_t k, float4 xk, V3 r, float r2) {
             if (TAPS) ++ncand;
             if (r2 <= h2) {
                 if (TAPS) ++nnbr;
@@ -185,7 +185,7 @@ __device__ __forceinline__ void gather(V3 xi, float h, uint32_t mask, const int2
             }
         });
     } else {
-        nnbr = collect_flush<TAPS>(xi, h, mask, cells, X, self, col, [&](uint32_t, uint32_t cnt) {
+        nnbr = collect_flush<TAPS>(xi, h, dm, cells, X, self, col, [&](uint32_t, uint32_t cnt) {
             walk_list(cnt, X, [&](uint32_t e) { return col[e * kBlock]; }, [&](uint32_t j, float4 xj) {
                 const V3 r = sub(xi, xyz(xj));
                 body(j, xj, r, dot(r, r));
@@ -195,6 +195,7 @@ __device__ __forceinline__ void gather(V3 xi, float h, uint32_t mask, const int2
 }
 
 // CS_PredictAndHash.glsl:88-130
+template <bool B>
 __global__ __launch_bounds__(kBlock) void k_predict(StepConst c, const float4* __restrict__ pos,
                                                     float4* __restrict__ vel, float4* __restrict__ xstar,
                                                     uint32_t* __restrict__ keys) {
@@ -220,7 +221,8 @@ __global__ __launch_bounds__(kBlock) void k_predict(StepConst c, const float4* _
     xstar[id] = make_float4(xp.x, xp.y, xp.z, 0.0f);
     int32_t cx, cy, cz;
     cell_of(xp, c.h, cx, cy, cz);
-    keys[id] = cell_hash(cx, cy, cz, c.hash_mask);
+    const Domain& dom = domain_of<B>(c, id);
+    keys[id] = dom.hash_base + cell_hash(cx, cy, cz, dom.hash_mask);
 }
 
 // CS_BuildOffsets.glsl:55-78, clearMode = 1
@@ -242,6 +244,7 @@ __global__ __launch_bounds__(kBlock) void k_build_cells(const uint32_t* __restri
 // Gather the predicted state into key-sorted order (lambda starts at 0,
 // CS_PredictAndHash.glsl:121) and bin every slot by the Morton code of its
 // cell; the bins give the spatial processing order of the gather passes.
+template <bool B>
 __global__ __launch_bounds__(kBlock) void k_reorder(StepConst c, const uint32_t* __restrict__ perm,
                                                     const float4* __restrict__ xstar,
                                                     const float4* __restrict__ vel, float4* __restrict__ P,
@@ -257,7 +260,7 @@ __global__ __launch_bounds__(kBlock) void k_reorder(StepConst c, const uint32_t*
     vs[k] = vel[id];
     int32_t cx, cy, cz;
     cell_of(xyz(x), c.h, cx, cy, cz);
-    const uint32_t b = morton_bin(c, cx, cy, cz);
+    const uint32_t b = morton_bin(domain_of<B>(c, k), cx, cy, cz);
     bin_of[k] = b;
     bin_rank[k] = atomicAdd(&bins[b], 1u);
 }
@@ -294,7 +297,7 @@ __global__ __launch_bounds__(kBlock) void k_lod(StepConst c, const float4* __res
 // is also written to HBM, one row of kNbrCap slots per thread (entry r of
 // thread t at list[t * kNbrCap + r]), for the delta-p pass of the same
 // iteration, which sees the same positions.
-template <int V, bool TAPS>
+template <int V, bool TAPS, bool B>
 __global__ __launch_bounds__(kBlock) void k_lambda(StepConst c, const uint32_t* __restrict__ order,
                                                    const float4* __restrict__ P, float4* __restrict__ PL,
                                                    float* __restrict__ rho_out, const uint32_t* __restrict__ lod,
@@ -323,11 +326,11 @@ __global__ __launch_bounds__(kBlock) void k_lambda(StepConst c, const uint32_t* 
         gi = add(gi, gw);
     };
     if constexpr (V == 0) {
-        gather<0, TAPS>(xi, c.h, c.hash_mask, cells, P, k, nullptr, body, ncand, nnbr);
+        gather<0, TAPS>(xi, c.h, domain_of<B>(c, k), cells, P, k, nullptr, body, ncand, nnbr);
     } else {
         __shared__ uint32_t lst[kLdsCap * kBlock];
         uint32_t* col = &lst[threadIdx.x];
-        nnbr = collect_flush<TAPS>(xi, c.h, c.hash_mask, cells, P, k, col, [&](uint32_t first, uint32_t cnt) {
+        nnbr = collect_flush<TAPS>(xi, c.h, domain_of<B>(c, k), cells, P, k, col, [&](uint32_t first, uint32_t cnt) {
 #pragma unroll 1
             for (uint32_t e = 0; e < cnt; ++e)
                 if (first + e < (uint32_t)kNbrCap) list[(size_t)t * kNbrCap + first + e] = col[e * kBlock];
@@ -357,7 +360,7 @@ __global__ __launch_bounds__(kBlock) void k_lambda(StepConst c, const uint32_t* 
 // position to P (the SDF projection follows in k_apply_sdf).  In the two-phase
 // form the neighbours come from the list the lambda pass of this iteration
 // wrote (same positions, same order).
-template <int V, bool TAPS>
+template <int V, bool TAPS, bool B>
 __global__ __launch_bounds__(kBlock) void k_deltap(StepConst c, const uint32_t* __restrict__ order,
                                                    const float4* __restrict__ PL, float4* __restrict__ P,
                                                    const uint32_t* __restrict__ lod, const int2* __restrict__ cells,
@@ -378,7 +381,7 @@ __global__ __launch_bounds__(kBlock) void k_deltap(StepConst c, const uint32_t* 
         dp = add(dp, scl(gw, li + lj + sc));
     };
     auto single = [&]() {
-        scan_buckets(xi, c.h, c.hash_mask, cells, PL, k, [&](uint32_t, float4 xj, V3 r, float r2) {
+        scan_buckets(xi, c.h, domain_of<B>(c, k), cells, PL, k, [&](uint32_t, float4 xj, V3 r, float r2) {
             if (r2 <= h2) body(xj.w, r, r2);
         });
     };
@@ -430,6 +433,7 @@ __global__ __launch_bounds__(kBlock) void k_deltap(StepConst c, const uint32_t* 
 // The SDF projection of CS_ApplyDeltaP.glsl:108-121 on the corrected
 // positions, in place (a kernel of its own so that the seven trilinear
 // samples do not raise the register count of the delta-p gather).
+template <bool B>
 __global__ __launch_bounds__(kBlock) void k_apply_sdf(StepConst c, const uint32_t* __restrict__ order,
                                                       float4* __restrict__ P, const uint32_t* __restrict__ lod,
                                                       uint32_t iter) {
@@ -437,14 +441,16 @@ __global__ __launch_bounds__(kBlock) void k_apply_sdf(StepConst c, const uint32_
     if (t >= c.n) return;
     const uint32_t k = order[t];
     if (c.enable_apbf != 0u && lod[k] < iter) return;
+    const Domain& dom = domain_of<B>(c, k);
+    if (dom.enable_sdf != 1u) return;
     V3 p = xyz(P[k]);
     V3 nrm;
-    if (sdf_project(c, p, nrm)) P[k] = make_float4(p.x, p.y, p.z, 0.0f);
+    if (sdf_project(dom, p, nrm)) P[k] = make_float4(p.x, p.y, p.z, 0.0f);
 }
 
 // CS_Integrate.glsl:135-280.  Neighbour velocities come from the sorted
 // post-predict snapshot vs (the Jacobi form of the racy read at :171).
-template <int V>
+template <int V, bool B>
 __global__ __launch_bounds__(kBlock) void k_integrate(StepConst c, const uint32_t* __restrict__ order, const float4* __restrict__ P,
                                                       const float4* __restrict__ vs,
                                                       const uint32_t* __restrict__ perm,
@@ -454,6 +460,7 @@ __global__ __launch_bounds__(kBlock) void k_integrate(StepConst c, const uint32_
     const uint32_t t = xcd_gid();
     if (t >= c.n) return;
     const uint32_t k = order[t];
+    const Domain& dom = domain_of<B>(c, k);
     
     const uint32_t id = perm[k];
     const float4 op4 = pos[id];
@@ -470,7 +477,7 @@ __global__ __launch_bounds__(kBlock) void k_integrate(StepConst c, const uint32_
         V3 curl = mk(0.0f, 0.0f, 0.0f);
         uint32_t ncand = 0, nnbr = 0;
         __shared__ uint32_t lst[kLdsCap * kBlock];
-        gather<V, false>(xp, c.h, c.hash_mask, cells, P, k, &lst[threadIdx.x], [&](uint32_t j, float4, V3 r, float r2) {
+        gather<V, false>(xp, c.h, dom, cells, P, k, &lst[threadIdx.x], [&](uint32_t j, float4, V3 r, float r2) {
             const V3 vij = sub(xyz(vs[j]), old_vel);
             if (do_visc) {
                 const float w = poly6_r2(c, r2);
@@ -488,15 +495,15 @@ __global__ __launch_bounds__(kBlock) void k_integrate(StepConst c, const uint32_
         }
     }
     bool cxh = false, cyh = false, czh = false;
-    if (xp.x <= c.bmin[0]) { xp.x = c.bmin[0]; cxh = true; }
-    if (xp.x >= c.bmax[0]) { xp.x = c.bmax[0]; cxh = true; }
-    if (xp.y <= c.bmin[1]) { xp.y = c.bmin[1]; cyh = true; }
-    if (xp.y >= c.bmax[1]) { xp.y = c.bmax[1]; cyh = true; }
-    if (xp.z <= c.bmin[2]) { xp.z = c.bmin[2]; czh = true; }
-    if (xp.z >= c.bmax[2]) { xp.z = c.bmax[2]; czh = true; }
+    if (xp.x <= dom.bmin[0]) { xp.x = dom.bmin[0]; cxh = true; }
+    if (xp.x >= dom.bmax[0]) { xp.x = dom.bmax[0]; cxh = true; }
+    if (xp.y <= dom.bmin[1]) { xp.y = dom.bmin[1]; cyh = true; }
+    if (xp.y >= dom.bmax[1]) { xp.y = dom.bmax[1]; cyh = true; }
+    if (xp.z <= dom.bmin[2]) { xp.z = dom.bmin[2]; czh = true; }
+    if (xp.z >= dom.bmax[2]) { xp.z = dom.bmax[2]; czh = true; }
     bool hit = false;
     V3 nrm = mk(0.0f, 0.0f, 0.0f);
-    if (c.enable_sdf == 1u) hit = sdf_project(c, xp, nrm);
+    if (dom.enable_sdf == 1u) hit = sdf_project(dom, xp, nrm);
     const float inv_dt = (c.sub_dt > 0.0f) ? (1.0f / c.sub_dt) : 0.0f;
     V3 v = scl(sub(xp, old_pos), inv_dt);
     v = add(v, visc_delta);
@@ -515,7 +522,7 @@ __global__ __launch_bounds__(kBlock) void k_integrate(StepConst c, const uint32_
 }
 
 // CS_VorticityConfinement.glsl:133-167 (pass 1)
-template <int V>
+template <int V, bool B>
 __global__ __launch_bounds__(kBlock) void k_vorticity(StepConst c, const uint32_t* __restrict__ order, const float4* __restrict__ posS,
                                                       const float4* __restrict__ curl,
                                                       const uint32_t* __restrict__ perm,
@@ -528,7 +535,7 @@ __global__ __launch_bounds__(kBlock) void k_vorticity(StepConst c, const uint32_
     V3 eta = mk(0.0f, 0.0f, 0.0f);
     uint32_t ncand = 0, nnbr = 0;
     __shared__ uint32_t lst[kLdsCap * kBlock];
-    gather<V, false>(pi, c.h, c.hash_mask, cells, posS, k, &lst[threadIdx.x], [&](uint32_t j, float4, V3 r, float r2) {
+    gather<V, false>(pi, c.h, domain_of<B>(c, k), cells, posS, k, &lst[threadIdx.x], [&](uint32_t j, float4, V3 r, float r2) {
         const V3 gw = spiky(c, r, r2);
         const float om = len(xyz(curl[j]));
         eta = add(eta, scl(gw, om));
@@ -545,36 +552,52 @@ __global__ __launch_bounds__(kBlock) void k_vorticity(StepConst c, const uint32_
 }
 
 // CS_ContactFilter.glsl:68-83, as a deterministic compaction in id order.
+template <bool B>
 __global__ __launch_bounds__(kBlock) void k_contact_flag(StepConst c, const float4* __restrict__ pos, float band,
                                                          uint32_t* __restrict__ flags) {
     const uint32_t id = gid();
     if (id >= c.n) return;
-    const float d = sdf_query(c, xyz(pos[id]));
+    const Domain& dom = domain_of<B>(c, id);
+    if (dom.enable_sdf != 1u) {
+        flags[id] = 0u;
+        return;
+    }
+    const float d = sdf_query(dom, xyz(pos[id]));
     const float ad = d < 0.0f ? -d : d;
     flags[id] = ad < band ? 1u : 0u;
 }
 
+template <bool B>
 __global__ __launch_bounds__(kBlock) void k_contact_write(StepConst c, const float4* __restrict__ pos,
                                                           const float4* __restrict__ vel,
-                                                          const uint32_t* __restrict__ slots, float band,
+                                                          const uint32_t* __restrict__ flags,
+                                                          const uint32_t* __restrict__ slots,
                                                           float4* __restrict__ out, uint32_t* __restrict__ count) {
     const uint32_t id = gid();
     if (id >= c.n) return;
-    const float4 p4 = pos[id];
-    const float d = sdf_query(c, xyz(p4));
-    const float ad = d < 0.0f ? -d : d;
-    if (ad < band) {
+    const bool hit = flags[id] != 0u;
+    if (hit) {
         const uint32_t s = slots[id];
+        const float4 p4 = pos[id];
         out[2 * (size_t)s] = make_float4(p4.x, p4.y, p4.z, 0.0f);
         out[2 * (size_t)s + 1] = vel[id];
     }
-    if (id == c.n - 1) count[0] = slots[id] + (ad < band ? 1u : 0u);
+    if (id == c.n - 1) count[0] = slots[id] + (hit ? 1u : 0u);
 }
 
 }  // namespace
 
+#define FSI_B_SWITCH(b, CALL) \
+    if (b) {                      \
+        CALL(true);               \
+    } else {                      \
+        CALL(false);              \
+    }
+
 hipError_t launch_predict(const StepConst& c, DeviceState& d, hipStream_t st) {
-    hipLaunchKernelGGL(k_predict, grid_for(c.n), dim3(kBlock), 0, st, c, d.pos, d.vel, d.xstar, d.keys);
+#define L(B) hipLaunchKernelGGL((k_predict<B>), grid_for(c.n), dim3(kBlock), 0, st, c, d.pos, d.vel, d.xstar, d.keys);
+    FSI_B_SWITCH(d.batched, L)
+#undef L
     return hipGetLastError();
 }
 
@@ -586,8 +609,11 @@ hipError_t launch_cells(const StepConst& c, DeviceState& d, hipStream_t st) {
 
 hipError_t launch_reorder(const StepConst& c, DeviceState& d, hipStream_t st) {
     hipLaunchKernelGGL(k_fill_u32, grid_for(kOrderBins), dim3(kBlock), 0, st, d.bins, kOrderBins, 0u);
-    hipLaunchKernelGGL(k_reorder, grid_for(c.n), dim3(kBlock), 0, st, c, d.perm, d.xstar, d.vel, d.P, d.PL, d.vs,
-                       d.bins, d.bin_of, d.bin_rank);
+#define L(B)                                                                                                    \
+    hipLaunchKernelGGL((k_reorder<B>), grid_for(c.n), dim3(kBlock), 0, st, c, d.perm, d.xstar, d.vel, d.P, d.PL, \
+                       d.vs, d.bins, d.bin_of, d.bin_rank);
+    FSI_B_SWITCH(d.batched, L)
+#undef L
     hipError_t e = scan_exclusive_u32(d.bins, d.bins, kOrderBins, d.sort.partial, st);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_order, grid_for(c.n), dim3(kBlock), 0, st, c.n, d.bins, d.bin_of, d.bin_rank, d.order);
@@ -599,64 +625,75 @@ hipError_t launch_lod(const StepConst& c, DeviceState& d, hipStream_t st) {
     return hipGetLastError();
 }
 
-#define FSI_VARIANT_SWITCH(v, CALL) \
-    switch (v) {                        \
-        case 1: CALL(1); break;         \
-        default: CALL(0); break;        \
+// gather form V (tuning knob) x taps x batched
+#define FSI_GATHER_SWITCH(CALL)                         \
+    if (d.variant == 1) {                               \
+        if (d.batched) { CALL(1, true); } else { CALL(1, false); } \
+    } else {                                            \
+        if (d.batched) { CALL(0, true); } else { CALL(0, false); } \
     }
 
 hipError_t launch_lambda(const StepConst& c, DeviceState& d, uint32_t iter, bool taps, hipStream_t st) {
-#define L(V)                                                                                                   \
-    if (taps)                                                                                                  \
-        hipLaunchKernelGGL((k_lambda<V, true>), grid_for(c.n), dim3(kBlock), 0, st, c, d.order, d.P, d.PL, d.rho, d.lod, \
-                           d.cells, iter, d.list_count, d.list, d.cand, d.nbr);                                 \
-    else                                                                                                       \
-        hipLaunchKernelGGL((k_lambda<V, false>), grid_for(c.n), dim3(kBlock), 0, st, c, d.order, d.P, d.PL, d.rho,      \
-                           d.lod, d.cells, iter, d.list_count, d.list, d.cand, d.nbr);
-    FSI_VARIANT_SWITCH(d.variant, L)
+#define L(V, B)                                                                                                 \
+    if (taps)                                                                                                   \
+        hipLaunchKernelGGL((k_lambda<V, true, B>), grid_for(c.n), dim3(kBlock), 0, st, c, d.order, d.P, d.PL,   \
+                           d.rho, d.lod, d.cells, iter, d.list_count, d.list, d.cand, d.nbr);                   \
+    else                                                                                                        \
+        hipLaunchKernelGGL((k_lambda<V, false, B>), grid_for(c.n), dim3(kBlock), 0, st, c, d.order, d.P, d.PL,  \
+                           d.rho, d.lod, d.cells, iter, d.list_count, d.list, d.cand, d.nbr);
+    FSI_GATHER_SWITCH(L)
 #undef L
     return hipGetLastError();
 }
 
 hipError_t launch_deltap(const StepConst& c, DeviceState& d, uint32_t iter, bool taps, hipStream_t st) {
-#define L(V)                                                                                                    \
+#define L(V, B)                                                                                                 \
     if (taps)                                                                                                   \
-        hipLaunchKernelGGL((k_deltap<V, true>), grid_for(c.n), dim3(kBlock), 0, st, c, d.order, d.PL, d.P, d.lod, d.cells, \
-                           iter, d.list_count, d.list, d.dp_tap);                                               \
+        hipLaunchKernelGGL((k_deltap<V, true, B>), grid_for(c.n), dim3(kBlock), 0, st, c, d.order, d.PL, d.P,   \
+                           d.lod, d.cells, iter, d.list_count, d.list, d.dp_tap);                               \
     else                                                                                                        \
-        hipLaunchKernelGGL((k_deltap<V, false>), grid_for(c.n), dim3(kBlock), 0, st, c, d.order, d.PL, d.P, d.lod,        \
-                           d.cells, iter, d.list_count, d.list, d.dp_tap);
-    FSI_VARIANT_SWITCH(d.variant, L)
+        hipLaunchKernelGGL((k_deltap<V, false, B>), grid_for(c.n), dim3(kBlock), 0, st, c, d.order, d.PL, d.P,  \
+                           d.lod, d.cells, iter, d.list_count, d.list, d.dp_tap);
+    FSI_GATHER_SWITCH(L)
 #undef L
-    if (c.enable_sdf == 1u)
-        hipLaunchKernelGGL(k_apply_sdf, grid_for(c.n), dim3(kBlock), 0, st, c, d.order, d.P, d.lod, iter);
+    if (d.any_sdf) {
+#define L(B) hipLaunchKernelGGL((k_apply_sdf<B>), grid_for(c.n), dim3(kBlock), 0, st, c, d.order, d.P, d.lod, iter);
+        FSI_B_SWITCH(d.batched, L)
+#undef L
+    }
     return hipGetLastError();
 }
 
 hipError_t launch_integrate(const StepConst& c, DeviceState& d, hipStream_t st) {
-#define L(V)                                                                                                     \
-    hipLaunchKernelGGL((k_integrate<V>), grid_for(c.n), dim3(kBlock), 0, st, c, d.order, d.P, d.vs, d.perm, d.pos, d.vel, \
-                       d.cells, d.curl, d.posS);
-    FSI_VARIANT_SWITCH(d.variant, L)
+#define L(V, B)                                                                                                   \
+    hipLaunchKernelGGL((k_integrate<V, B>), grid_for(c.n), dim3(kBlock), 0, st, c, d.order, d.P, d.vs, d.perm, d.pos, \
+                       d.vel, d.cells, d.curl, d.posS);
+    FSI_GATHER_SWITCH(L)
 #undef L
     return hipGetLastError();
 }
 
 hipError_t launch_vorticity(const StepConst& c, DeviceState& d, hipStream_t st) {
-#define L(V) \
-    hipLaunchKernelGGL((k_vorticity<V>), grid_for(c.n), dim3(kBlock), 0, st, c, d.order, d.posS, d.curl, d.perm, d.vel, d.cells);
-    FSI_VARIANT_SWITCH(d.variant, L)
+#define L(V, B)                                                                                                   \
+    hipLaunchKernelGGL((k_vorticity<V, B>), grid_for(c.n), dim3(kBlock), 0, st, c, d.order, d.posS, d.curl, d.perm, \
+                       d.vel, d.cells);
+    FSI_GATHER_SWITCH(L)
 #undef L
     return hipGetLastError();
 }
 
 hipError_t launch_contacts(const StepConst& c, DeviceState& d, float band, float4* out, uint32_t* count,
                            hipStream_t st) {
-    hipLaunchKernelGGL(k_contact_flag, grid_for(c.n), dim3(kBlock), 0, st, c, d.pos, band, d.flags);
-    hipError_t e = scan_exclusive_u32(d.flags, d.flags, c.n, d.sort.partial, st);
+#define L(B) hipLaunchKernelGGL((k_contact_flag<B>), grid_for(c.n), dim3(kBlock), 0, st, c, d.pos, band, d.flags);
+    FSI_B_SWITCH(d.batched, L)
+#undef L
+    hipError_t e = scan_exclusive_u32(d.flags, d.slots, c.n, d.sort.partial, st);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_contact_write, grid_for(c.n), dim3(kBlock), 0, st, c, d.pos, d.vel, d.flags, band, out,
-                       count);
+#define L(B)                                                                                                   \
+    hipLaunchKernelGGL((k_contact_write<B>), grid_for(c.n), dim3(kBlock), 0, st, c, d.pos, d.vel, d.flags,     \
+                       d.slots, out, count);
+    FSI_B_SWITCH(d.batched, L)
+#undef L
     return hipGetLastError();
 }
 

@@ -57,7 +57,8 @@ struct DeviceState {
     float4* dp_tap = nullptr;  // sorted: last delta-p (taps only)
     uint32_t* cand = nullptr;  // sorted: candidates of the last lambda pass (taps only)
     uint32_t* nbr = nullptr;   // sorted: neighbours of the last lambda pass (taps only)
-    uint32_t* flags = nullptr; // n: contact flags / scan
+    uint32_t* flags = nullptr; // n: contact flags
+    uint32_t* slots = nullptr; // n: exclusive scan of the contact flags
     uint32_t* bins = nullptr;  // kOrderBins: Morton-bin counts, then their exclusive scan
     uint32_t* bin_of = nullptr;  // sorted slot -> Morton bin
     uint32_t* bin_rank = nullptr;  // sorted slot -> rank inside its bin
@@ -66,6 +67,8 @@ struct DeviceState {
     uint32_t* list = nullptr;  // n x kNbrCap: neighbour slots, one row per thread
     SortScratch sort{};
     int variant = 1;           // gather form (tuning knob, see gather() in pbf_kernels.hip)
+    bool batched = false;      // several scenes (StepConst::doms / scene_of)
+    bool any_sdf = false;      // some domain has an SDF boundary
 };
 
 hipError_t launch_predict(const StepConst& c, DeviceState& d, hipStream_t st);
