@@ -273,8 +273,63 @@ enum fsi_pbf_knob {
 int fsi_pbf_set_tuning(fsi_pbf* h, int knob, int value);
 int fsi_pbf_get_tuning(fsi_pbf* h, int knob, int* value);
 
+/* Autotuner: times `frames` frames of the current state under every
+ * combination of the knobs above (restoring the particle state after each
+ * trial), keeps the fastest and returns it.  No counterpart in the reference. */
+int fsi_pbf_autotune(fsi_pbf* h, uint32_t frames, int* gather, int* digit_bits);
+
+/* Replay each frame from a HIP graph captured on first use (re-captured when
+ * parameters, buffers or knobs change).  Per-pass timing is not recorded in
+ * this mode.  No counterpart in the reference. */
+int fsi_pbf_set_graph(fsi_pbf* h, int enable);
+
 /* Device pointer of the HIP stream the handle enqueues on (hipStream_t). */
 int fsi_pbf_get_stream(fsi_pbf* h, void** stream);
+
+/* ------------------------------------------------------------------------- */
+/* Multi-scene solver: many independent scenes, one launch per pass            */
+/* (no counterpart in the reference, which runs one scene per PBFluids; every */
+/* scene follows PBFluids::step, src/fluid/PBFluids.cpp:323-517)               */
+/* ------------------------------------------------------------------------- */
+
+typedef struct fsi_scene_desc {
+    uint32_t particle_offset; /* scenes tile the particle array in order      */
+    uint32_t particle_count;
+    uint32_t hash_size;       /* power of two: this scene's cell table        */
+    float bounds_min[3];      /* the box particles are kept in, i.e. the      */
+    float bounds_max[3];      /* bounds after PBFluids::setBounds' padding;   */
+                              /* damping is the shared config's               */
+    const float* sdf_grid;    /* NULL: no boundary; else x fastest, copied    */
+    int32_t sdf_dims[3];
+    float sdf_origin[3];
+    float sdf_cell;
+    float sdf_padding;
+    uint32_t sdf_containment;
+} fsi_scene_desc;
+
+typedef struct fsi_batch fsi_batch; /* opaque handle */
+
+/* Shared solver parameters of every scene (cfg->hash_size, bounds and
+ * particle_count are taken per scene). */
+int fsi_batch_create(const fsi_fluid_config* cfg, fsi_batch** out);
+int fsi_batch_destroy(fsi_batch* b);
+/* particles: total records, scene s owning [offset_s, offset_s + count_s). */
+int fsi_batch_set_scenes(fsi_batch* b, const fsi_particle* particles, uint32_t total, const fsi_scene_desc* scenes,
+                         uint32_t nscenes);
+/* One frame of every scene, enqueued on the handle's stream. */
+int fsi_batch_step(fsi_batch* b);
+int fsi_batch_sync(fsi_batch* b);
+int fsi_batch_read_particles(fsi_batch* b, fsi_particle* out, uint32_t cap, uint32_t* n);
+/* Scene s of the last substep, numbered as a single-scene handle numbers it:
+ * keys inside the scene's table, ids from the scene's first particle. */
+int fsi_batch_read_grid(fsi_batch* b, uint32_t scene, uint32_t* keys, uint32_t* ids, int32_t* cells);
+int fsi_batch_set_graph(fsi_batch* b, int enable);
+int fsi_batch_set_tuning(fsi_batch* b, int knob, int value);
+int fsi_batch_get_tuning(fsi_batch* b, int knob, int* value);
+int fsi_batch_autotune(fsi_batch* b, uint32_t frames, int* gather, int* digit_bits);
+int fsi_batch_set_timing(fsi_batch* b, int enable);
+int fsi_batch_timing_reset(fsi_batch* b);
+int fsi_batch_read_timing(fsi_batch* b, fsi_pbf_timing* out);
 
 /* ------------------------------------------------------------------------- */
 /* Host-side helpers for scenes and boundaries                                */

@@ -1,0 +1,493 @@
+// pbf_core.hip — buffers, frame sequence, timing, tuning and graph capture
+// shared by the single-scene and multi-scene solver handles.
+#include "pbf_core.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace fsi {
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+int fail(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+
+const char* last_error() { return g_last_error.c_str(); }
+
+bool is_pow2(uint32_t x) { return x != 0 && (x & (x - 1)) == 0; }
+
+int log2u(uint32_t x) {
+    int b = 0;
+    while ((1ull << b) < (unsigned long long)x) ++b;
+    return b;
+}
+
+int check_config(const fsi_fluid_config* cfg) {
+    if (!cfg) return fail(FSI_ERR_INVALID_ARG, "config is NULL");
+    if (!is_pow2(cfg->hash_size))
+        return fail(FSI_ERR_INVALID_ARG, "hash_size must be a power of two (hashMask = hashSize - 1)");
+    if (cfg->hash_size > (1u << 30)) return fail(FSI_ERR_INVALID_ARG, "hash_size above 2^30");
+    if (cfg->substep_iterations == 0) return fail(FSI_ERR_INVALID_ARG, "substep_iterations must be >= 1");
+    if (!(cfg->h > 0.0f) || !std::isfinite(cfg->h)) return fail(FSI_ERR_INVALID_ARG, "h must be positive");
+    if (!(cfg->rho0 != 0.0f) || !std::isfinite(cfg->rho0)) return fail(FSI_ERR_INVALID_ARG, "rho0 must be nonzero");
+    if (cfg->enable_apbf && cfg->max_lod > 1024u) return fail(FSI_ERR_INVALID_ARG, "max_lod above 1024");
+    if (!cfg->enable_apbf && cfg->solver_iterations > 1024u)
+        return fail(FSI_ERR_INVALID_ARG, "solver_iterations above 1024");
+    return FSI_OK;
+}
+
+// calcPoly6Kernel of src/fluid/Kernels.cpp:9-27 (reached through
+// calcSCorrKernel, :216-219) in FP32: the s_corr reference value wq.
+static float wq_of(float dq, float h) {
+    const float pi = 3.14159265358979323846f;
+    const float coeff = 315.0f / (64.0f * pi);
+    const float h2 = h * h;
+    const float r2 = dq * dq + 0.0f * 0.0f + 0.0f * 0.0f;
+    if (r2 > h2) return 0.0f;
+    const float h4 = h2 * h2;
+    const float h9 = h4 * h4 * h;
+    const float diff = h2 - r2;
+    const float diff3 = diff * diff * diff;
+    return (coeff / h9) * diff3;
+}
+
+void pack_constants(const fsi_fluid_config& p, const float bmin[3], const float bmax[3], float damping, uint32_t n,
+                    fsi_pbf_constants& u) {
+    std::memset(&u, 0, sizeof(u));
+    for (int a = 0; a < 3; ++a) {
+        u.bounds_min[a] = bmin[a];
+        u.bounds_max[a] = bmax[a];
+        u.gravity_dt[a] = p.gravity[a];
+    }
+    u.gravity_dt[3] = p.dt / (float)p.substep_iterations;
+    u.h = p.h;
+    u.rho0 = p.rho0;
+    u.eps = p.eps;
+    u.wq = wq_of(p.delta_q * p.h, p.h);
+    u.k_corr = p.k_corr;
+    u.n_corr = p.n_corr;
+    u.viscosity = p.viscosity;
+    u.bound_damping = damping;
+    u.hash_size = p.hash_size;
+    u.particle_count = n;
+    u.enable_scorr = p.enable_scorr ? 1u : 0u;
+    u.enable_viscosity = p.enable_viscosity ? 1u : 0u;
+    u.cohesion_strength = p.cohesion_strength;
+    u.interaction_radius = p.interaction_radius;
+    u.interaction_strength = p.interaction_strength;
+    const float PI = 3.14159265358979323846f;
+    const float hh = p.h;
+    const float h2 = hh * hh;
+    const float h3 = h2 * hh;
+    const float h6 = h3 * h3;
+    const float h9 = h6 * h3;
+    u.w0_self = (315.0f / (64.0f * PI)) / h3;
+    u.poly6_coeff = 315.0f / (64.0f * PI * h9);
+    u.spiky_coeff = 45.0f / (PI * h6);
+    u.inv_rho0 = 1.0f / p.rho0;
+    u.hash_mask = p.hash_size - 1u;
+    u.min_lod = p.enable_apbf ? p.min_lod : p.solver_iterations;
+    u.max_lod = p.enable_apbf ? p.max_lod : p.solver_iterations;
+    u.lod_max_dist = p.lod_max_dist;
+    u.enable_apbf = p.enable_apbf ? 1u : 0u;
+}
+
+void set_binning(Domain& d, const float bmin[3], const float bmax[3], float h, uint32_t axis_bits,
+                 uint32_t bin_base) {
+    auto cell_clamped = [&](float x) {
+        const double v = std::floor((double)x / (double)h);
+        return (int64_t)std::max(-1.0e9, std::min(1.0e9, std::isfinite(v) ? v : 0.0));
+    };
+    int64_t span = 1;
+    for (int a = 0; a < 3; ++a) {
+        const int64_t lo = cell_clamped(bmin[a]) - 2;
+        const int64_t hi = cell_clamped(bmax[a]) + 2;
+        d.dcmin[a] = (int32_t)lo;
+        span = std::max<int64_t>(span, hi - lo + 1);
+    }
+    uint32_t shift = 0;
+    while ((span >> shift) > ((int64_t)1 << axis_bits) && shift < 31) ++shift;
+    d.dshift = shift;
+    d.axis_bits = axis_bits;
+    d.bin_base = bin_base;
+}
+
+StepConst step_const_common(const fsi_fluid_config& cfg, const fsi_pbf_constants& u, uint32_t n, bool interacting,
+                            const float ipos[3], const float cam[3]) {
+    StepConst c{};
+    for (int a = 0; a < 3; ++a) {
+        c.g[a] = u.gravity_dt[a];
+        c.ipos[a] = ipos[a];
+        c.cam[a] = cam[a];
+    }
+    c.sub_dt = u.gravity_dt[3];
+    c.h = u.h;
+    c.rho0 = u.rho0;
+    c.eps = u.eps;
+    c.wq = u.wq;
+    c.k_corr = u.k_corr;
+    c.n_corr = u.n_corr;
+    c.viscosity = u.viscosity;
+    c.damping = u.bound_damping;
+    c.cohesion = u.cohesion_strength;
+    c.irad = u.interaction_radius;
+    c.istr = u.interaction_strength;
+    c.w0_self = u.w0_self;
+    c.poly6 = u.poly6_coeff;
+    c.spiky = u.spiky_coeff;
+    c.inv_rho0 = u.inv_rho0;
+    c.lod_max_dist = u.lod_max_dist;
+    c.n = n;
+    c.enable_scorr = u.enable_scorr;
+    c.enable_visc = u.enable_viscosity;
+    c.min_lod = u.min_lod;
+    c.max_lod = u.max_lod;
+    c.enable_apbf = u.enable_apbf;
+    c.enable_vort = cfg.enable_vorticity ? 1u : 0u;
+    c.interacting = interacting ? 1u : 0u;
+    c.vort_eps = cfg.vorticity_epsilon;
+    return c;
+}
+
+int core_init(Core& c) {
+    FSI_HIP_TRY(hipGetDevice(&c.device));
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, c.device) == hipSuccess && std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(FSI_ERR_NO_DEVICE, std::string("device is ") + prop.gcnArchName + ", this library is built for gfx950");
+    FSI_HIP_TRY(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+    return FSI_OK;
+}
+
+void core_invalidate_graph(Core& c) {
+    if (c.exec) (void)hipGraphExecDestroy(c.exec);
+    if (c.graph) (void)hipGraphDestroy(c.graph);
+    c.exec = nullptr;
+    c.graph = nullptr;
+    c.graph_valid = false;
+    c.graph_key.clear();
+}
+
+void core_destroy(Core& c) {
+    (void)hipSetDevice(c.device);
+    if (c.stream) (void)hipStreamSynchronize(c.stream);
+    core_invalidate_graph(c);
+    DeviceState& d = c.d;
+    dev_free(&d.pos);
+    dev_free(&d.vel);
+    dev_free(&d.xstar);
+    dev_free(&d.keys);
+    dev_free(&d.cells);
+    dev_free(&d.P);
+    dev_free(&d.PL);
+    dev_free(&d.vs);
+    dev_free(&d.curl);
+    dev_free(&d.posS);
+    dev_free(&d.rho);
+    dev_free(&d.lod);
+    dev_free(&d.dp_tap);
+    dev_free(&d.cand);
+    dev_free(&d.nbr);
+    dev_free(&d.flags);
+    dev_free(&d.slots);
+    dev_free(&d.bins);
+    dev_free(&d.bin_of);
+    dev_free(&d.bin_rank);
+    dev_free(&d.order);
+    dev_free(&d.list_count);
+    dev_free(&d.list);
+    dev_free(&d.sort.keys[0]);
+    dev_free(&d.sort.keys[1]);
+    dev_free(&d.sort.ids[0]);
+    dev_free(&d.sort.ids[1]);
+    dev_free(&d.sort.hist);
+    dev_free(&d.sort.partial);
+    for (const TimedPair& t : c.pending) {
+        (void)hipEventDestroy(t.a);
+        (void)hipEventDestroy(t.b);
+    }
+    c.pending.clear();
+    for (hipEvent_t e : c.event_pool) (void)hipEventDestroy(e);
+    c.event_pool.clear();
+    if (c.stream) (void)hipStreamDestroy(c.stream);
+    c.stream = nullptr;
+}
+
+hipError_t core_alloc_cells(Core& c, uint32_t entries) {
+    if (c.cells_capacity != entries || !c.d.cells) {
+        hipError_t e = dev_alloc(&c.d.cells, entries);
+        if (e != hipSuccess) return e;
+        c.cells_capacity = entries;
+        core_invalidate_graph(c);
+    }
+    c.d.hash_size = entries;
+    std::vector<int32_t> neg(2 * (size_t)entries, -1);
+    return hipMemcpyAsync(c.d.cells, neg.data(), neg.size() * sizeof(int32_t), hipMemcpyHostToDevice, c.stream);
+}
+
+hipError_t core_alloc_particles(Core& c, uint32_t n) {
+    DeviceState& d = c.d;
+    hipError_t e = hipSuccess;
+#define ALLOC(ptr, count) \
+    if ((e = dev_alloc(&(ptr), (count))) != hipSuccess) return e;
+    ALLOC(d.pos, n);
+    ALLOC(d.vel, n);
+    ALLOC(d.xstar, n);
+    ALLOC(d.keys, n);
+    ALLOC(d.P, n);
+    ALLOC(d.PL, n);
+    ALLOC(d.vs, n);
+    ALLOC(d.curl, n);
+    ALLOC(d.posS, n);
+    ALLOC(d.rho, n);
+    ALLOC(d.lod, n);
+    ALLOC(d.dp_tap, n);
+    ALLOC(d.cand, n);
+    ALLOC(d.nbr, n);
+    ALLOC(d.flags, n);
+    ALLOC(d.slots, n);
+    ALLOC(d.bins, kOrderBins);
+    ALLOC(d.bin_of, n);
+    ALLOC(d.bin_rank, n);
+    ALLOC(d.order, n);
+    ALLOC(d.list_count, n);
+    ALLOC(d.list, (size_t)kNbrCap * n);
+    ALLOC(d.sort.keys[0], n);
+    ALLOC(d.sort.keys[1], n);
+    ALLOC(d.sort.ids[0], n);
+    ALLOC(d.sort.ids[1], n);
+    const size_t hist = sort_hist_entries(n, kSortMaxDigitBits);
+    ALLOC(d.sort.hist, hist);
+    const size_t parts =
+        std::max({scan_partials_needed(hist), scan_partials_needed(n), scan_partials_needed(kOrderBins)});
+    ALLOC(d.sort.partial, parts);
+#undef ALLOC
+    c.capacity = n;
+    d.n = n;
+    core_invalidate_graph(c);
+    return hipSuccess;
+}
+
+namespace {
+
+hipEvent_t take_event(Core& c) {
+    if (!c.event_pool.empty()) {
+        hipEvent_t e = c.event_pool.back();
+        c.event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// Records a HIP event pair around the launches of one pass (on the handle's
+// stream) while timing is on and no graph is being captured.
+struct PassTimer {
+    Core& c;
+    bool on;
+    int pass;
+    hipEvent_t a = nullptr, b = nullptr;
+    PassTimer(Core& cc, bool enabled, int p) : c(cc), on(enabled), pass(p) {
+        if (!on) return;
+        a = take_event(c);
+        b = take_event(c);
+        if (a && b) (void)hipEventRecord(a, c.stream);
+    }
+    ~PassTimer() {
+        if (!on || !a || !b) return;
+        (void)hipEventRecord(b, c.stream);
+        c.pending.push_back(TimedPair{pass, a, b});
+    }
+};
+
+// One frame of PBFluids::step (src/fluid/PBFluids.cpp:344-505).
+int enqueue_frame(Core& c, const StepConst& k, const FramePlan& plan, bool timed) {
+    DeviceState& d = c.d;
+    hipStream_t st = c.stream;
+    for (uint32_t s = 0; s < plan.substeps; ++s) {
+        {
+            PassTimer t(c, timed, FSI_PASS_PREDICT);
+            FSI_HIP_TRY(launch_predict(k, d, st));
+        }
+        {
+            PassTimer t(c, timed, FSI_PASS_SORT);
+            FSI_HIP_TRY(radix_sort_pairs(d.keys, d.n, plan.key_bits, c.sort_digit_bits, d.sort, &d.skeys, &d.perm, st));
+        }
+        {
+            PassTimer t(c, timed, FSI_PASS_TABLES);
+            FSI_HIP_TRY(launch_cells(k, d, st));
+            FSI_HIP_TRY(launch_reorder(k, d, st));
+        }
+        if (k.enable_apbf) {
+            PassTimer t(c, timed, FSI_PASS_LOD);
+            FSI_HIP_TRY(launch_lod(k, d, st));
+        }
+        for (uint32_t it = 1; it <= plan.max_iter; ++it) {
+            {
+                PassTimer t(c, timed, FSI_PASS_LAMBDA);
+                FSI_HIP_TRY(launch_lambda(k, d, it, c.taps, st));
+            }
+            {
+                PassTimer t(c, timed, FSI_PASS_DELTAP);
+                FSI_HIP_TRY(launch_deltap(k, d, it, c.taps, st));
+            }
+        }
+        {
+            PassTimer t(c, timed, FSI_PASS_INTEGRATE);
+            FSI_HIP_TRY(launch_integrate(k, d, st));
+        }
+    }
+    if (plan.vorticity) {
+        PassTimer t(c, timed, FSI_PASS_VORTICITY);
+        FSI_HIP_TRY(launch_vorticity(k, d, st));
+    }
+    return FSI_OK;
+}
+
+}  // namespace
+
+int core_frame(Core& c, const StepConst& k, const FramePlan& plan) {
+    if (c.d.n == 0) return FSI_OK;
+    int rc = FSI_OK;
+    if (!c.use_graph) {
+        rc = enqueue_frame(c, k, plan, c.timing);
+    } else {
+        // key: everything the launches of a frame depend on
+        std::vector<unsigned char> key(sizeof(StepConst) + sizeof(FramePlan) + 4 * sizeof(int));
+        std::memcpy(key.data(), &k, sizeof(StepConst));
+        std::memcpy(key.data() + sizeof(StepConst), &plan, sizeof(FramePlan));
+        const int knobs[4] = {c.d.variant, c.sort_digit_bits, c.taps ? 1 : 0, c.d.any_sdf ? 1 : 0};
+        std::memcpy(key.data() + sizeof(StepConst) + sizeof(FramePlan), knobs, sizeof(knobs));
+        if (!c.graph_valid || key != c.graph_key) {
+            core_invalidate_graph(c);
+            FSI_HIP_TRY(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+            rc = enqueue_frame(c, k, plan, false);
+            hipGraph_t g = nullptr;
+            const hipError_t e = hipStreamEndCapture(c.stream, &g);
+            if (rc != FSI_OK) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc;
+            }
+            if (e != hipSuccess) return fail(FSI_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+            c.graph = g;
+            FSI_HIP_TRY(hipGraphInstantiate(&c.exec, c.graph, nullptr, nullptr, 0));
+            c.graph_valid = true;
+            c.graph_key = key;
+        }
+        FSI_HIP_TRY(hipGraphLaunch(c.exec, c.stream));
+    }
+    if (rc != FSI_OK) return rc;
+    c.acc.substeps += plan.substeps;
+    c.acc.frames++;
+    c.stepped = true;
+    return FSI_OK;
+}
+
+int core_autotune(Core& c, const StepConst& k, const FramePlan& plan, uint32_t frames, int* gather, int* digit_bits) {
+    const uint32_t n = c.d.n;
+    if (n == 0 || frames == 0) {
+        if (gather) *gather = c.d.variant;
+        if (digit_bits) *digit_bits = c.sort_digit_bits;
+        return FSI_OK;
+    }
+    float4 *pos0 = nullptr, *vel0 = nullptr;
+    FSI_HIP_TRY(dev_alloc(&pos0, n));
+    FSI_HIP_TRY(dev_alloc(&vel0, n));
+    FSI_HIP_TRY(hipMemcpyAsync(pos0, c.d.pos, sizeof(float4) * n, hipMemcpyDeviceToDevice, c.stream));
+    FSI_HIP_TRY(hipMemcpyAsync(vel0, c.d.vel, sizeof(float4) * n, hipMemcpyDeviceToDevice, c.stream));
+    const bool timing = c.timing, graph = c.use_graph;
+    const fsi_pbf_timing acc = c.acc;
+    c.timing = false;
+    c.use_graph = false;
+    hipEvent_t ea = nullptr, eb = nullptr;
+    FSI_HIP_TRY(hipEventCreate(&ea));
+    FSI_HIP_TRY(hipEventCreate(&eb));
+    float best = 1e30f;
+    int best_g = c.d.variant, best_b = c.sort_digit_bits;
+    int rc = FSI_OK;
+    const int bits[] = {6, 8, 10};
+    for (int g = 0; g <= 1 && rc == FSI_OK; ++g)
+        for (int b : bits) {
+            c.d.variant = g;
+            c.sort_digit_bits = b;
+            if ((rc = enqueue_frame(c, k, plan, false)) != FSI_OK) break;  // warm-up
+            (void)hipEventRecord(ea, c.stream);
+            for (uint32_t f = 0; f < frames && rc == FSI_OK; ++f) rc = enqueue_frame(c, k, plan, false);
+            (void)hipEventRecord(eb, c.stream);
+            if (rc != FSI_OK) break;
+            if (hipEventSynchronize(eb) != hipSuccess) {
+                rc = fail(FSI_ERR_HIP, "autotune: event wait failed");
+                break;
+            }
+            float ms = 0.0f;
+            (void)hipEventElapsedTime(&ms, ea, eb);
+            if (ms < best) {
+                best = ms;
+                best_g = g;
+                best_b = b;
+            }
+            (void)hipMemcpyAsync(c.d.pos, pos0, sizeof(float4) * n, hipMemcpyDeviceToDevice, c.stream);
+            (void)hipMemcpyAsync(c.d.vel, vel0, sizeof(float4) * n, hipMemcpyDeviceToDevice, c.stream);
+        }
+    (void)hipMemcpyAsync(c.d.pos, pos0, sizeof(float4) * n, hipMemcpyDeviceToDevice, c.stream);
+    (void)hipMemcpyAsync(c.d.vel, vel0, sizeof(float4) * n, hipMemcpyDeviceToDevice, c.stream);
+    (void)hipStreamSynchronize(c.stream);
+    (void)hipEventDestroy(ea);
+    (void)hipEventDestroy(eb);
+    dev_free(&pos0);
+    dev_free(&vel0);
+    c.d.variant = best_g;
+    c.sort_digit_bits = best_b;
+    c.timing = timing;
+    c.use_graph = graph;
+    c.acc = acc;
+    core_invalidate_graph(c);
+    if (gather) *gather = best_g;
+    if (digit_bits) *digit_bits = best_b;
+    return rc;
+}
+
+int core_drain_timing(Core& c) {
+    if (c.pending.empty()) return FSI_OK;
+    FSI_HIP_TRY(hipStreamSynchronize(c.stream));
+    for (const TimedPair& t : c.pending) {
+        float ms = 0.0f;
+        FSI_HIP_TRY(hipEventElapsedTime(&ms, t.a, t.b));
+        c.acc.ms[t.pass] += ms;
+        c.acc.launches[t.pass] += 1;
+        c.event_pool.push_back(t.a);
+        c.event_pool.push_back(t.b);
+    }
+    c.pending.clear();
+    return FSI_OK;
+}
+
+int core_set_tuning(Core& c, int knob, int value) {
+    switch (knob) {
+        case FSI_TUNE_GATHER:
+            if (value < 0 || value > 1) return fail(FSI_ERR_INVALID_ARG, "gather form must be 0 or 1");
+            c.d.variant = value;
+            return FSI_OK;
+        case FSI_TUNE_SORT_DIGIT_BITS:
+            if (value < 1 || value > kSortMaxDigitBits) return fail(FSI_ERR_INVALID_ARG, "digit bits must be 1..10");
+            c.sort_digit_bits = value;
+            return FSI_OK;
+        default:
+            return fail(FSI_ERR_INVALID_ARG, "unknown knob");
+    }
+}
+
+int core_get_tuning(const Core& c, int knob, int* value) {
+    switch (knob) {
+        case FSI_TUNE_GATHER: *value = c.d.variant; return FSI_OK;
+        case FSI_TUNE_SORT_DIGIT_BITS: *value = c.sort_digit_bits; return FSI_OK;
+        default: return fail(FSI_ERR_INVALID_ARG, "unknown knob");
+    }
+}
+
+}  // namespace fsi

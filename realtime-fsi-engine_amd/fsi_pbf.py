@@ -63,6 +63,15 @@ class CConstants(C.Structure):
     ]
 
 
+class CSceneDesc(C.Structure):
+    _fields_ = [
+        ("particle_offset", C.c_uint32), ("particle_count", C.c_uint32), ("hash_size", C.c_uint32),
+        ("bounds_min", C.c_float * 3), ("bounds_max", C.c_float * 3), ("sdf_grid", C.POINTER(C.c_float)),
+        ("sdf_dims", C.c_int32 * 3), ("sdf_origin", C.c_float * 3), ("sdf_cell", C.c_float),
+        ("sdf_padding", C.c_float), ("sdf_containment", C.c_uint32),
+    ]
+
+
 class CTiming(C.Structure):
     _fields_ = [("ms", C.c_double * 8), ("launches", C.c_uint64 * 8), ("substeps", C.c_uint64),
                 ("frames", C.c_uint64)]
@@ -174,6 +183,22 @@ def load() -> C.CDLL:
         "fsi_pbf_get_stream": ([P, C.POINTER(P)], C.c_int),
         "fsi_pbf_set_tuning": ([P, C.c_int, C.c_int], C.c_int),
         "fsi_pbf_get_tuning": ([P, C.c_int, C.POINTER(C.c_int)], C.c_int),
+        "fsi_pbf_autotune": ([P, C.c_uint32, C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
+        "fsi_pbf_set_graph": ([P, C.c_int], C.c_int),
+        "fsi_batch_create": ([C.POINTER(CFluidConfig), C.POINTER(P)], C.c_int),
+        "fsi_batch_destroy": ([P], C.c_int),
+        "fsi_batch_set_scenes": ([P, P, C.c_uint32, C.POINTER(CSceneDesc), C.c_uint32], C.c_int),
+        "fsi_batch_step": ([P], C.c_int),
+        "fsi_batch_sync": ([P], C.c_int),
+        "fsi_batch_read_particles": ([P, P, C.c_uint32, u32p], C.c_int),
+        "fsi_batch_read_grid": ([P, C.c_uint32, u32p, u32p, i32p], C.c_int),
+        "fsi_batch_set_graph": ([P, C.c_int], C.c_int),
+        "fsi_batch_set_tuning": ([P, C.c_int, C.c_int], C.c_int),
+        "fsi_batch_get_tuning": ([P, C.c_int, C.POINTER(C.c_int)], C.c_int),
+        "fsi_batch_autotune": ([P, C.c_uint32, C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
+        "fsi_batch_set_timing": ([P, C.c_int], C.c_int),
+        "fsi_batch_timing_reset": ([P], C.c_int),
+        "fsi_batch_read_timing": ([P, C.POINTER(CTiming)], C.c_int),
         "fsi_scene_lattice": ([C.c_uint32, C.c_uint32, C.c_uint32, f32p, C.c_float, C.c_float, C.c_uint32, f32p, P],
                               C.c_int),
         "fsi_mesh_icosphere": ([C.c_uint32, f64p, C.c_double, f64p, i32p], C.c_int),
@@ -374,6 +399,15 @@ class Solver:
         _check(self.lib.fsi_pbf_get_tuning(self.h, knob, C.byref(v)))
         return v.value
 
+    def set_graph(self, on: bool):
+        _check(self.lib.fsi_pbf_set_graph(self.h, 1 if on else 0))
+
+    def autotune(self, frames: int = 3):
+        """Time every (gather form, sort digit bits) pair; keeps the fastest. Particle state is restored."""
+        g, d = C.c_int(0), C.c_int(0)
+        _check(self.lib.fsi_pbf_autotune(self.h, frames, C.byref(g), C.byref(d)))
+        return g.value, d.value
+
     def set_timing(self, on: bool):
         _check(self.lib.fsi_pbf_set_timing(self.h, 1 if on else 0))
 
@@ -383,9 +417,147 @@ class Solver:
     def read_timing(self) -> dict:
         t = CTiming()
         _check(self.lib.fsi_pbf_read_timing(self.h, C.byref(t)))
-        return {
-            "ms": {PASS_NAMES[i]: t.ms[i] for i in range(8)},
-            "launches": {PASS_NAMES[i]: int(t.launches[i]) for i in range(8)},
-            "substeps": int(t.substeps),
-            "frames": int(t.frames),
-        }
+        return _timing_dict(t)
+
+
+def _timing_dict(t: CTiming) -> dict:
+    return {
+        "ms": {PASS_NAMES[i]: t.ms[i] for i in range(8)},
+        "launches": {PASS_NAMES[i]: int(t.launches[i]) for i in range(8)},
+        "substeps": int(t.substeps),
+        "frames": int(t.frames),
+    }
+
+
+@dataclass
+class Scene:
+    """One scene of a Batch: its particles, cell-table size, box and optional SDF."""
+    particles: np.ndarray
+    hash_size: int
+    bounds_min: tuple
+    bounds_max: tuple
+    sdf: Optional[SdfGrid] = None
+    sdf_padding: float = 0.0
+    sdf_containment: bool = False
+
+
+class Batch:
+    """Many independent scenes advanced together, one launch per pass.
+
+    Every scene follows the same frame as a ``Solver`` (PBFluids::step,
+    src/fluid/PBFluids.cpp:323-517) with the shared ``FluidConfig`` (its
+    particle_count, hash_size and bounds are taken per scene; bounds are the
+    box after padding).
+    """
+
+    def __init__(self, cfg: FluidConfig):
+        self.lib = load()
+        self.cfg = cfg
+        self._c = cfg.to_c()
+        h = C.c_void_p()
+        _check(self.lib.fsi_batch_create(C.byref(self._c), C.byref(h)))
+        self.h = h
+        self.counts: list = []
+        self.offsets: list = []
+        self.hash_sizes: list = []
+        self.n = 0
+
+    def close(self):
+        if self.h:
+            self.lib.fsi_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_scenes(self, scenes: list):
+        parts = [np.ascontiguousarray(s.particles, dtype=np.float32).reshape(-1, 12) for s in scenes]
+        allp = np.ascontiguousarray(np.concatenate(parts, axis=0)) if parts else np.zeros((0, 12), np.float32)
+        descs = (CSceneDesc * len(scenes))()
+        keep = []
+        off = 0
+        self.counts, self.offsets, self.hash_sizes = [], [], []
+        for i, (s, p) in enumerate(zip(scenes, parts)):
+            d = descs[i]
+            d.particle_offset = off
+            d.particle_count = p.shape[0]
+            d.hash_size = int(s.hash_size)
+            for a in range(3):
+                d.bounds_min[a] = float(np.float32(s.bounds_min[a]))
+                d.bounds_max[a] = float(np.float32(s.bounds_max[a]))
+            if s.sdf is not None:
+                g = np.ascontiguousarray(s.sdf.grid, dtype=np.float32)
+                keep.append(g)
+                d.sdf_grid = _ptr(g, C.c_float)
+                for a in range(3):
+                    d.sdf_dims[a] = s.sdf.dims[a]
+                    d.sdf_origin[a] = float(np.float32(s.sdf.origin[a]))
+                d.sdf_cell = float(np.float32(s.sdf.cell))
+                d.sdf_padding = float(np.float32(s.sdf_padding))
+                d.sdf_containment = 1 if s.sdf_containment else 0
+            self.counts.append(p.shape[0])
+            self.offsets.append(off)
+            self.hash_sizes.append(int(s.hash_size))
+            off += p.shape[0]
+        self.n = off
+        _check(self.lib.fsi_batch_set_scenes(self.h, allp.ctypes.data_as(C.c_void_p), off, descs, len(scenes)))
+
+    def step(self, frames: int = 1):
+        for _ in range(frames):
+            _check(self.lib.fsi_batch_step(self.h))
+
+    def sync(self):
+        _check(self.lib.fsi_batch_sync(self.h))
+
+    def read_particles(self) -> np.ndarray:
+        out = np.zeros((self.n, 12), dtype=np.float32)
+        n = C.c_uint32(0)
+        _check(self.lib.fsi_batch_read_particles(self.h, out.ctypes.data_as(C.c_void_p), self.n, C.byref(n)))
+        return out
+
+    def scene_particles(self, out: np.ndarray, s: int) -> np.ndarray:
+        return out[self.offsets[s]:self.offsets[s] + self.counts[s]]
+
+    def read_grid(self, s: int):
+        keys = np.zeros(self.counts[s], dtype=np.uint32)
+        ids = np.zeros(self.counts[s], dtype=np.uint32)
+        cells = np.zeros((self.hash_sizes[s], 2), dtype=np.int32)
+        _check(self.lib.fsi_batch_read_grid(self.h, s, _ptr(keys, C.c_uint32), _ptr(ids, C.c_uint32),
+                                            _ptr(cells, C.c_int32)))
+        return keys, ids, cells
+
+    def set_graph(self, on: bool):
+        _check(self.lib.fsi_batch_set_graph(self.h, 1 if on else 0))
+
+    def set_tuning(self, knob: int, value: int):
+        _check(self.lib.fsi_batch_set_tuning(self.h, knob, value))
+
+    def get_tuning(self, knob: int) -> int:
+        v = C.c_int(0)
+        _check(self.lib.fsi_batch_get_tuning(self.h, knob, C.byref(v)))
+        return v.value
+
+    def autotune(self, frames: int = 3):
+        g, d = C.c_int(0), C.c_int(0)
+        _check(self.lib.fsi_batch_autotune(self.h, frames, C.byref(g), C.byref(d)))
+        return g.value, d.value
+
+    def set_timing(self, on: bool):
+        _check(self.lib.fsi_batch_set_timing(self.h, 1 if on else 0))
+
+    def timing_reset(self):
+        _check(self.lib.fsi_batch_timing_reset(self.h))
+
+    def read_timing(self) -> dict:
+        t = CTiming()
+        _check(self.lib.fsi_batch_read_timing(self.h, C.byref(t)))
+        return _timing_dict(t)
