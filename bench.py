@@ -12,7 +12,16 @@ particle-iterations/s = N * substeps * iterations * frames / wall time, inputs
 resident in HBM.  Per-pass kernel time comes from HIP events recorded on the
 solver's stream around every launch during the timed region.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+Second leg (reported under "batched"): 256 independent dam-break scenes with
+particle counts log-uniform in [4096, 131072] (seeded), each a lattice block
+in the corner of its own box next to a static sphere obstacle of radius
+U[0.3, 1.0] given as an SDF grid; 4 iterations, 2 substeps.  The multi-scene
+handle (one launch per pass for all scenes, frames replayed from a HIP graph,
+knobs picked by its autotuner) is timed against a loop of one single-scene
+handle per scene.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch-steps K2]
+                       [--no-batched] [--no-cpu-baseline]
 """
 import argparse
 import json
@@ -59,7 +68,102 @@ def algorithmic_bytes(n, sdf_bytes):
     }
 
 
-def cpu_baseline(fsi, cfg, parts, sdf, frames=2):
+BATCH_SCENES, BATCH_ITERS, BATCH_SUBSTEPS, BATCH_SEED = 256, 4, 2, 2024
+
+
+def sphere_sdf(fsi, centre, r, cell=0.05, margin=4):
+    """Signed distance |x - c| - r sampled at voxel centres origin + i*cell,
+    covering the sphere plus `margin` cells (x fastest)."""
+    ext = r + margin * cell
+    n = int(np.ceil(2.0 * ext / cell)) + 1
+    origin = np.asarray(centre, dtype=np.float64) - ext
+    ax = [origin[a] + cell * np.arange(n) for a in range(3)]
+    z, y, x = np.meshgrid(ax[2], ax[1], ax[0], indexing="ij")
+    d = np.sqrt((x - centre[0]) ** 2 + (y - centre[1]) ** 2 + (z - centre[2]) ** 2) - r
+    return fsi.SdfGrid(grid=d.astype(np.float32), origin=origin, cell=cell, dims=(n, n, n))
+
+
+def make_batch_scenes(fsi):
+    """256 seeded scenes: n ~ logU[4096, 131072], the first n particles of a
+    near-cubic lattice block (spacing 0.05, jitter 0.001) in the corner of the
+    box, a sphere of radius U[0.3, 1.0] beside it, hash size next_pow2(n)."""
+    rng = np.random.default_rng(BATCH_SEED)
+    scenes = []
+    for s in range(BATCH_SCENES):
+        n = int(round(np.exp(rng.uniform(np.log(4096.0), np.log(131072.0)))))
+        r = float(rng.uniform(0.3, 1.0))
+        side = int(np.ceil(np.cbrt(n)))
+        ny = int(np.ceil(n / (side * side)))
+        parts = fsi.lattice(side, ny, side, (0.0, 0.0, 0.0), 0.05, 0.001, 1337 + s)[:n]
+        bx, by, bz = side * 0.05, ny * 0.05, side * 0.05
+        box = (bx + 2.0 * r + 0.4, max(1.5 * by, 2.0 * r + 0.4), max(bz, 2.0 * r + 0.4))
+        centre = (bx + r + 0.2, r + 0.1, 0.5 * box[2])
+        H = 1 << int(np.ceil(np.log2(n)))
+        scenes.append(fsi.Scene(particles=parts, hash_size=H, bounds_min=(0.0, 0.0, 0.0), bounds_max=box,
+                                sdf=sphere_sdf(fsi, centre, r), sdf_padding=0.025))
+    return scenes
+
+
+def batch_config(fsi, box=(1.0, 1.0, 1.0), hash_size=1 << 17):
+    return fsi.FluidConfig(dt=1.0 / 120.0, gravity=(0.0, -9.81, 0.0), h=0.1, rho0=6378.0,
+                           solver_iterations=BATCH_ITERS, substep_iterations=BATCH_SUBSTEPS, eps=600.0,
+                           bounds_min=(0.0, 0.0, 0.0), bounds_max=box, hash_size=hash_size)
+
+
+def batched_leg(fsi, steps, warmup):
+    """Multi-scene handle vs a loop of single-scene handles on the same scenes."""
+    scenes = make_batch_scenes(fsi)
+    total = sum(len(sc.particles) for sc in scenes)
+    work = total * BATCH_SUBSTEPS * BATCH_ITERS * steps
+
+    b = fsi.Batch(batch_config(fsi))
+    b.set_scenes(scenes)
+    gather, digit_bits = b.autotune(1)
+    b.set_graph(True)
+    b.step(warmup)
+    b.sync()
+    t0 = time.perf_counter()
+    b.step(steps)
+    b.sync()
+    wall_b = time.perf_counter() - t0
+    got = b.read_particles()
+    b.close()
+
+    solvers = []
+    for sc in scenes:
+        sv = fsi.Solver(batch_config(fsi, sc.bounds_max, sc.hash_size))
+        sv.set_particles(sc.particles)
+        sv.set_sdf(sc.sdf, sc.sdf_padding, False)
+        sv.set_tuning(fsi.TUNE_GATHER, gather)
+        sv.set_tuning(fsi.TUNE_SORT_DIGIT_BITS, digit_bits)
+        solvers.append(sv)
+    for _ in range(warmup):
+        for sv in solvers:
+            sv.step()
+    for sv in solvers:
+        sv.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for sv in solvers:
+            sv.step()
+    for sv in solvers:
+        sv.sync()
+    wall_l = time.perf_counter() - t0
+    same = all(np.array_equal(sv.read_particles(), got[o:o + c])
+               for sv, o, c in zip(solvers, np.cumsum([0] + [len(x.particles) for x in scenes]),
+                                   [len(x.particles) for x in scenes]))
+    for sv in solvers:
+        sv.close()
+    return {
+        "scenes": BATCH_SCENES, "particles": total, "iterations": BATCH_ITERS, "substeps": BATCH_SUBSTEPS,
+        "steps": steps, "warmup": warmup, "value": work / wall_b, "ms_per_step": wall_b / steps * 1e3,
+        "loop_value": work / wall_l, "loop_ms_per_step": wall_l / steps * 1e3, "speedup": wall_l / wall_b,
+        "autotuned": {"gather": gather, "sort_digit_bits": digit_bits}, "graph": True,
+        "identical_to_loop": bool(same),
+    }
+
+
+def cpu_baseline(fsi, cfg, parts, sdf, frames=6):
     """The CPU oracle (a C restatement of the reference shaders, OpenMP over
     particles) on the same workload, timed over `frames` frames."""
     threads = min(16, os.cpu_count() or 1)
@@ -81,12 +185,29 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--batch-steps", type=int, default=5)
+    ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain device copy of 256 MiB (3 launches), the reference point that scales "
+                         "the FETCH_SIZE / WRITE_SIZE counters to bytes")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the PBF substep of this build runs on a single MI355X "
               "(multi-GPU sharding is a design note, not built)", file=sys.stderr)
         sys.exit(2)
+
+    if args.calibrate_copy:
+        import torch
+
+        n = 64 << 20
+        a = torch.full((n,), 1.5, device="cuda:0", dtype=torch.float32)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            torch.mul(a, 1.0, out=b)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration": "copy", "bytes_read": 4 * n, "bytes_written": 4 * n, "launches": 3}))
+        return
 
     import fsi_pbf as fsi
 
@@ -129,10 +250,15 @@ def main():
     roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
                 "bytes_per_launch": ab[dom]}
-    tfile = os.path.join(ROOT, "profiles", f"traffic_{dom}_c2_1m.json")
+    # HBM bytes per launch of the dominant kernel: FETCH_SIZE + WRITE_SIZE of
+    # the same workload, calibrated against a copy kernel (profiles/)
+    tfile = os.path.join(ROOT, "profiles", "traffic_c2_1m.json")
     if os.path.exists(tfile):
         with open(tfile) as f:
-            roofline["traffic"] = json.load(f).get("bytes_per_launch")
+            kern = json.load(f)["kernels"]
+        hits = [v for k, v in kern.items() if k.startswith("k_" + dom + "<")]
+        if hits:
+            roofline["traffic"] = round(hits[0]["bytes_per_launch"])
 
     out = {
         "metric": "PBF particle-iterations/s at 1M particles (plain/batched) + adjoint/forward ratio",
@@ -144,6 +270,8 @@ def main():
         "roofline": roofline,
         "passes": passes,
     }
+    if not args.no_batched:
+        out["batched"] = batched_leg(fsi, args.batch_steps, 1)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(fsi, cfg, parts, sdf)
     print(json.dumps(out))

@@ -1,0 +1,118 @@
+"""Summaries of rocprofv3 CSV output for the PBF substep kernels.
+
+  python rocprof_summary.py kernels <trace_dir> --cmd "<command>"
+      per-kernel calls / average / share from <trace_dir>/run_kernel_stats.csv
+  python rocprof_summary.py traffic <fetch_dir> <write_dir> --calib-fetch <dir> --calib-write <dir>
+                            --calib-bytes B --cmd "<command>" [--json out.json]
+      HBM bytes per dispatch of each kernel: FETCH_SIZE and WRITE_SIZE (KiB per
+      dispatch) scaled by the factor that maps the same counters of a plain copy
+      kernel of B bytes read and B bytes written onto B.
+
+Kernels are grouped into the passes of a substep by name.
+"""
+import argparse
+import collections
+import csv
+import json
+import os
+import re
+import sys
+
+PASS_OF = [
+    ("k_predict", "predict"), ("k_radix", "sort"), ("k_scan", "sort"), ("k_iota", "sort"),
+    ("k_clear_cells", "tables"), ("k_build_cells", "tables"), ("k_reorder", "tables"), ("k_order", "tables"),
+    ("k_bin", "tables"), ("k_fill", "tables"), ("k_lod", "lod"), ("k_lambda", "lambda"), ("k_deltap", "deltap"),
+    ("k_apply_sdf", "deltap"), ("k_integrate", "integrate"), ("k_vorticity", "vorticity"), ("k_contact", "contacts"),
+]
+
+
+def short_name(name: str) -> str:
+    m = re.search(r"(k_[A-Za-z0-9_]+)(<[^()]*>)?\(", name)
+    if m:
+        return m.group(1) + (m.group(2) or "")
+    return name.split("(")[0][:60]
+
+
+def pass_of(short: str) -> str:
+    for prefix, p in PASS_OF:
+        if short.startswith(prefix):
+            return p
+    return "other"
+
+
+def kernels(args):
+    rows = list(csv.DictReader(open(os.path.join(args.dir, "run_kernel_stats.csv"))))
+    total = sum(float(r["TotalDurationNs"]) for r in rows)
+    print(f"# rocprofv3 kernel statistics\n\nCommand:\n\n    {args.cmd}\n")
+    print("| kernel | pass | calls | avg us | min us | max us | share |")
+    print("|---|---|---|---|---|---|---|")
+    by_pass = collections.defaultdict(float)
+    for r in sorted(rows, key=lambda r: -float(r["TotalDurationNs"])):
+        s = short_name(r["Name"])
+        p = pass_of(s)
+        by_pass[p] += float(r["TotalDurationNs"])
+        print(f"| `{s}` | {p} | {r['Calls']} | {float(r['AverageNs']) / 1e3:.2f} | {float(r['MinNs']) / 1e3:.2f} | "
+              f"{float(r['MaxNs']) / 1e3:.2f} | {float(r['TotalDurationNs']) / total:.4f} |")
+    print("\n| pass | kernel time share |\n|---|---|")
+    for p, t in sorted(by_pass.items(), key=lambda kv: -kv[1]):
+        print(f"| {p} | {t / total:.4f} |")
+
+
+def counter_means(d: str, counter: str):
+    acc = collections.defaultdict(list)
+    for r in csv.DictReader(open(os.path.join(d, "run_counter_collection.csv"))):
+        if r["Counter_Name"] == counter:
+            acc[short_name(r["Kernel_Name"])].append(float(r["Counter_Value"]))
+    return {k: sum(v) / len(v) for k, v in acc.items()}, {k: len(v) for k, v in acc.items()}
+
+
+def traffic(args):
+    fetch, calls = counter_means(args.fetch_dir, "FETCH_SIZE")
+    write, _ = counter_means(args.write_dir, "WRITE_SIZE")
+    cf, _ = counter_means(args.calib_fetch, "FETCH_SIZE")
+    cw, _ = counter_means(args.calib_write, "WRITE_SIZE")
+    copy = [k for k in cf if re.search(args.calib_kernel, k)]
+    if not copy:
+        sys.exit(f"no calibration kernel matching {args.calib_kernel!r} in {sorted(cf)}")
+    ck = max(copy, key=lambda k: cf[k])
+    kr = args.calib_bytes / (cf[ck] * 1024.0)
+    kw = args.calib_bytes / (cw[ck] * 1024.0)
+    print(f"# HBM traffic per dispatch (FETCH_SIZE / WRITE_SIZE)\n\nCommand:\n\n    {args.cmd}\n")
+    print(f"Calibration: `{ck}` copying {args.calib_bytes} bytes read FETCH_SIZE {cf[ck]:.1f} KiB and WRITE_SIZE "
+          f"{cw[ck]:.1f} KiB per dispatch, so bytes = FETCH_SIZE x 1024 x {kr:.4f} and WRITE_SIZE x 1024 x {kw:.4f}.\n")
+    print("| kernel | pass | dispatches | FETCH KiB | WRITE KiB | read MB | written MB | total MB |")
+    print("|---|---|---|---|---|---|---|---|")
+    out = {"calibration": {"kernel": ck, "bytes": args.calib_bytes, "read_factor": kr, "write_factor": kw},
+           "kernels": {}}
+    for k in sorted(fetch, key=lambda k: -fetch[k]):
+        rd = fetch[k] * 1024.0 * kr
+        wr = write.get(k, 0.0) * 1024.0 * kw
+        out["kernels"][k] = {"pass": pass_of(k), "read_bytes": rd, "write_bytes": wr, "bytes_per_launch": rd + wr}
+        print(f"| `{k}` | {pass_of(k)} | {calls[k]} | {fetch[k]:.1f} | {write.get(k, 0.0):.1f} | {rd / 1e6:.2f} | "
+              f"{wr / 1e6:.2f} | {(rd + wr) / 1e6:.2f} |")
+    if args.json:
+        with open(args.json, "w") as f:
+            json.dump(out, f, indent=1)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="what", required=True)
+    k = sub.add_parser("kernels")
+    k.add_argument("dir")
+    k.add_argument("--cmd", required=True)
+    t = sub.add_parser("traffic")
+    t.add_argument("fetch_dir")
+    t.add_argument("write_dir")
+    t.add_argument("--calib-fetch", required=True)
+    t.add_argument("--calib-write", required=True)
+    t.add_argument("--calib-bytes", type=float, required=True)
+    t.add_argument("--calib-kernel", default="elementwise|Functor|copy")
+    t.add_argument("--cmd", required=True)
+    t.add_argument("--json")
+    a = ap.parse_args()
+    kernels(a) if a.what == "kernels" else traffic(a)
+
+
+if __name__ == "__main__":
+    main()
