@@ -20,8 +20,16 @@ handle (one launch per pass for all scenes, frames replayed from a HIP graph,
 knobs picked by its autotuner) is timed against a loop of one single-scene
 handle per scene.
 
+Third leg (reported under "adjoint"): the differentiable workload, 65,536
+particles (a 64x32x32 lattice block falling freely in the middle of a 6.4^3
+box, initial velocities N(0, 0.1) seeded), 20 frames of 4 iterations x 2 substeps,
+viscosity 0.01, eps 600.  The forward (frames + per-frame checkpoints) and the
+backward (replay of every frame with neighbour lists + reverse passes) of the
+hand-written adjoint are timed on the same inputs; the ratio is backward /
+forward.  The backward's input is the gradient of a 32^3 density-matching loss.
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch-steps K2]
-                       [--no-batched] [--no-cpu-baseline]
+                       [--no-batched] [--no-adjoint] [--no-cpu-baseline]
 """
 import argparse
 import json
@@ -163,6 +171,78 @@ def batched_leg(fsi, steps, warmup):
     }
 
 
+def adjoint_leg(fsi, reps=3):
+    """Forward / backward wall time of the adjoint handle at 65,536 particles."""
+    import ctypes as C
+
+    import torch
+    import fsi_torch
+
+    box = (6.4, 6.4, 6.4)
+    cfg = fsi.FluidConfig(dt=1.0 / 120.0, gravity=(0.0, -9.81, 0.0), h=0.1, rho0=6378.0, solver_iterations=4,
+                          substep_iterations=2, eps=600.0, bounds_min=(0.0, 0.0, 0.0), bounds_max=box,
+                          hash_size=1 << 17, viscosity=0.01)
+    frames = 20
+    parts = fsi.lattice(64, 32, 32, (1.6, 2.4, 2.4), 0.05, 0.001, 1337)
+    n = len(parts)
+    v0 = np.random.default_rng(1337).normal(0.0, 0.1, size=(n, 3)).astype(np.float32)
+    dev = torch.device("cuda:0")
+    pos0 = torch.zeros((n, 4), device=dev)
+    pos0[:, 0:3] = torch.tensor(parts[:, 0:3], device=dev)
+    vel0 = torch.zeros((n, 4), device=dev)
+    vel0[:, 0:3] = torch.tensor(v0, device=dev)
+    fh = fsi_torch.Frames(cfg, n, frames)
+    lib, h = fh.lib, fh.h
+    pos = torch.empty_like(pos0)
+    vel = torch.empty_like(vel0)
+    stream = C.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+    ptr = lambda t: C.c_void_p(t.data_ptr())  # noqa: E731
+
+    def forward():
+        fsi._check(lib.fsi_adj_forward(h, ptr(pos0), ptr(vel0), 0.01, 600.0, ptr(pos), ptr(vel), stream))
+
+    # the backward's input: d/dx of a 32^3 density loss against a perturbed run
+    vt = vel0.clone()
+    vt[:, 0:3] += torch.tensor(np.random.default_rng(99).normal(0.0, 0.05, size=(n, 3)).astype(np.float32),
+                               device=dev)
+    fsi._check(lib.fsi_adj_forward(h, ptr(pos0), ptr(vt), 0.01, 600.0, ptr(pos), ptr(vel), stream))
+    target = fsi_torch.density_splat(pos[:, 0:3].clone(), (0.0, 0.0, 0.0), box, 32)
+    forward()
+    x = pos[:, 0:3].clone().requires_grad_(True)
+    loss = ((fsi_torch.density_splat(x, (0.0, 0.0, 0.0), box, 32) - target) ** 2).sum()
+    (gx,) = torch.autograd.grad(loss, x)
+    gpos = torch.zeros_like(pos0)
+    gpos[:, 0:3] = gx
+    gp0 = torch.empty_like(pos0)
+    gv0 = torch.empty_like(pos0)
+    params = (C.c_double * 2)()
+
+    def backward():
+        fsi._check(lib.fsi_adj_backward(h, ptr(gpos), None, ptr(gp0), ptr(gv0), params, stream))
+
+    backward()  # warm
+    tf, tb = [], []
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        forward()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        tf.append(t1 - t0)
+        tb.append(t2 - t1)
+    st = fh.stats()
+    fh.close()
+    f_ms = float(np.median(tf)) * 1e3
+    b_ms = float(np.median(tb)) * 1e3
+    return {"particles": n, "frames": frames, "iterations": 4, "substeps": 2, "forward_ms": round(f_ms, 3),
+            "backward_ms": round(b_ms, 3), "adjoint_forward_ratio": round(b_ms / f_ms, 3), "reps": reps,
+            "loss": float(loss.item()), "grad_viscosity": params[0], "grad_eps": params[1],
+            "replay_mismatches": st["replay_mismatches"], "max_neighbours": st["max_neighbours"]}
+
+
 def cpu_baseline(fsi, cfg, parts, sdf, frames=6):
     """The CPU oracle (a C restatement of the reference shaders, OpenMP over
     particles) on the same workload, timed over `frames` frames."""
@@ -187,6 +267,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain device copy of 256 MiB (3 launches), the reference point that scales "
@@ -209,6 +290,8 @@ def main():
         print(json.dumps({"calibration": "copy", "bytes_read": 4 * n, "bytes_written": 4 * n, "launches": 3}))
         return
 
+    if not args.no_adjoint:
+        import torch  # noqa: F401  (before the library: one HIP runtime in the process)
     import fsi_pbf as fsi
 
     cfg = make_config(fsi)
@@ -272,6 +355,8 @@ def main():
     }
     if not args.no_batched:
         out["batched"] = batched_leg(fsi, args.batch_steps, 1)
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint_leg(fsi)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(fsi, cfg, parts, sdf)
     print(json.dumps(out))

@@ -359,6 +359,46 @@ int fsi_mesh_icosphere(uint32_t subdivisions, const double centre[3], double rad
 int fsi_sdf_build(const double* verts, uint32_t nverts, const int32_t* faces, uint32_t nfaces,
                   double cell, double origin[3], int32_t dims[3], float* grid);
 
+/* ------------------------------------------------------------------------- */
+/* Differentiable frames.  No counterpart in the reference (its PBFluids has  */
+/* no gradient): the reverse-mode derivative of `frames` whole frames of      */
+/* PBFluids::step (src/fluid/PBFluids.cpp:323-517) with respect to the        */
+/* initial positions and velocities, the XSPH coefficient (viscosity) and the */
+/* relaxation epsilon, for fitting them to a target state.  APBF levels of    */
+/* detail, the SDF boundary and the interaction force are not part of it.     */
+/* States are device arrays of n float4 in particle-id order (xyz + the       */
+/* record's pad, carried through).  `stream` is the caller's hipStream_t     */
+/* (NULL: the null stream): a call starts after the work already queued on it */
+/* and work queued on it afterwards starts after the call's work.             */
+/* ------------------------------------------------------------------------- */
+
+typedef struct fsi_adjoint fsi_adjoint; /* opaque handle */
+
+/* cfg: everything but viscosity and eps, which each forward call sets. */
+int fsi_adj_create(const fsi_fluid_config* cfg, uint32_t n, uint32_t frames, fsi_adjoint** out);
+int fsi_adj_destroy(fsi_adjoint* a);
+/* Runs the frames from (pos0, vel0), keeping a checkpoint of every frame's
+ * starting state; pos_out / vel_out (may be NULL) receive the final state. */
+int fsi_adj_forward(fsi_adjoint* a, const float* pos0, const float* vel0, float viscosity, float eps, float* pos_out,
+                    float* vel_out, void* stream);
+/* Given dL/d(final pos) and dL/d(final vel) (either may be NULL: zero),
+ * writes dL/d(pos0) and dL/d(vel0) (may be NULL) and, on the host,
+ * grad_params[0] = dL/d(viscosity), grad_params[1] = dL/d(eps).  Replays
+ * each frame from its checkpoint; fails with FSI_ERR_STATE if a replay does
+ * not reproduce the forward bit for bit, FSI_ERR_CAPACITY if a particle has
+ * more than 96 neighbours or is the neighbour of more than 128 particles in a
+ * pass (aliased buckets count once per visit).  Synchronises with the device. */
+int fsi_adj_backward(fsi_adjoint* a, const float* grad_pos, const float* grad_vel, float* grad_pos0, float* grad_vel0,
+                     double* grad_params, void* stream);
+/* Of the last backward: out[0] capacity bits (1: neighbours, 2: pairs into one
+ * particle), out[1] replayed records that differed from the forward, out[2]
+ * the largest neighbour count and out[3] the largest in-degree of any pass. */
+int fsi_adj_stats(fsi_adjoint* a, uint32_t out[4]);
+/* Of the last backward: out[2f], out[2f+1] = the largest |component| of the
+ * adjoint of the positions and of the velocities at the start of frame f
+ * (NaN if any was NaN); `frames` pairs. */
+int fsi_adj_frame_norms(fsi_adjoint* a, float* out);
+
 #ifdef __cplusplus
 }
 #endif

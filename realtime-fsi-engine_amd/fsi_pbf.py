@@ -199,6 +199,12 @@ def load() -> C.CDLL:
         "fsi_batch_set_timing": ([P, C.c_int], C.c_int),
         "fsi_batch_timing_reset": ([P], C.c_int),
         "fsi_batch_read_timing": ([P, C.POINTER(CTiming)], C.c_int),
+        "fsi_adj_create": ([C.POINTER(CFluidConfig), C.c_uint32, C.c_uint32, C.POINTER(P)], C.c_int),
+        "fsi_adj_destroy": ([P], C.c_int),
+        "fsi_adj_forward": ([P, P, P, C.c_float, C.c_float, P, P, P], C.c_int),
+        "fsi_adj_backward": ([P, P, P, P, P, f64p, P], C.c_int),
+        "fsi_adj_stats": ([P, u32p], C.c_int),
+        "fsi_adj_frame_norms": ([P, f32p], C.c_int),
         "fsi_scene_lattice": ([C.c_uint32, C.c_uint32, C.c_uint32, f32p, C.c_float, C.c_float, C.c_uint32, f32p, P],
                               C.c_int),
         "fsi_mesh_icosphere": ([C.c_uint32, f64p, C.c_double, f64p, i32p], C.c_int),
