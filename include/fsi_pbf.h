@@ -258,6 +258,10 @@ int fsi_pbf_read_counts(fsi_pbf* h, uint32_t* candidates, uint32_t* neighbours);
 
 /* Per-pass device timing. */
 int fsi_pbf_set_timing(fsi_pbf* h, int enable);
+/* Timing of the passes whose bit (1 << FSI_PASS_*) is set in mask only: each
+ * timed launch is bracketed by two events, which cost time of their own, so
+ * a run that times one pass is closer to an untimed one. */
+int fsi_pbf_set_timing_passes(fsi_pbf* h, uint32_t mask);
 int fsi_pbf_timing_reset(fsi_pbf* h);
 int fsi_pbf_read_timing(fsi_pbf* h, fsi_pbf_timing* out);
 
@@ -266,7 +270,8 @@ int fsi_pbf_read_timing(fsi_pbf* h, fsi_pbf_timing* out);
  * passes, src/fluid/PBFluids.cpp:330,355-361).  Every setting computes the
  * same results; only speed differs. */
 enum fsi_pbf_knob {
-    FSI_TUNE_GATHER = 0,          /* neighbour-gather form: 0 single loop, 1 two-phase list  */
+    FSI_TUNE_GATHER = 0,          /* neighbour-gather form: 0 single loop, 1 two-phase list, */
+                                  /* 2 LDS-staged 4x4x4-cell tiles                           */
     FSI_TUNE_SORT_DIGIT_BITS = 1, /* widest radix digit, 1..10                                */
     FSI_TUNE_COUNT = 2
 };
@@ -328,6 +333,7 @@ int fsi_batch_set_tuning(fsi_batch* b, int knob, int value);
 int fsi_batch_get_tuning(fsi_batch* b, int knob, int* value);
 int fsi_batch_autotune(fsi_batch* b, uint32_t frames, int* gather, int* digit_bits);
 int fsi_batch_set_timing(fsi_batch* b, int enable);
+int fsi_batch_set_timing_passes(fsi_batch* b, uint32_t mask);
 int fsi_batch_timing_reset(fsi_batch* b);
 int fsi_batch_read_timing(fsi_batch* b, fsi_pbf_timing* out);
 

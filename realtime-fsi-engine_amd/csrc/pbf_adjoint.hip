@@ -848,7 +848,7 @@ StepConst step_const(const fsi_adjoint* a) {
     }
     d.hash_mask = a->k.hash_mask;
     d.hash_base = 0;
-    set_binning(d, a->k.bounds_min, a->k.bounds_max, a->k.h, 7, 0);
+    set_binning(d, a->k.bounds_min, a->k.bounds_max, a->k.h, 7, 0, false);
     return c;
 }
 
@@ -1078,6 +1078,7 @@ int fsi_adj_forward(fsi_adjoint* a, const float* pos0, const float* vel0, float 
     pack_constants(a->cfg, a->cfg.bounds_min, a->cfg.bounds_max, std::clamp(a->cfg.bound_damping, 0.0f, 1.0f), a->n,
                    a->k);
     const StepConst c = step_const(a);
+    a->core.d.tiled = c.dom.dshift == kTileShift;
     FramePlan plan;
     plan.substeps = a->S;
     plan.max_iter = a->I;

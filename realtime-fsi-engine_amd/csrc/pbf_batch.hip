@@ -39,6 +39,7 @@ struct fsi_batch {
     Domain* d_doms = nullptr;
     uint16_t* d_scene_of = nullptr;
     uint32_t total_hash = 0;
+    uint32_t axis_bits = 1;  // Morton bits per axis of every scene's bins
     bool any_sdf = false;
     std::vector<float> host_pred;
 };
@@ -80,6 +81,9 @@ FramePlan frame_plan(const fsi_batch* b) {
 void prepare(fsi_batch* b) {
     b->core.d.batched = true;
     b->core.d.any_sdf = b->any_sdf;
+    bool tiled = true;
+    for (const Domain& d : b->doms) tiled = tiled && d.dshift == kTileShift;
+    b->core.d.tiled = tiled;
 }
 
 }  // namespace
@@ -144,6 +148,7 @@ int fsi_batch_set_scenes(fsi_batch* b, const fsi_particle* ps, uint32_t total, c
     int sbits = 0;
     while ((1u << sbits) < nscenes) ++sbits;
     const uint32_t axis_bits = (uint32_t)std::max(1, std::min(7, (21 - sbits) / 3));
+    b->axis_bits = axis_bits;
     b->scenes.resize(nscenes);
     b->doms.resize(nscenes);
     b->any_sdf = false;
@@ -163,7 +168,7 @@ int fsi_batch_set_scenes(fsi_batch* b, const fsi_particle* ps, uint32_t total, c
         }
         d.hash_mask = h.hash_size - 1u;
         d.hash_base = h.hash_base;
-        set_binning(d, d.bmin, d.bmax, b->cfg.h, axis_bits, s << (3 * axis_bits));
+        set_binning(d, d.bmin, d.bmax, b->cfg.h, axis_bits, s << (3 * axis_bits), c.d.variant == 2);
         if (sc[s].sdf_grid) {
             const size_t cnt = (size_t)sc[s].sdf_dims[0] * sc[s].sdf_dims[1] * sc[s].sdf_dims[2];
             FSI_HIP_TRY(dev_alloc(&h.sdf_grid, cnt));
@@ -295,8 +300,22 @@ int fsi_batch_set_graph(fsi_batch* b, int enable) {
 }
 
 int fsi_batch_set_tuning(fsi_batch* b, int knob, int value) {
-    if (!b) return fail(FSI_ERR_INVALID_ARG, "handle is NULL");
-    return core_set_tuning(b->core, knob, value);
+    int rc = bind(b);
+    if (rc) return rc;
+    const bool tiles_before = b->core.d.variant == 2;
+    if ((rc = core_set_tuning(b->core, knob, value))) return rc;
+    const bool tiles = b->core.d.variant == 2;
+    if (tiles != tiles_before && !b->doms.empty()) {
+        // the tiled gathers take bins of 2^kTileShift cells
+        FSI_HIP_TRY(hipStreamSynchronize(b->core.stream));
+        for (size_t s = 0; s < b->doms.size(); ++s) {
+            Domain& d = b->doms[s];
+            set_binning(d, d.bmin, d.bmax, b->cfg.h, b->axis_bits, (uint32_t)s << (3 * b->axis_bits), tiles);
+        }
+        FSI_HIP_TRY(hipMemcpy(b->d_doms, b->doms.data(), sizeof(Domain) * b->doms.size(), hipMemcpyHostToDevice));
+        core_invalidate_graph(b->core);
+    }
+    return FSI_OK;
 }
 
 int fsi_batch_get_tuning(fsi_batch* b, int knob, int* value) {
@@ -314,6 +333,14 @@ int fsi_batch_autotune(fsi_batch* b, uint32_t frames, int* gather, int* digit_bi
 int fsi_batch_set_timing(fsi_batch* b, int enable) {
     if (!b) return fail(FSI_ERR_INVALID_ARG, "handle is NULL");
     b->core.timing = enable != 0;
+    b->core.timing_mask = 0xffu;
+    return FSI_OK;
+}
+
+int fsi_batch_set_timing_passes(fsi_batch* b, uint32_t mask) {
+    if (!b) return fail(FSI_ERR_INVALID_ARG, "handle is NULL");
+    b->core.timing = mask != 0;
+    b->core.timing_mask = mask;
     return FSI_OK;
 }
 

@@ -74,7 +74,7 @@ StepConst step_const(const fsi_pbf* h) {
     d.sdf_cell = h->sdf_cell;
     d.sdf_pad = h->sdf_padding;
     d.sdf_grid = h->sdf_grid;
-    set_binning(d, u.bounds_min, u.bounds_max, u.h, 7, 0);
+    set_binning(d, u.bounds_min, u.bounds_max, u.h, 7, 0, h->core.d.variant == 2);
     return c;
 }
 
@@ -379,6 +379,7 @@ int fsi_pbf_step(fsi_pbf* h) {
     const StepConst c = step_const(h);
     h->core.d.batched = false;
     h->core.d.any_sdf = c.dom.enable_sdf == 1u;
+    h->core.d.tiled = c.dom.dshift == kTileShift;
     rc = core_frame(h->core, c, frame_plan(h));
     h->contacts_valid = false;
     return rc;
@@ -503,6 +504,14 @@ int fsi_pbf_read_counts(fsi_pbf* h, uint32_t* cand, uint32_t* nbr) {
 int fsi_pbf_set_timing(fsi_pbf* h, int enable) {
     if (!h) return fail(FSI_ERR_INVALID_ARG, "handle is NULL");
     h->core.timing = enable != 0;
+    h->core.timing_mask = 0xffu;
+    return FSI_OK;
+}
+
+int fsi_pbf_set_timing_passes(fsi_pbf* h, uint32_t mask) {
+    if (!h) return fail(FSI_ERR_INVALID_ARG, "handle is NULL");
+    h->core.timing = mask != 0;
+    h->core.timing_mask = mask;
     return FSI_OK;
 }
 
@@ -539,6 +548,7 @@ int fsi_pbf_autotune(fsi_pbf* h, uint32_t frames, int* gather, int* digit_bits) 
     const StepConst c = step_const(h);
     h->core.d.batched = false;
     h->core.d.any_sdf = c.dom.enable_sdf == 1u;
+    h->core.d.tiled = c.dom.dshift == kTileShift;
     return core_autotune(h->core, c, frame_plan(h), frames, gather, digit_bits);
 }
 

@@ -69,8 +69,9 @@ struct Core {
     bool stepped = false;
     bool taps = false;
     int sort_digit_bits = kSortMaxDigitBits;
-    // timing
+    // timing: HIP events around the launches of the passes in timing_mask
     bool timing = false;
+    uint32_t timing_mask = 0xffu;  // bit FSI_PASS_* per pass
     std::vector<hipEvent_t> event_pool;
     std::vector<TimedPair> pending;
     fsi_pbf_timing acc{};
@@ -95,14 +96,17 @@ int core_drain_timing(Core& c);
 int core_set_tuning(Core& c, int knob, int value);
 int core_get_tuning(const Core& c, int knob, int* value);
 void core_invalidate_graph(Core& c);
-// Times `frames` frames under every knob combination, restoring the particle
-// state (pos, vel) after each trial; keeps and returns the fastest.
+// Times `frames` frames under the gather forms 0 and 1 and every radix digit
+// width, restoring the particle state (pos, vel) after each trial; keeps and
+// returns the fastest.
 int core_autotune(Core& c, const StepConst& k, const FramePlan& plan, uint32_t frames, int* gather, int* digit_bits);
 
 // Morton binning of one scene's processing order: cover its bounds (plus two
-// cells) with at most 2^axis_bits bins per axis, numbered from bin_base.
+// cells) with at most 2^axis_bits bins per axis, numbered from bin_base, each
+// bin as few cells as possible -- or, with `tiles`, 2^kTileShift cells per
+// axis when that covers the box (the tiles of the tiled gathers).
 void set_binning(Domain& d, const float bmin[3], const float bmax[3], float h, uint32_t axis_bits,
-                 uint32_t bin_base);
+                 uint32_t bin_base, bool tiles);
 
 // The step constants shared by every scene (FluidConfigUBO minus the box).
 StepConst step_const_common(const fsi_fluid_config& cfg, const fsi_pbf_constants& u, uint32_t n, bool interacting,

@@ -98,7 +98,7 @@ void pack_constants(const fsi_fluid_config& p, const float bmin[3], const float 
 }
 
 void set_binning(Domain& d, const float bmin[3], const float bmax[3], float h, uint32_t axis_bits,
-                 uint32_t bin_base) {
+                 uint32_t bin_base, bool tiles) {
     auto cell_clamped = [&](float x) {
         const double v = std::floor((double)x / (double)h);
         return (int64_t)std::max(-1.0e9, std::min(1.0e9, std::isfinite(v) ? v : 0.0));
@@ -110,11 +110,22 @@ void set_binning(Domain& d, const float bmin[3], const float bmax[3], float h, u
         d.dcmin[a] = (int32_t)lo;
         span = std::max<int64_t>(span, hi - lo + 1);
     }
+    d.bin_base = bin_base;
+    if (tiles) {
+        // bins of 2^kTileShift cells (the tiles of the tiled gathers) when
+        // axis_bits bins per axis can cover the box with them
+        uint32_t bits = 1;
+        while (bits < axis_bits && span > ((int64_t)1 << (bits + kTileShift))) ++bits;
+        if (span <= ((int64_t)1 << (bits + kTileShift))) {
+            d.dshift = kTileShift;
+            d.axis_bits = bits;
+            return;
+        }
+    }
     uint32_t shift = 0;
     while ((span >> shift) > ((int64_t)1 << axis_bits) && shift < 31) ++shift;
     d.dshift = shift;
     d.axis_bits = axis_bits;
-    d.bin_base = bin_base;
 }
 
 StepConst step_const_common(const fsi_fluid_config& cfg, const fsi_pbf_constants& u, uint32_t n, bool interacting,
@@ -200,6 +211,12 @@ void core_destroy(Core& c) {
     dev_free(&d.order);
     dev_free(&d.list_count);
     dev_free(&d.list);
+    dev_free(&d.tflag);
+    dev_free(&d.tidx);
+    dev_free(&d.tiles);
+    dev_free(&d.ntiles);
+    dev_free(&d.defer);
+    dev_free(&d.ndefer);
     dev_free(&d.sort.keys[0]);
     dev_free(&d.sort.keys[1]);
     dev_free(&d.sort.ids[0]);
@@ -256,6 +273,12 @@ hipError_t core_alloc_particles(Core& c, uint32_t n) {
     ALLOC(d.order, n);
     ALLOC(d.list_count, n);
     ALLOC(d.list, (size_t)kNbrCap * n);
+    ALLOC(d.tflag, n);
+    ALLOC(d.tidx, n);
+    ALLOC(d.tiles, (size_t)n + 1);
+    ALLOC(d.ntiles, 1);
+    ALLOC(d.defer, n);
+    ALLOC(d.ndefer, 1);
     ALLOC(d.sort.keys[0], n);
     ALLOC(d.sort.keys[1], n);
     ALLOC(d.sort.ids[0], n);
@@ -292,7 +315,7 @@ struct PassTimer {
     bool on;
     int pass;
     hipEvent_t a = nullptr, b = nullptr;
-    PassTimer(Core& cc, bool enabled, int p) : c(cc), on(enabled), pass(p) {
+    PassTimer(Core& cc, bool enabled, int p) : c(cc), on(enabled && ((cc.timing_mask >> p) & 1u)), pass(p) {
         if (!on) return;
         a = take_event(c);
         b = take_event(c);
@@ -470,7 +493,7 @@ int core_drain_timing(Core& c) {
 int core_set_tuning(Core& c, int knob, int value) {
     switch (knob) {
         case FSI_TUNE_GATHER:
-            if (value < 0 || value > 1) return fail(FSI_ERR_INVALID_ARG, "gather form must be 0 or 1");
+            if (value < 0 || value > 2) return fail(FSI_ERR_INVALID_ARG, "gather form must be 0, 1 or 2");
             c.d.variant = value;
             return FSI_OK;
         case FSI_TUNE_SORT_DIGIT_BITS:

@@ -108,6 +108,16 @@ __device__ __forceinline__ uint32_t morton_bin(const Domain& d, int32_t cx, int3
                          (spread7(clamp_bin(cz, d.dcmin[2], d.dshift, b)) << 2));
 }
 constexpr uint32_t kOrderBins = 1u << 21;
+// Inverse of spread7: the 7-bit coordinate held in every third bit.
+__device__ __forceinline__ uint32_t compact7(uint32_t v) {
+    v &= 0x09249249u;
+    v = (v | (v >> 2)) & 0x030c30c3u;
+    v = (v | (v >> 4)) & 0x0300f00fu;
+    v = (v | (v >> 8)) & 0x7fu;
+    return v;
+}
+// Bins of 2^kTileShift cells per axis are the tiles of the tiled gathers.
+constexpr uint32_t kTileShift = 2;
 
 // Poly6 (CS_ComputeLambdas.glsl:74-81) given r2.
 __device__ __forceinline__ float poly6_r2(const StepConst& c, float r2) {

@@ -223,7 +223,22 @@ __global__ __launch_bounds__(kBlock) void k_reorder(StepConst c, const uint32_t*
     cell_of(xyz(x), c.h, cx, cy, cz);
     const uint32_t b = morton_bin(domain_of<B>(c, k), cx, cy, cz);
     bin_of[k] = b;
-    bin_rank[k] = atomicAdd(&bins[b], 1u);
+    // one atomic per distinct bin in the wave (the members of a hash bucket
+    // are consecutive slots and share a bin)
+    const uint32_t lane = __lane_id();
+    unsigned long long active = __ballot(1);
+    uint32_t rank = 0;
+    while (active) {
+        const int leader = __ffsll((long long)active) - 1;
+        const uint32_t lb = __shfl(b, leader);
+        const unsigned long long same = __ballot(b == lb) & active;
+        uint32_t base = 0;
+        if ((int)lane == leader) base = atomicAdd(&bins[lb], (uint32_t)__popcll(same));
+        base = __shfl(base, leader);
+        if ((same >> lane) & 1ull) rank = base + (uint32_t)__popcll(same & ((1ull << lane) - 1ull));
+        active &= ~same;
+    }
+    bin_rank[k] = rank;
 }
 
 __global__ __launch_bounds__(kBlock) void k_fill_u32(uint32_t* __restrict__ p, uint32_t n, uint32_t v) {
@@ -546,8 +561,374 @@ __global__ __launch_bounds__(kBlock) void k_contact_write(StepConst c, const flo
     if (id == c.n - 1) count[0] = slots[id] + (hit ? 1u : 0u);
 }
 
+// ---- tiled gathers (gather form 2) -----------------------------------------
+// The Morton bins of 4x4x4 cells are tiles.  One workgroup takes a tile: it
+// loads the cell-table entries of the 6x6x6 cells around the tile and copies
+// those buckets' records into LDS once, in bucket order; every particle of the
+// tile whose current cell is inside the tile then visits its 27 buckets from
+// LDS, in the reference order (an aliased bucket appears once per halo cell
+// that hashes to it, as it is visited once per occurrence).  A particle that
+// has moved out of its tile, or a tile whose buckets exceed the LDS capacity,
+// takes the global single-loop gather.  Tiles are spread over the XCDs in
+// contiguous Morton ranges.
+
+constexpr int kTileW = 1 << kTileShift;
+constexpr int kHaloW = kTileW + 2;
+constexpr int kHaloCells = kHaloW * kHaloW * kHaloW;
+constexpr uint32_t kTileCap = 2560;  // staged records per tile (40 KiB)
+constexpr uint32_t kTileGrid = 768;  // persistent workgroups: 3 per CU, a multiple of 8
+constexpr uint32_t kDeferGrid = 256; // workgroups of the deferred passes
+
+struct TileStage {
+    int32_t o[3];  // first cell of the tile
+    uint32_t total;
+    int32_t hstart[kHaloCells];  // first slot of the bucket of halo cell h, -1 empty
+    uint32_t hcnt[kHaloCells];
+    uint32_t hoff[kHaloCells];   // its first record in x[]
+    float4 x[kTileCap];
+};
+
+// Tile range of this workgroup's XCD and its first tile and stride in it.
+__device__ __forceinline__ void tile_span(uint32_t nt, uint32_t& t, uint32_t& end, uint32_t& stride) {
+    const uint32_t x = blockIdx.x & 7u, lb = blockIdx.x >> 3;
+    t = (uint32_t)(((uint64_t)nt * x) >> 3) + lb;
+    end = (uint32_t)(((uint64_t)nt * (x + 1)) >> 3);
+    stride = gridDim.x >> 3;
+}
+
+// Stages tile [start, ...) of the processing order; returns whether its
+// buckets fit.  Ends with a barrier.
+template <bool B>
+__device__ bool stage_tile(const StepConst& c, TileStage& ts, const float4* __restrict__ X,
+                           const int2* __restrict__ cells, const uint32_t* __restrict__ order,
+                           const uint32_t* __restrict__ bin_of, uint32_t start) {
+    const uint32_t k0 = order[start];
+    const Domain& dm = domain_of<B>(c, k0);
+    const uint32_t code = bin_of[k0] - dm.bin_base;
+    const int32_t ox = dm.dcmin[0] + (int32_t)(compact7(code) << kTileShift);
+    const int32_t oy = dm.dcmin[1] + (int32_t)(compact7(code >> 1) << kTileShift);
+    const int32_t oz = dm.dcmin[2] + (int32_t)(compact7(code >> 2) << kTileShift);
+    const uint32_t tid = threadIdx.x;
+    if (tid < (uint32_t)kHaloCells) {
+        const int32_t hx = (int32_t)(tid % kHaloW), hy = (int32_t)((tid / kHaloW) % kHaloW),
+                      hz = (int32_t)(tid / (kHaloW * kHaloW));
+        const int2 se = cells[dm.hash_base + cell_hash(ox - 1 + hx, oy - 1 + hy, oz - 1 + hz, dm.hash_mask)];
+        ts.hstart[tid] = se.x;
+        ts.hcnt[tid] = se.x < 0 ? 0u : (uint32_t)(se.y - se.x);
+    }
+    if (tid == 0) {
+        ts.o[0] = ox;
+        ts.o[1] = oy;
+        ts.o[2] = oz;
+    }
+    __syncthreads();
+    if (tid < 64) {  // wave 0: exclusive scan of the counts, four per lane
+        uint32_t local = 0;
+        for (int j = 0; j < 4; ++j) {
+            const uint32_t h = 4 * tid + j;
+            if (h < (uint32_t)kHaloCells) local += ts.hcnt[h];
+        }
+        uint32_t v = local;
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t u = __shfl_up(v, d, 64);
+            if ((int)tid >= d) v += u;
+        }
+        uint32_t run = v - local;
+        for (int j = 0; j < 4; ++j) {
+            const uint32_t h = 4 * tid + j;
+            if (h < (uint32_t)kHaloCells) {
+                ts.hoff[h] = run;
+                run += ts.hcnt[h];
+            }
+        }
+        if (tid == 63) ts.total = v;
+    }
+    __syncthreads();
+    const uint32_t total = ts.total;
+    const bool ok = total <= kTileCap;
+    if (ok) {
+        for (uint32_t e = tid; e < total; e += kBlock) {
+            uint32_t lo = 0, hi = kHaloCells - 1;  // last halo cell whose records start at or before e
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi + 1) >> 1;
+                if (ts.hoff[mid] <= e) lo = mid;
+                else hi = mid - 1;
+            }
+            ts.x[e] = X[ts.hstart[lo] + (int32_t)(e - ts.hoff[lo])];
+        }
+    }
+    __syncthreads();
+    return ok;
+}
+
+// Phase 1 of the two-phase gather over the staged tile: as collect_flush, with
+// candidates read from LDS and their record numbers in x[] collected.  (lx,
+// ly, lz) is the particle's cell relative to the tile, each in [0, kTileW).
+template <bool TAPS, class Flush>
+__device__ __forceinline__ uint32_t collect_flush_tile(const TileStage& ts, V3 x, float h, int32_t lx, int32_t ly,
+                                                       int32_t lz, uint32_t self, uint16_t* col, Flush&& flush,
+                                                       uint32_t& ncand) {
+    const float h2 = h * h;
+    uint32_t cnt = 0, total = 0;
+#pragma unroll 1
+    for (int q = 0; q < 27; ++q) {
+        const int hc = (lx + q % 3) + kHaloW * (ly + (q / 3) % 3) + kHaloW * kHaloW * (lz + q / 9);
+        const uint32_t nb = ts.hcnt[hc];
+        const uint32_t base = ts.hoff[hc];
+        const uint32_t s0 = (uint32_t)ts.hstart[hc];
+        for (uint32_t e = 0; e < nb; ++e) {
+            if (s0 + e == self) continue;
+            if (TAPS) ++ncand;
+            const V3 r = sub(x, xyz(ts.x[base + e]));
+            if (dot(r, r) <= h2) {
+                col[cnt * kBlock] = (uint16_t)(base + e);
+                if (++cnt == kLdsCap) {
+                    flush(total, cnt);
+                    total += cnt;
+                    cnt = 0;
+                }
+            }
+        }
+        if (__any(cnt >= kFlushAt)) {
+            flush(total, cnt);
+            total += cnt;
+            cnt = 0;
+        }
+    }
+    flush(total, cnt);
+    return total + cnt;
+}
+
+// The end of CS_ComputeLambdas.glsl:108-162 for one particle, given its sums.
+template <bool TAPS>
+__device__ __forceinline__ void lambda_store(const StepConst& c, uint32_t k, V3 xi, float rho, float sum_g2, V3 gi,
+                                             uint32_t ncand, uint32_t nnbr, float4* __restrict__ PL,
+                                             float* __restrict__ rho_out, uint32_t* __restrict__ cand_out,
+                                             uint32_t* __restrict__ nbr_out) {
+    gi = scl(gi, c.inv_rho0);
+    float C = rho / c.rho0 - 1.0f;
+    const float floorc = -c.cohesion;
+    C = C > floorc ? C : floorc;
+    sum_g2 = sum_g2 + dot(gi, gi);
+    const float lambda = -C / (sum_g2 + c.eps);
+    PL[k] = make_float4(xi.x, xi.y, xi.z, lambda);
+    rho_out[k] = rho;
+    if (TAPS) {
+        cand_out[k] = ncand;
+        nbr_out[k] = nnbr;
+    }
+}
+
+// CS_ComputeLambdas.glsl:108-162, tiled.  Particles outside their tile are
+// appended to defer[] for k_lambda_deferred.
+template <bool TAPS, bool B>
+__global__ __launch_bounds__(kBlock) void k_lambda_tile(StepConst c, const uint32_t* __restrict__ tiles,
+                                                        const uint32_t* __restrict__ ntiles,
+                                                        const uint32_t* __restrict__ order,
+                                                        const uint32_t* __restrict__ bin_of,
+                                                        const float4* __restrict__ P, float4* __restrict__ PL,
+                                                        float* __restrict__ rho_out, const uint32_t* __restrict__ lod,
+                                                        const int2* __restrict__ cells, uint32_t iter,
+                                                        uint32_t* __restrict__ cand_out, uint32_t* __restrict__ nbr_out,
+                                                        uint32_t* __restrict__ defer, uint32_t* __restrict__ ndefer) {
+    __shared__ TileStage ts;
+    __shared__ uint16_t lst[kLdsCap * kBlock];
+    uint16_t* col = &lst[threadIdx.x];
+    uint32_t tile, tend, stride;
+    tile_span(*ntiles, tile, tend, stride);
+    const float inv = c.inv_rho0;
+    for (; tile < tend; tile += stride) {
+        const uint32_t start = tiles[tile], end = tiles[tile + 1];
+        const bool ok = stage_tile<B>(c, ts, P, cells, order, bin_of, start);
+        for (uint32_t t = start + threadIdx.x; t < end; t += kBlock) {
+            const uint32_t k = order[t];
+            const V3 xi = xyz(P[k]);
+            if (c.enable_apbf != 0u && lod[k] < iter) {
+                PL[k] = make_float4(xi.x, xi.y, xi.z, PL[k].w);
+                continue;
+            }
+            int32_t cx, cy, cz;
+            cell_of(xi, c.h, cx, cy, cz);
+            const int32_t lx = cx - ts.o[0], ly = cy - ts.o[1], lz = cz - ts.o[2];
+            if (!(ok && lx >= 0 && lx < kTileW && ly >= 0 && ly < kTileW && lz >= 0 && lz < kTileW)) {
+                defer[atomicAdd(ndefer, 1u)] = k;
+                continue;
+            }
+            float rho = c.w0_self;
+            float sum_g2 = 0.0f;
+            V3 gi = mk(0.0f, 0.0f, 0.0f);
+            uint32_t ncand = 0;
+            const uint32_t nnbr = collect_flush_tile<TAPS>(ts, xi, c.h, lx, ly, lz, k, col, [&](uint32_t, uint32_t cnt) {
+#pragma unroll 1
+                for (uint32_t e = 0; e < cnt; ++e) {
+                    const V3 d = sub(xi, xyz(ts.x[col[e * kBlock]]));
+                    const float r2 = dot(d, d);
+                    rho = rho + poly6_r2(c, r2);
+                    const V3 gw = spiky(c, d, r2);
+                    const float gj2 = dot(gw, gw);
+                    sum_g2 = sum_g2 + gj2 * (inv * inv);
+                    gi = add(gi, gw);
+                }
+            }, ncand);
+            lambda_store<TAPS>(c, k, xi, rho, sum_g2, gi, ncand, nnbr, PL, rho_out, cand_out, nbr_out);
+        }
+        __syncthreads();
+    }
+}
+
+// The particles the tiled lambda pass deferred, single-loop gather from HBM.
+template <bool TAPS, bool B>
+__global__ __launch_bounds__(kBlock) void k_lambda_deferred(StepConst c, const uint32_t* __restrict__ defer,
+                                                            const uint32_t* __restrict__ ndefer,
+                                                            const float4* __restrict__ P, float4* __restrict__ PL,
+                                                            float* __restrict__ rho_out, const int2* __restrict__ cells,
+                                                            uint32_t* __restrict__ cand_out,
+                                                            uint32_t* __restrict__ nbr_out) {
+    const uint32_t nd = *ndefer;
+    const float inv = c.inv_rho0;
+    const float h2 = c.h * c.h;
+    for (uint32_t i = gid(); i < nd; i += gridDim.x * kBlock) {
+        const uint32_t k = defer[i];
+        const V3 xi = xyz(P[k]);
+        float rho = c.w0_self;
+        float sum_g2 = 0.0f;
+        V3 gi = mk(0.0f, 0.0f, 0.0f);
+        uint32_t ncand = 0, nnbr = 0;
+        scan_buckets(xi, c.h, domain_of<B>(c, k), cells, P, k, [&](uint32_t, float4, V3 r, float r2) {
+            if (TAPS) ++ncand;
+            if (r2 <= h2) {
+                if (TAPS) ++nnbr;
+                rho = rho + poly6_r2(c, r2);
+                const V3 gw = spiky(c, r, r2);
+                const float gj2 = dot(gw, gw);
+                sum_g2 = sum_g2 + gj2 * (inv * inv);
+                gi = add(gi, gw);
+            }
+        });
+        lambda_store<TAPS>(c, k, xi, rho, sum_g2, gi, ncand, nnbr, PL, rho_out, cand_out, nbr_out);
+    }
+}
+
+// The clamp and add of CS_ApplyDeltaP.glsl:80-106 for one particle.
+template <bool TAPS>
+__device__ __forceinline__ void deltap_store(const StepConst& c, uint32_t k, V3 xi, V3 dp, float4* __restrict__ P,
+                                             float4* __restrict__ dp_tap) {
+    dp = scl(dp, c.inv_rho0);
+    if (TAPS) dp_tap[k] = make_float4(dp.x, dp.y, dp.z, 0.0f);
+    const float maxdp = 0.5f * c.h;
+    const float l2 = dot(dp, dp);
+    const float max2 = maxdp * maxdp;
+    if (l2 > max2) {
+        const float l = sqrtf(l2);
+        dp = scl(dp, maxdp / l);
+    }
+    const V3 p = add(xi, dp);
+    P[k] = make_float4(p.x, p.y, p.z, 0.0f);
+}
+
+// CS_ComputeDeltaP.glsl:120-166 + the clamp and add of CS_ApplyDeltaP.glsl:80-106,
+// tiled; particles outside their tile go to defer[] for k_deltap_deferred.
+template <bool TAPS, bool B>
+__global__ __launch_bounds__(kBlock) void k_deltap_tile(StepConst c, const uint32_t* __restrict__ tiles,
+                                                        const uint32_t* __restrict__ ntiles,
+                                                        const uint32_t* __restrict__ order,
+                                                        const uint32_t* __restrict__ bin_of,
+                                                        const float4* __restrict__ PL, float4* __restrict__ P,
+                                                        const uint32_t* __restrict__ lod,
+                                                        const int2* __restrict__ cells, uint32_t iter,
+                                                        float4* __restrict__ dp_tap, uint32_t* __restrict__ defer,
+                                                        uint32_t* __restrict__ ndefer) {
+    __shared__ TileStage ts;
+    __shared__ uint16_t lst[kLdsCap * kBlock];
+    uint16_t* col = &lst[threadIdx.x];
+    uint32_t tile, tend, stride;
+    tile_span(*ntiles, tile, tend, stride);
+    for (; tile < tend; tile += stride) {
+        const uint32_t start = tiles[tile], end = tiles[tile + 1];
+        const bool ok = stage_tile<B>(c, ts, PL, cells, order, bin_of, start);
+        for (uint32_t t = start + threadIdx.x; t < end; t += kBlock) {
+            const uint32_t k = order[t];
+            if (c.enable_apbf != 0u && lod[k] < iter) continue;  // P[k] already holds x
+            const float4 xi4 = PL[k];
+            const V3 xi = xyz(xi4);
+            int32_t cx, cy, cz;
+            cell_of(xi, c.h, cx, cy, cz);
+            const int32_t lx = cx - ts.o[0], ly = cy - ts.o[1], lz = cz - ts.o[2];
+            if (!(ok && lx >= 0 && lx < kTileW && ly >= 0 && ly < kTileW && lz >= 0 && lz < kTileW)) {
+                defer[atomicAdd(ndefer, 1u)] = k;
+                continue;
+            }
+            const float li = xi4.w;
+            V3 dp = mk(0.0f, 0.0f, 0.0f);
+            uint32_t ncand = 0;
+            collect_flush_tile<false>(ts, xi, c.h, lx, ly, lz, k, col, [&](uint32_t, uint32_t cnt) {
+#pragma unroll 1
+                for (uint32_t e = 0; e < cnt; ++e) {
+                    const float4 a = ts.x[col[e * kBlock]];
+                    const V3 d = sub(xi, xyz(a));
+                    const float r2 = dot(d, d);
+                    const V3 gw = spiky(c, d, r2);
+                    const float sc = scorr_r2(c, r2);
+                    dp = add(dp, scl(gw, li + a.w + sc));
+                }
+            }, ncand);
+            deltap_store<TAPS>(c, k, xi, dp, P, dp_tap);
+        }
+        __syncthreads();
+    }
+}
+
+template <bool TAPS, bool B>
+__global__ __launch_bounds__(kBlock) void k_deltap_deferred(StepConst c, const uint32_t* __restrict__ defer,
+                                                            const uint32_t* __restrict__ ndefer,
+                                                            const float4* __restrict__ PL, float4* __restrict__ P,
+                                                            const int2* __restrict__ cells,
+                                                            float4* __restrict__ dp_tap) {
+    const uint32_t nd = *ndefer;
+    const float h2 = c.h * c.h;
+    for (uint32_t i = gid(); i < nd; i += gridDim.x * kBlock) {
+        const uint32_t k = defer[i];
+        const float4 xi4 = PL[k];
+        const V3 xi = xyz(xi4);
+        const float li = xi4.w;
+        V3 dp = mk(0.0f, 0.0f, 0.0f);
+        scan_buckets(xi, c.h, domain_of<B>(c, k), cells, PL, k, [&](uint32_t, float4 xj, V3 r, float r2) {
+            if (r2 <= h2) {
+                const V3 gw = spiky(c, r, r2);
+                const float sc = scorr_r2(c, r2);
+                dp = add(dp, scl(gw, li + xj.w + sc));
+            }
+        });
+        deltap_store<TAPS>(c, k, xi, dp, P, dp_tap);
+    }
+}
+
+// Tile list: the positions of the processing order where the bin changes.
+__global__ __launch_bounds__(kBlock) void k_tile_flags(uint32_t n, const uint32_t* __restrict__ order,
+                                                       const uint32_t* __restrict__ bin_of,
+                                                       uint32_t* __restrict__ tflag) {
+    const uint32_t t = gid();
+    if (t >= n) return;
+    tflag[t] = (t == 0 || bin_of[order[t]] != bin_of[order[t - 1]]) ? 1u : 0u;
+}
+
+__global__ __launch_bounds__(kBlock) void k_tile_list(uint32_t n, const uint32_t* __restrict__ tflag,
+                                                      const uint32_t* __restrict__ tidx, uint32_t* __restrict__ tiles,
+                                                      uint32_t* __restrict__ ntiles) {
+    const uint32_t t = gid();
+    if (t >= n) return;
+    if (tflag[t]) tiles[tidx[t]] = t;
+    if (t == n - 1) {
+        const uint32_t nt = tidx[t] + tflag[t];
+        ntiles[0] = nt;
+        tiles[nt] = n;
+    }
+}
+
 }  // namespace
 
+#define L_T(B) L(true, B)
+#define L_F(B) L(false, B)
 #define FSI_B_SWITCH(b, CALL) \
     if (b) {                      \
         CALL(true);               \
@@ -578,6 +959,12 @@ hipError_t launch_reorder(const StepConst& c, DeviceState& d, hipStream_t st) {
     hipError_t e = scan_exclusive_u32(d.bins, d.bins, kOrderBins, d.sort.partial, st);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_order, grid_for(c.n), dim3(kBlock), 0, st, c.n, d.bins, d.bin_of, d.bin_rank, d.order);
+    if (d.variant == 2 && d.tiled) {
+        hipLaunchKernelGGL(k_tile_flags, grid_for(c.n), dim3(kBlock), 0, st, c.n, d.order, d.bin_of, d.tflag);
+        e = scan_exclusive_u32(d.tflag, d.tidx, c.n, d.sort.partial, st);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_tile_list, grid_for(c.n), dim3(kBlock), 0, st, c.n, d.tflag, d.tidx, d.tiles, d.ntiles);
+    }
     return hipGetLastError();
 }
 
@@ -588,13 +975,28 @@ hipError_t launch_lod(const StepConst& c, DeviceState& d, hipStream_t st) {
 
 // gather form V (tuning knob) x taps x batched
 #define FSI_GATHER_SWITCH(CALL)                         \
-    if (d.variant == 1) {                               \
+    if (d.variant >= 1) {                               \
         if (d.batched) { CALL(1, true); } else { CALL(1, false); } \
     } else {                                            \
         if (d.batched) { CALL(0, true); } else { CALL(0, false); } \
     }
 
 hipError_t launch_lambda(const StepConst& c, DeviceState& d, uint32_t iter, bool taps, hipStream_t st) {
+    if (d.variant == 2 && d.tiled) {
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(kBlock), 0, st, d.ndefer, 1u, 0u);
+#define L(T, B)                                                                                                  \
+    hipLaunchKernelGGL((k_lambda_tile<T, B>), dim3(kTileGrid), dim3(kBlock), 0, st, c, d.tiles, d.ntiles, d.order, \
+                       d.bin_of, d.P, d.PL, d.rho, d.lod, d.cells, iter, d.cand, d.nbr, d.defer, d.ndefer);     \
+    hipLaunchKernelGGL((k_lambda_deferred<T, B>), dim3(kDeferGrid), dim3(kBlock), 0, st, c, d.defer, d.ndefer, d.P, \
+                       d.PL, d.rho, d.cells, d.cand, d.nbr);
+        if (taps) {
+            FSI_B_SWITCH(d.batched, L_T)
+        } else {
+            FSI_B_SWITCH(d.batched, L_F)
+        }
+#undef L
+        return hipGetLastError();
+    }
 #define L(V, B)                                                                                                 \
     if (taps)                                                                                                   \
         hipLaunchKernelGGL((k_lambda<V, true, B>), grid_for(c.n), dim3(kBlock), 0, st, c, d.order, d.P, d.PL,   \
@@ -608,15 +1010,30 @@ hipError_t launch_lambda(const StepConst& c, DeviceState& d, uint32_t iter, bool
 }
 
 hipError_t launch_deltap(const StepConst& c, DeviceState& d, uint32_t iter, bool taps, hipStream_t st) {
+    if (d.variant == 2 && d.tiled) {
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(kBlock), 0, st, d.ndefer, 1u, 0u);
+#define L(T, B)                                                                                                   \
+    hipLaunchKernelGGL((k_deltap_tile<T, B>), dim3(kTileGrid), dim3(kBlock), 0, st, c, d.tiles, d.ntiles, d.order,   \
+                       d.bin_of, d.PL, d.P, d.lod, d.cells, iter, d.dp_tap, d.defer, d.ndefer);                  \
+    hipLaunchKernelGGL((k_deltap_deferred<T, B>), dim3(kDeferGrid), dim3(kBlock), 0, st, c, d.defer, d.ndefer, d.PL, \
+                       d.P, d.cells, d.dp_tap);
+        if (taps) {
+            FSI_B_SWITCH(d.batched, L_T)
+        } else {
+            FSI_B_SWITCH(d.batched, L_F)
+        }
+#undef L
+    } else {
 #define L(V, B)                                                                                                 \
     if (taps)                                                                                                   \
         hipLaunchKernelGGL((k_deltap<V, true, B>), grid_for(c.n), dim3(kBlock), 0, st, c, d.order, d.PL, d.P,   \
                            d.lod, d.cells, iter, d.list_count, d.list, d.dp_tap);                               \
     else                                                                                                        \
-        hipLaunchKernelGGL((k_deltap<V, false, B>), grid_for(c.n), dim3(kBlock), 0, st, c, d.order, d.PL, d.P,  \
+        hipLaunchKernelGGL((k_deltap<V, false, B>), grid_for(c.n), dim3(kBlock), 0, st, c, d.order, d.PL, d.P,   \
                            d.lod, d.cells, iter, d.list_count, d.list, d.dp_tap);
-    FSI_GATHER_SWITCH(L)
+        FSI_GATHER_SWITCH(L)
 #undef L
+    }
     if (d.any_sdf) {
 #define L(B) hipLaunchKernelGGL((k_apply_sdf<B>), grid_for(c.n), dim3(kBlock), 0, st, c, d.order, d.P, d.lod, iter);
         FSI_B_SWITCH(d.batched, L)

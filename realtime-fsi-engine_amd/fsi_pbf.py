@@ -178,6 +178,8 @@ def load() -> C.CDLL:
         "fsi_pbf_set_taps": ([P, C.c_int], C.c_int),
         "fsi_pbf_read_counts": ([P, u32p, u32p], C.c_int),
         "fsi_pbf_set_timing": ([P, C.c_int], C.c_int),
+        "fsi_pbf_set_timing_passes": ([P, C.c_uint32], C.c_int),
+        "fsi_batch_set_timing_passes": ([P, C.c_uint32], C.c_int),
         "fsi_pbf_timing_reset": ([P], C.c_int),
         "fsi_pbf_read_timing": ([P, C.POINTER(CTiming)], C.c_int),
         "fsi_pbf_get_stream": ([P, C.POINTER(P)], C.c_int),
@@ -416,6 +418,13 @@ class Solver:
 
     def set_timing(self, on: bool):
         _check(self.lib.fsi_pbf_set_timing(self.h, 1 if on else 0))
+
+    def set_timing_passes(self, names):
+        """Time only the named passes (see PASS_NAMES)."""
+        mask = 0
+        for nm in names:
+            mask |= 1 << PASS_NAMES.index(nm)
+        _check(self.lib.fsi_pbf_set_timing_passes(self.h, mask))
 
     def timing_reset(self):
         _check(self.lib.fsi_pbf_timing_reset(self.h))

@@ -57,12 +57,13 @@ def run_single(base, sc, frames):
     return out, grid
 
 
-@pytest.mark.parametrize("frames,graph", [(1, False), (3, True)])
-def test_batch_scenes_equal_single_scene_runs(gpu, frames, graph):
+@pytest.mark.parametrize("frames,graph,gather", [(1, False, 1), (3, True, 1), (3, True, 2)])
+def test_batch_scenes_equal_single_scene_runs(gpu, frames, graph, gather):
     fsi = _fsi()
     base = dam_break_config(n_iter=3, substeps=2)
     scenes = make_scenes()
     b = fsi.Batch(base)
+    b.set_tuning(fsi.TUNE_GATHER, gather)
     b.set_graph(graph)
     b.set_scenes(scenes)
     b.step(frames)
@@ -116,7 +117,7 @@ def test_batch_autotune_restores_state_and_graph_matches(gpu):
     b = fsi.Batch(base)
     b.set_scenes(scenes)
     gather, bits = b.autotune(1)
-    assert gather in (0, 1) and bits in (6, 8, 10)
+    assert gather in (0, 1, 2) and bits in (6, 8, 10)
     assert b.get_tuning(fsi.TUNE_GATHER) == gather and b.get_tuning(fsi.TUNE_SORT_DIGIT_BITS) == bits
     b.set_graph(True)
     b.set_timing(True)

@@ -29,9 +29,11 @@ def _oracle():
 
 
 def run_pair(cfg, parts, frames=1, taps=True, sdf=None, sdf_padding=0.0, containment=False, interaction=None,
-             camera=(0.0, 0.0, 0.0), collision_padding=0.0):
+             camera=(0.0, 0.0, 0.0), collision_padding=0.0, gather=None):
     fsi = _fsi()
     s = fsi.Solver(cfg)
+    if gather is not None:
+        s.set_tuning(fsi.TUNE_GATHER, gather)
     if collision_padding:
         s.set_collision_padding(collision_padding)
         s.set_params(cfg)
@@ -108,15 +110,16 @@ SCENES = {
 }
 
 
+@pytest.mark.parametrize("gather", [1, 2])
 @pytest.mark.parametrize("name", list(SCENES))
-def test_one_substep_grid_counts_positions(gpu, name):
+def test_one_substep_grid_counts_positions(gpu, name, gather):
     fsi = _fsi()
     dims, H, iters = SCENES[name]
     cfg = dam_break_config(n_iter=iters, substeps=1, hash_size=H)
     parts = fsi.lattice(*dims, (0.0, 0.0, 0.0), 0.05, 0.001, 1337)
     if name == "ragged_4999":
         parts = parts[:4999].copy()
-    s, ref = run_pair(cfg, parts)
+    s, ref = run_pair(cfg, parts, gather=gather)
     assert_grid_equal(s, ref)
     cand, nbr = s.read_counts()
     assert np.array_equal(cand, ref.cand), "candidate counts"
@@ -142,12 +145,13 @@ def test_full_step_two_substeps_vorticity(gpu):
     s.close()
 
 
-def test_c1_ten_frames(gpu):
+@pytest.mark.parametrize("gather", [1, 2])
+def test_c1_ten_frames(gpu, gather):
     """The 32,768-particle dam break over ten frames (I=4, S=2)."""
     fsi = _fsi()
     cfg = dam_break_config(n_iter=4, substeps=2)
     parts = fsi.lattice(32, 32, 32, (0.0, 0.0, 0.0), 0.05, 0.001, 1337)
-    s, ref = run_pair(cfg, parts, frames=10, taps=False)
+    s, ref = run_pair(cfg, parts, frames=10, taps=False, gather=gather)
     assert_state_close(s, ref, cfg, frames=10, label="c1 ten frames")
     s.close()
 
@@ -276,7 +280,7 @@ def test_knobs_and_graph_do_not_change_results(gpu, graph):
     ref = _oracle().OracleSim(cfg, parts, taps=False, sdf=sdf, sdf_padding=0.025)
     ref.step(2)
     exp = ref.particles()
-    for gather in (0, 1):
+    for gather in (0, 1, 2):
         for bits in (6, 8, 10):
             with fsi.Solver(cfg) as s:
                 s.set_particles(parts)
