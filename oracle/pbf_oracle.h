@@ -24,11 +24,13 @@
  *     nested (1-a)*t0 + a*t1 blends along x, then y, then z.
  *   - GLSL round() in CS_ComputeLOD.glsl:76 rounds half away from zero.
  *
- * Parity pins (tests/test_oracle_*.py): closed forms of the kernels and the
+ * Parity pins (tests/test_oracle_pins.py): closed forms of the kernels and the
  * coefficient block, integer-hash brute force, stable-sort invariants,
  * O(N^2) brute-force neighbour visits including aliased duplicates, isolated
- * particle / particle pair closed forms, ballistic flight, wall damping and
- * an analytic sphere SDF.  The reference holds no golden vectors or tests for
+ * particle / particle pair closed forms, ballistic flight, wall damping, an
+ * analytic sphere SDF, the XSPH pair closed form, the curl of a rigid
+ * rotation, vorticity confinement on a linear |w| field, the LOD and
+ * interaction-force closed forms.  The reference holds no golden vectors or tests for
  * this path, and its shaders cannot run here (no GL context), so any result
  * those pins leave open is "parity unpinned".
  */

@@ -229,6 +229,9 @@ class OracleSim:
     def apply_pass(self, it: int = 1):
         self._call("apply", C.byref(self.ubo), C.byref(self.uni), C.byref(self.st), it)
 
+    def lod_pass(self):
+        self._call("lod", C.byref(self.ubo), C.byref(self.uni), C.byref(self.st))
+
     def integrate(self):
         self._call("integrate", C.byref(self.ubo), C.byref(self.uni), C.byref(self.st))
 

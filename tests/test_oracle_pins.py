@@ -362,3 +362,136 @@ def test_contact_filter_band_and_order():
     # id order: the x positions appear in the order of the input
     ids = [int(np.nonzero(np.all(parts[:, 0:3] == g[0:3], axis=1))[0][0]) for g in got]
     assert ids == sorted(ids)
+
+
+# ---------------------------------------------------------------- XSPH, curl, vorticity, LOD, interaction
+
+
+def _pair(cfg, xi, xj, vi, vj):
+    parts = np.zeros((2, 12), dtype=np.float32)
+    parts[0, 0:3], parts[1, 0:3] = xi, xj
+    parts[0, 8:11], parts[1, 8:11] = vi, vj
+    parts[:, 4:7] = parts[:, 0:3]
+    return parts
+
+
+def test_xsph_pair_closed_form_and_momentum():
+    """Two particles, no gravity, lambda ~ 0 (eps = 1e30), no s_corr: the
+    positions are x* = x + v dt and the velocity is v + c/rho0 (v_j - v_i) W(r)
+    (CS_Integrate.glsl:135-280), so the pair's momentum is conserved."""
+    c_visc = 0.2
+    cfg = dam_break_config(substeps=1, n_iter=1, gravity=(0.0, 0.0, 0.0), eps=1e30, enable_scorr=False,
+                           enable_vorticity=False, viscosity=c_visc, box=(4.0, 4.0, 4.0), hash_size=1 << 20)
+    vi, vj = np.array([0.3, -0.1, 0.2]), np.array([-0.2, 0.4, 0.1])
+    parts = _pair(cfg, (1.0, 1.0, 1.0), (1.04, 1.01, 0.98), vi, vj)
+    sim = O().OracleSim(cfg, parts, precision="f64")
+    sim.step(1)
+    u = sim.ubo
+    x0 = parts[:, 0:3].astype(np.float64)
+    v0 = parts[:, 8:11].astype(np.float64)
+    xs = x0 + v0 * u.sub_dt
+    r2 = float(((xs[0] - xs[1]) ** 2).sum())
+    W = u.poly6_coeff * (u.h * u.h - r2) ** 3
+    assert W > 0
+    expect_i = v0[0] + c_visc * u.inv_rho0 * (v0[1] - v0[0]) * W
+    np.testing.assert_allclose(sim.vel[0, 0:3], expect_i, rtol=0, atol=1e-9)
+    np.testing.assert_allclose(sim.vel[0, 0:3] + sim.vel[1, 0:3], v0[0] + v0[1], rtol=0, atol=1e-9)
+    np.testing.assert_allclose(sim.pos[:, 0:3], xs, rtol=0, atol=1e-12)
+
+
+def _still_lattice(cfg, side, origin, vel_fn):
+    parts = fsi().lattice(side, side, side, origin, 0.05, 0.0)
+    x = parts[:, 0:3].astype(np.float64)
+    parts[:, 8:11] = vel_fn(x)
+    parts[:, 4:7] = parts[:, 0:3]
+    return parts
+
+
+def test_curl_of_rigid_rotation_is_minus_two_omega():
+    """For a rigid rotation v = W x (x - c), (v_j - v_i) = -W x r and, with
+    V_j = 1/rho0, the SPH identity sum_j V_j r (x) grad W ~ -(rho/rho0) I
+    gives curl = -2 W rho/rho0 for an interior particle (the gradient is
+    taken with r = x_i - x_j, CS_Integrate.glsl:171-193).  The lattice sums
+    of the spiky gradient and of poly6 differ from their integrals by a few
+    per cent (5.3% at spacing h/2); a flipped sign or cross order would give
+    +2 W, a dropped 1/rho0 a factor of rho0."""
+    cfg = dam_break_config(substeps=1, n_iter=1, gravity=(0.0, 0.0, 0.0), eps=1e30, enable_scorr=False,
+                           box=(4.0, 4.0, 4.0), hash_size=1 << 20)
+    omega = np.array([0.0, 0.0, 2.0])
+    centre = np.array([1.4, 1.4, 1.4])
+    parts = _still_lattice(cfg, 16, (1.0, 1.0, 1.0), lambda x: np.cross(omega, x - centre))
+    sim = O().OracleSim(cfg, parts, precision="f64")
+    sim.step(1)
+    x = parts[:, 0:3].astype(np.float64)
+    interior = (np.abs(x - centre) < 0.4 - 0.2 + 1e-9).all(1)
+    assert interior.sum() > 100
+    curl = sim.solver[interior, 4:7]
+    rho = sim.solver[interior, 7]
+    expect = -2.0 * omega[None, :] * (rho / sim.ubo.rho0)[:, None]
+    np.testing.assert_allclose(curl[:, 0:2], 0.0, atol=1e-3 * 2.0)
+    rel = np.abs(curl[:, 2] / expect[:, 2] - 1.0)
+    assert rel.max() < 0.08, rel.max()
+
+
+def test_vorticity_confinement_points_up_the_vorticity_gradient():
+    """|w| = a x on a still lattice: eta = inv sum |w_j| grad W ~ (rho/rho0)
+    grad|w| points along +x, so v gains eps_v dt (x^ x w) = -eps_v dt a x y^
+    (CS_VorticityConfinement.glsl:133-167), up to the direction of eta (the
+    FP32 lattice coordinates are symmetric to ~1e-8)."""
+    cfg = dam_break_config(substeps=1, n_iter=1, gravity=(0.0, 0.0, 0.0), box=(4.0, 4.0, 4.0), hash_size=1 << 20)
+    a = 3.0
+    parts = _still_lattice(cfg, 16, (1.0, 1.0, 1.0), lambda x: np.zeros_like(x))
+    sim = O().OracleSim(cfg, parts, precision="f64")
+    sim.predict()
+    sim.sort()
+    sim.build_cells()
+    x = parts[:, 0:3].astype(np.float64)
+    sim.solver[:, 4:7] = 0.0
+    sim.solver[:, 6] = a * x[:, 0]
+    sim.vorticity()
+    interior = (np.abs(x - 1.4) < 0.2 + 1e-9).all(1)
+    dv = sim.vel[interior, 0:3]
+    expect_y = -cfg.f32("vorticity_epsilon") * sim.ubo.sub_dt * a * x[interior, 0]
+    np.testing.assert_allclose(dv[:, 1], expect_y, rtol=1e-6)
+    assert np.abs(dv[:, [0, 2]]).max() <= 1e-3 * np.abs(expect_y).max()
+
+
+def test_lod_closed_form():
+    """CS_ComputeLOD.glsl:66-79: round(mix(max, min, clamp(d / maxDist, 0, 1)))."""
+    cfg = dam_break_config(enable_apbf=True, min_lod=1, max_lod=7, lod_max_dist=4.0, gravity=(0.0, 0.0, 0.0))
+    cam = np.array([0.5, 0.5, 0.5])
+    d = np.array([0.0, 1.0, 2.0, 2.25, 3.99, 10.0])
+    parts = np.zeros((len(d), 12), dtype=np.float32)
+    parts[:, 0] = cam[0] + d
+    parts[:, 1:3] = cam[1:3]
+    parts[:, 4:7] = parts[:, 0:3]
+    sim = O().OracleSim(cfg, parts, precision="f64", camera=tuple(cam))
+    sim.predict()
+    sim.lod_pass()
+    t = np.clip(d / 4.0, 0.0, 1.0)
+    m = 7.0 * (1.0 - t) + 1.0 * t
+    expect = np.clip(np.floor(m + 0.5), 1, 7)  # half away from zero (m > 0)
+    np.testing.assert_array_equal(sim.lod, expect.astype(np.uint32))
+    assert list(sim.lod) == [7, 6, 4, 4, 1, 1]
+
+
+def test_interaction_force_closed_form():
+    """CS_PredictAndHash.glsl:105-112: v += normalize(p_i - x) * strength * (1 - d / r) * dt."""
+    cfg = dam_break_config(substeps=1, gravity=(0.0, -9.81, 0.0), interaction_radius=1.5, interaction_strength=25.0)
+    parts = np.zeros((3, 12), dtype=np.float32)
+    parts[:, 0:3] = [[1.0, 1.0, 1.0], [2.0, 1.0, 1.0], [3.0, 1.0, 1.0]]
+    parts[:, 4:7] = parts[:, 0:3]
+    ip = np.array([1.5, 1.3, 1.0])
+    sim = O().OracleSim(cfg, parts, precision="f64", interaction=tuple(ip))
+    sim.predict()
+    dt = sim.ubo.sub_dt
+    g = np.array(cfg.gravity, dtype=np.float32).astype(np.float64)
+    for i in range(3):
+        p = parts[i, 0:3].astype(np.float64)
+        to = ip.astype(np.float32).astype(np.float64) - p
+        dist = np.linalg.norm(to)
+        v = g * dt
+        if 1e-5 < dist < 1.5:
+            v = v + to / dist * 25.0 * (1.0 - dist / 1.5) * dt
+        np.testing.assert_allclose(sim.vel[i, 0:3], v, rtol=0, atol=1e-12)
+    assert np.linalg.norm(parts[2, 0:3] - ip) > 1.5  # the third is outside the radius: gravity only
