@@ -9,6 +9,25 @@
 
 namespace fsi {
 
+// Bucket numbers of the 27 cells around (cx, cy, cz) in the reference's visit
+// order.  cell_hash multiplies each coordinate (wrapping int32, then XOR,
+// CS_PredictAndHash.glsl:77-83); (c + d) * P = c * P + d * P mod 2^32, so
+// the three products are formed once and the offsets added per bucket.
+struct BucketWalk {
+    uint32_t hx, hy, hz, mask, base;
+    __device__ __forceinline__ BucketWalk(int32_t cx, int32_t cy, int32_t cz, const Domain& dm)
+        : hx((uint32_t)(cx - 1) * 73856093u),
+          hy((uint32_t)(cy - 1) * 19349663u),
+          hz((uint32_t)(cz - 1) * 83492791u),
+          mask(dm.hash_mask),
+          base(dm.hash_base) {}
+    // q = 9 dz + 3 dy + dx over offsets 0..2 (dx fastest)
+    __device__ __forceinline__ uint32_t bucket(int q) const {
+        const uint32_t ix = (uint32_t)(q % 3), iy = (uint32_t)((q / 3) % 3), iz = (uint32_t)(q / 9);
+        return base + (((hx + ix * 73856093u) ^ (hy + iy * 19349663u) ^ (hz + iz * 83492791u)) & mask);
+    }
+};
+
 // Visit the candidates of the 27 buckets around x in the reference order
 // (CS_ComputeLambdas.glsl:126-150: bucket offsets dz, dy, dx with dx fastest,
 // slots in sorted order, the particle's own slot skipped, an aliased bucket
@@ -22,23 +41,21 @@ __device__ __forceinline__ void scan_buckets(V3 x, float h, const Domain& dm, co
                                              const float4* __restrict__ X, uint32_t self, F&& f) {
     int32_t cx, cy, cz;
     cell_of(x, h, cx, cy, cz);
-    int2 se = cells[dm.hash_base + cell_hash(cx - 1, cy - 1, cz - 1, dm.hash_mask)];
-    auto visit = [&](int32_t k, float4 xk) {
-        if ((uint32_t)k == self) return;
+    const BucketWalk bw(cx, cy, cz, dm);
+    int2 se = cells[bw.bucket(0)];
+    auto visit = [&](uint32_t k, float4 xk) {
+        if (k == self) return;
         const V3 r = sub(x, xyz(xk));
-        f((uint32_t)k, xk, r, dot(r, r));
+        f(k, xk, r, dot(r, r));
     };
 #pragma unroll 1
     for (int q = 0; q < 27; ++q) {
         int2 nxt = make_int2(-1, -1);
-        if (q < 26) {
-            const int qn = q + 1;
-            nxt = cells[dm.hash_base + cell_hash(cx + qn % 3 - 1, cy + (qn / 3) % 3 - 1, cz + qn / 9 - 1, dm.hash_mask)];
-        }
+        if (q < 26) nxt = cells[bw.bucket(q + 1)];
         // an empty bucket is (-1, -1): no iterations
-        const int32_t last = se.y - 1;
-        for (int32_t k = se.x; k < se.y; k += 4) {
-            const int32_t k1 = min(k + 1, last), k2 = min(k + 2, last), k3 = min(k + 3, last);
+        const uint32_t end = (uint32_t)se.y, last = end - 1u;
+        for (uint32_t k = (uint32_t)se.x; k < end; k += 4) {
+            const uint32_t k1 = min(k + 1, last), k2 = min(k + 2, last), k3 = min(k + 3, last);
             const float4 a0 = X[k], a1 = X[k1], a2 = X[k2], a3 = X[k3];
             visit(k, a0);
             if (k + 1 <= last) visit(k + 1, a1);

@@ -61,36 +61,36 @@ __device__ __forceinline__ uint32_t collect_flush(V3 x, float h, const Domain& d
     const float h2 = h * h;
     int32_t cx, cy, cz;
     cell_of(x, h, cx, cy, cz);
-    int2 se = cells[dm.hash_base + cell_hash(cx - 1, cy - 1, cz - 1, dm.hash_mask)];
+    const BucketWalk bw(cx, cy, cz, dm);
+    int2 se = cells[bw.bucket(0)];
     uint32_t cnt = 0, total = 0;
-    auto visit = [&](int32_t k, float4 xk) {
-        if ((uint32_t)k == self) return;
-        if (TAPS) ++ncand;
+    // branch-free candidate test: the slot is always written to the column
+    // and the fill level advances only for a neighbour (a valid slot other
+    // than the particle's own, inside the support)
+    auto visit = [&](uint32_t k, float4 xk, bool valid) {
         const V3 r = sub(x, xyz(xk));
-        if (dot(r, r) <= h2) {
-            col[cnt * kBlock] = (uint32_t)k;
-            if (++cnt == kLdsCap) {
-                flush(total, cnt);
-                total += cnt;
-                cnt = 0;
-            }
+        const bool other = valid && k != self;
+        if (TAPS) ncand += other ? 1u : 0u;
+        col[cnt * kBlock] = k;
+        cnt += (other && dot(r, r) <= h2) ? 1u : 0u;
+        if (cnt == kLdsCap) {
+            flush(total, cnt);
+            total += cnt;
+            cnt = 0;
         }
     };
 #pragma unroll 1
     for (int q = 0; q < 27; ++q) {
         int2 nxt = make_int2(-1, -1);
-        if (q < 26) {
-            const int qn = q + 1;
-            nxt = cells[dm.hash_base + cell_hash(cx + qn % 3 - 1, cy + (qn / 3) % 3 - 1, cz + qn / 9 - 1, dm.hash_mask)];
-        }
-        const int32_t last = se.y - 1;
-        for (int32_t k = se.x; k < se.y; k += 4) {
-            const int32_t k1 = min(k + 1, last), k2 = min(k + 2, last), k3 = min(k + 3, last);
+        if (q < 26) nxt = cells[bw.bucket(q + 1)];
+        const uint32_t end = (uint32_t)se.y, last = end - 1u;
+        for (uint32_t k = (uint32_t)se.x; k < end; k += 4) {
+            const uint32_t k1 = min(k + 1, last), k2 = min(k + 2, last), k3 = min(k + 3, last);
             const float4 a0 = X[k], a1 = X[k1], a2 = X[k2], a3 = X[k3];
-            visit(k, a0);
-            if (k + 1 <= last) visit(k + 1, a1);
-            if (k + 2 <= last) visit(k + 2, a2);
-            if (k + 3 <= last) visit(k + 3, a3);
+            visit(k, a0, true);
+            visit(k + 1, a1, k + 1 <= last);
+            visit(k + 2, a2, k + 2 <= last);
+            visit(k + 3, a3, k + 3 <= last);
         }
         se = nxt;
         if (__any(cnt >= kFlushAt)) {
