@@ -32,8 +32,15 @@ backward (replay of every frame with neighbour lists + reverse passes) of the
 hand-written adjoint are timed on the same inputs; the ratio is backward /
 forward.  The backward's input is the gradient of a 32^3 density-matching loss.
 
+Fourth leg (reported under "fsi"): the FSI coupling workload, 262,144
+particles (a 64^3 lattice block) falling onto a cantilever plate of 128 x 32
+quads (4.0 x 1.0, 8,192 triangles, 4,257 vertices) whose SDF (cell 0.05) lives
+on the device; every frame: contact forces from the last frame's candidates,
+a prescribed bending of the plate, dirty SDF refit, step, contact filter.
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch-steps K2]
-                       [--no-batched] [--no-adjoint] [--no-cpu-baseline]
+                       [--fsi-steps K3] [--no-batched] [--no-adjoint] [--no-fsi]
+                       [--no-cpu-baseline]
 """
 import argparse
 import json
@@ -247,6 +254,91 @@ def adjoint_leg(fsi, reps=3):
             "replay_mismatches": st["replay_mismatches"], "max_neighbours": st["max_neighbours"]}
 
 
+def plate_mesh(nx=128, nz=32, size=(4.0, 1.0), origin=(1.2, 1.0, 2.7)):
+    """The cantilever plate of the FSI workload: an nx x nz quad grid in the
+    plane y = origin[1], two triangles per quad (normals +y)."""
+    xs = np.linspace(0.0, size[0], nx + 1)
+    zs = np.linspace(0.0, size[1], nz + 1)
+    X, Z = np.meshgrid(xs, zs, indexing="xy")
+    verts = np.stack([X.ravel() + origin[0], np.full(X.size, float(origin[1])), Z.ravel() + origin[2]], axis=1)
+    a = (np.arange(nz)[:, None] * (nx + 1) + np.arange(nx)[None, :]).ravel()
+    b, c, d = a + 1, a + nx + 1, a + nx + 2
+    faces = np.stack([np.stack([a, d, b], 1), np.stack([a, c, d], 1)], 1).reshape(-1, 3).astype(np.int32)
+    return verts, faces
+
+
+def fsi_leg(fsi, frames):
+    """The FSI coupling loop of src/main.cpp:1674-1707 on the device:
+    contact forces from the last frame's candidates, a prescribed bending of
+    the plate (the host Laplacian deformation is not part of this build:
+    the tip moves by at most 0.02 per frame, the reference's displacement
+    clamp), dirty SDF refit, the fluid step, the contact filter."""
+    verts, faces = plate_mesh()
+    cell, band, thresh, max_disp = 0.05, 0.12, 0.5, 0.02
+    dirty_band = max(0.025, 2.0 * cell) + max_disp + cell
+    cfg = fsi.FluidConfig(dt=1.0 / 120.0, gravity=(0.0, -9.81, 0.0), h=0.1, rho0=6378.0, solver_iterations=6,
+                          substep_iterations=2, eps=600.0, bounds_min=(0.0, 0.0, 0.0), bounds_max=(6.4, 6.4, 6.4),
+                          hash_size=1 << 19)
+    parts = fsi.lattice(64, 64, 64, (1.6, 2.0, 1.6), 0.05, 0.001, 1337)
+    n = len(parts)
+    x0, length = verts[:, 0].min(), np.ptp(verts[:, 0])
+    shape = ((verts[:, 0] - x0) / length) ** 2  # cantilever: the first column stays put
+
+    def bent(f):
+        return verts + np.outer(0.3 * np.sin(0.02 * f) * shape, (0.0, -1.0, 0.0))
+
+    mesh = fsi.Mesh(verts, faces, cell)
+    s = fsi.Solver(cfg)
+    s.set_particles(parts)
+    s.set_sdf_from_mesh(mesh, padding=0.025)
+    t_contact = t_refit = t_step = 0.0
+    contacts = dirty = 0
+    s.filter_contacts(band)
+    s.sync()
+    t0 = time.perf_counter()
+    for f in range(frames):
+        a = time.perf_counter()
+        _, (nc, _) = s.contact_forces(mesh, band, True, thresh)
+        b = time.perf_counter()
+        nd, _, _ = mesh.refit_dirty(bent(f + 1), dirty_band, 1e-4)
+        s.set_sdf_from_mesh(mesh, padding=0.025)
+        c = time.perf_counter()
+        s.step()
+        s.filter_contacts(band)
+        s.sync()
+        d = time.perf_counter()
+        t_contact += b - a
+        t_refit += c - b
+        t_step += d - c
+        contacts += nc
+        dirty += nd
+    wall = time.perf_counter() - t0
+    # a full refit (SDFBoundary::refitFromMesh) on the device vs the host build
+    reps = 5
+    mesh.refit(bent(frames))
+    a = time.perf_counter()
+    for _ in range(reps):
+        mesh.refit(bent(frames))
+    full_ms = (time.perf_counter() - a) / reps * 1e3
+    a = time.perf_counter()
+    fsi.sdf_build(bent(frames), faces, cell)
+    host_ms = (time.perf_counter() - a) * 1e3
+    got = s.read_particles()
+    s.close()
+    mesh.close()
+    return {"particles": n, "frames": frames, "iterations": 6, "substeps": 2,
+            "plate": {"vertices": len(verts), "triangles": len(faces), "sdf_cell": cell, "sdf_dims": list(mesh.dims)},
+            "ms_per_frame": round(wall / frames * 1e3, 3),
+            "value": n * 12 * frames / wall, "unit": "particle-iterations/s",
+            "contact_ms_per_frame": round(t_contact / frames * 1e3, 3),
+            "sdf_refit_ms_per_frame": round(t_refit / frames * 1e3, 3),
+            "step_ms_per_frame": round(t_step / frames * 1e3, 3),
+            "mean_contacts": round(contacts / frames, 1), "mean_dirty_voxels": round(dirty / frames, 1),
+            "full_refit_ms": round(full_ms, 3), "host_sdf_build_ms": round(host_ms, 3),
+            "host_sdf_build_threads": min(16, os.cpu_count() or 1),
+            "finite": bool(np.all(np.isfinite(got[:, 0:3])))}
+
+
 def cpu_baseline(fsi, cfg, parts, sdf, frames=6):
     """The CPU oracle (a C restatement of the reference shaders, OpenMP over
     particles) on the same workload, timed over `frames` frames."""
@@ -273,6 +365,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--fsi-steps", type=int, default=300)
+    ap.add_argument("--no-fsi", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain device copy of 256 MiB (3 launches), the reference point that scales "
                          "the FETCH_SIZE / WRITE_SIZE counters to bytes")
@@ -371,6 +465,8 @@ def main():
         out["batched"] = batched_leg(fsi, args.batch_steps, 1)
     if not args.no_adjoint:
         out["adjoint"] = adjoint_leg(fsi)
+    if not args.no_fsi:
+        out["fsi"] = fsi_leg(fsi, args.fsi_steps)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(fsi, cfg, parts, sdf)
     print(json.dumps(out))

@@ -366,6 +366,57 @@ int fsi_sdf_build(const double* verts, uint32_t nverts, const int32_t* faces, ui
                   double cell, double origin[3], int32_t dims[3], float* grid);
 
 /* ------------------------------------------------------------------------- */
+/* FSI coupling on the device: the obstacle mesh, its signed-distance grid    */
+/* and the contact forces.  The reference keeps these on the host            */
+/* (SDFBoundary, src/fluid/SDFBoundary.cpp; computeContactForces,             */
+/* src/main.cpp:604-710) and reads the contact candidates back every frame.   */
+/* ------------------------------------------------------------------------- */
+
+typedef struct fsi_mesh fsi_mesh; /* opaque handle */
+
+/* SDFBoundary::buildFromMesh (src/fluid/SDFBoundary.cpp:278-398) with the
+ * grid on the device: triangles, pseudo-normals and the BVH are prepared on
+ * the host (FP64), uploaded, and every voxel is computed on the device with
+ * the same arithmetic as fsi_sdf_build (which gives the identical grid). */
+int fsi_mesh_create(const double* verts, uint32_t nverts, const int32_t* faces, uint32_t nfaces,
+                    double cell, fsi_mesh** out);
+int fsi_mesh_destroy(fsi_mesh* m);
+/* Grid placement: origin = bbox - 5*cell, dims = ceil(span / cell). */
+int fsi_mesh_grid_info(fsi_mesh* m, double origin[3], double* cell, int32_t dims[3]);
+/* The grid (dims product floats, x fastest) to the host. */
+int fsi_mesh_read_sdf(fsi_mesh* m, float* grid);
+/* SDFBoundary::refitFromMesh (src/fluid/SDFBoundary.cpp:454-467): moved
+ * vertices (same topology, 3 doubles each): corners and face normals
+ * refreshed, BVH boxes refit, every voxel recomputed.  Edge and vertex
+ * pseudo-normals keep their build values, as in the reference. */
+int fsi_mesh_refit(fsi_mesh* m, const double* verts);
+/* SDFBoundary::refitDirty (src/fluid/SDFBoundary.cpp:469-548): only voxels
+ * within band_radius of a vertex that moved more than move_eps since the
+ * last refit (around its old and its new position) are recomputed.  Out:
+ * the number of recomputed voxels and their bounding box in voxel (x, y, z)
+ * coordinates (min > max when none). */
+int fsi_mesh_refit_dirty(fsi_mesh* m, const double* verts, double band_radius, double move_eps,
+                         int32_t dirty_min[3], int32_t dirty_max[3], uint32_t* dirty_voxels);
+/* The per-vertex forces of the last fsi_pbf_contact_forces (3 floats each). */
+int fsi_mesh_read_forces(fsi_mesh* m, float* forces);
+
+/* The mesh's current grid as the solver's boundary (PBFluids::setSDFBoundary
+ * + SDFBoundary::uploadToGPU, include/fluid/PBFluids.h:120): a device copy,
+ * ordered after the mesh's last refit; call again after a refit. */
+int fsi_pbf_set_sdf_from_mesh(fsi_pbf* h, fsi_mesh* m);
+
+/* computeContactForces (src/main.cpp:604-710) on the device over the
+ * candidates of the last fsi_pbf_filter_contacts (no readback): closest
+ * triangle, push direction (particle to surface, or the inward face normal
+ * within 1e-6), magnitude max(0, v . dir) or 1, barycentric split onto the
+ * three vertices.  Per-vertex sums run in particle-id order of the
+ * candidates.  forces (3 floats per vertex, may be NULL) and counts (may be
+ * NULL: [0] contacts, [1] vertices with |F| >= max(1e-5, control_threshold))
+ * are copied to the host. */
+int fsi_pbf_contact_forces(fsi_pbf* h, fsi_mesh* m, float band, int use_vel_magnitude,
+                           float control_threshold, float* forces, uint32_t counts[2]);
+
+/* ------------------------------------------------------------------------- */
 /* Differentiable frames.  No counterpart in the reference (its PBFluids has  */
 /* no gradient): the reverse-mode derivative of `frames` whole frames of      */
 /* PBFluids::step (src/fluid/PBFluids.cpp:323-517) with respect to the        */
@@ -383,8 +434,19 @@ typedef struct fsi_adjoint fsi_adjoint; /* opaque handle */
 /* cfg: everything but viscosity and eps, which each forward call sets. */
 int fsi_adj_create(const fsi_fluid_config* cfg, uint32_t n, uint32_t frames, fsi_adjoint** out);
 int fsi_adj_destroy(fsi_adjoint* a);
+/* Precision of the differentiated trajectory: 32 (default) runs the frames
+ * on the solver's own FP32 kernels, and the gradient is that of the FP32
+ * trajectory, whose rounding (about 1e-7 of a position per operation) moves
+ * a derivative of a many-frame run by up to about 1e-2 relative; 64 runs
+ * the same passes in FP64 (coefficients of PBFluids::setParams,
+ * src/fluid/PBFluids.cpp:44-101, evaluated in FP64) over explicit neighbour
+ * lists, for derivatives that central differences of an FP64 simulation
+ * confirm to 1e-3 and better.  The reverse passes compute in FP64 either
+ * way.  Drops the checkpoints of the last forward. */
+int fsi_adj_set_precision(fsi_adjoint* a, uint32_t bits);
 /* Runs the frames from (pos0, vel0), keeping a checkpoint of every frame's
- * starting state; pos_out / vel_out (may be NULL) receive the final state. */
+ * starting state; pos_out / vel_out (may be NULL) receive the final state
+ * (rounded to FP32 in the FP64 mode). */
 int fsi_adj_forward(fsi_adjoint* a, const float* pos0, const float* vel0, float viscosity, float eps, float* pos_out,
                     float* vel_out, void* stream);
 /* Given dL/d(final pos) and dL/d(final vel) (either may be NULL: zero),
@@ -396,6 +458,13 @@ int fsi_adj_forward(fsi_adjoint* a, const float* pos0, const float* vel0, float 
  * pass (aliased buckets count once per visit).  Synchronises with the device. */
 int fsi_adj_backward(fsi_adjoint* a, const float* grad_pos, const float* grad_vel, float* grad_pos0, float* grad_vel0,
                      double* grad_params, void* stream);
+/* The same two calls with FP64 states (n double4 each) and parameters, for
+ * the FP64 trajectory (FSI_ERR_STATE otherwise): a loss evaluated on FP32
+ * positions carries their rounding into its gradient. */
+int fsi_adj_forward64(fsi_adjoint* a, const double* pos0, const double* vel0, double viscosity, double eps,
+                      double* pos_out, double* vel_out, void* stream);
+int fsi_adj_backward64(fsi_adjoint* a, const double* grad_pos, const double* grad_vel, double* grad_pos0,
+                       double* grad_vel0, double* grad_params, void* stream);
 /* Of the last backward: out[0] capacity bits (1: neighbours, 2: pairs into one
  * particle), out[1] replayed records that differed from the forward, out[2]
  * the largest neighbour count and out[3] the largest in-degree of any pass. */

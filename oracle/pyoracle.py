@@ -106,8 +106,61 @@ def load() -> C.CDLL:
         fn.restype = C.c_double
         getattr(lib, f"oracle_sdf_gradient_{p}").argtypes = [K, d3, d3]
     lib.oracle_density_error_f64.argtypes = [f32p, C.c_uint32, U, d3, d3, d3]
+    i32p, u32p, u8p = C.POINTER(C.c_int32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint8)
+    lib.oracle_mesh_sdf.argtypes = [d3, d3, C.c_uint32, i32p, C.c_uint32, d3, C.c_double, i32p, f32p]
+    lib.oracle_dirty_mask.argtypes = [d3, d3, C.c_uint32, C.c_double, C.c_double, d3, C.c_double, i32p, u8p, i32p]
+    lib.oracle_dirty_mask.restype = C.c_uint32
+    lib.oracle_contact_forces.argtypes = [d3, C.c_uint32, i32p, C.c_uint32, f32p, C.c_uint32, C.c_float, C.c_int,
+                                          C.c_float, f32p, u32p]
     _lib = lib
     return lib
+
+
+def _p(a, ct):
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+def mesh_sdf(verts, faces, origin, cell, dims, verts_build=None) -> np.ndarray:
+    """Brute-force signed distance grid (dims[2], dims[1], dims[0]; x fastest
+    in memory) of a mesh; verts_build: the vertices its edge / vertex
+    pseudo-normals were built from (default: verts)."""
+    v = np.ascontiguousarray(verts, dtype=np.float64).reshape(-1, 3)
+    vb = v if verts_build is None else np.ascontiguousarray(verts_build, dtype=np.float64).reshape(-1, 3)
+    f = np.ascontiguousarray(faces, dtype=np.int32).reshape(-1, 3)
+    o = np.ascontiguousarray(origin, dtype=np.float64)
+    d = np.ascontiguousarray(dims, dtype=np.int32)
+    grid = np.zeros((int(d[2]), int(d[1]), int(d[0])), dtype=np.float32)
+    load().oracle_mesh_sdf(_p(v, C.c_double), _p(vb, C.c_double), len(v), _p(f, C.c_int32), len(f),
+                           _p(o, C.c_double), float(cell), _p(d, C.c_int32), _p(grid, C.c_float))
+    return grid
+
+
+def dirty_mask(old_verts, new_verts, band, move_eps, origin, cell, dims):
+    """(count, mask (z, y, x), box[6] = min xyz, max xyz) of a dirty refit."""
+    ov = np.ascontiguousarray(old_verts, dtype=np.float64).reshape(-1, 3)
+    nv = np.ascontiguousarray(new_verts, dtype=np.float64).reshape(-1, 3)
+    o = np.ascontiguousarray(origin, dtype=np.float64)
+    d = np.ascontiguousarray(dims, dtype=np.int32)
+    mask = np.zeros((int(d[2]), int(d[1]), int(d[0])), dtype=np.uint8)
+    box = np.zeros(6, dtype=np.int32)
+    n = load().oracle_dirty_mask(_p(ov, C.c_double), _p(nv, C.c_double), len(ov), float(band), float(move_eps),
+                                 _p(o, C.c_double), float(cell), _p(d, C.c_int32), _p(mask, C.c_uint8),
+                                 _p(box, C.c_int32))
+    return int(n), mask, box
+
+
+def contact_forces(verts, faces, cand, band, use_vel=True, threshold=0.0):
+    """Per-vertex forces (nv, 3) float32 and (contacts, control points) of the
+    candidate records cand (ncand, 8) float32 = pos xyzw, vel xyzw."""
+    v = np.ascontiguousarray(verts, dtype=np.float64).reshape(-1, 3)
+    f = np.ascontiguousarray(faces, dtype=np.int32).reshape(-1, 3)
+    c = np.ascontiguousarray(cand, dtype=np.float32).reshape(-1, 8)
+    out = np.zeros((len(v), 3), dtype=np.float32)
+    counts = np.zeros(2, dtype=np.uint32)
+    load().oracle_contact_forces(_p(v, C.c_double), len(v), _p(f, C.c_int32), len(f), _p(c, C.c_float), len(c),
+                                 float(band), 1 if use_vel else 0, float(threshold), _p(out, C.c_float),
+                                 _p(counts, C.c_uint32))
+    return out, (int(counts[0]), int(counts[1]))
 
 
 def f32(x) -> float:

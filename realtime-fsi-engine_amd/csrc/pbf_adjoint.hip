@@ -286,12 +286,13 @@ __global__ __launch_bounds__(kAB) void k_rec_vorticity(StepConst c, const float4
     vel[id] = make_float4(v4.x + dv.x, v4.y + dv.y, v4.z + dv.z, v4.w);
 }
 
-// ---- reverse passes -------------------------------------------------------
+
+// ---- FP64 arithmetic --------------------------------------------------------
 // The reverse passes compute in FP64.  A pair (i, j) contributes to the
 // position adjoint of both ends with nearly opposite terms (one computed by
 // slot i, one by slot j); in FP32 the rounding of each large term would
-// survive their cancellation.  The tape (the forward's FP32 values) is read
-// as is, so the derivative is that of the FP32 trajectory.
+// survive their cancellation.  The FP64 trajectory mode runs the forward in
+// FP64 too (kernels *64 below).
 
 struct D3 {
     double x, y, z;
@@ -300,47 +301,55 @@ __device__ __forceinline__ D3 dmk(double x, double y, double z) { return D3{x, y
 __device__ __forceinline__ D3 dz3() { return D3{0.0, 0.0, 0.0}; }
 __device__ __forceinline__ D3 d3(float4 v) { return D3{(double)v.x, (double)v.y, (double)v.z}; }
 __device__ __forceinline__ D3 d3(double4 v) { return D3{v.x, v.y, v.z}; }
-__device__ __forceinline__ double4 d4(D3 v) { return make_double4(v.x, v.y, v.z, 0.0); }
+__device__ __forceinline__ double4 d4(D3 v, double w = 0.0) { return make_double4(v.x, v.y, v.z, w); }
 __device__ __forceinline__ D3 add(D3 a, D3 b) { return D3{a.x + b.x, a.y + b.y, a.z + b.z}; }
 __device__ __forceinline__ D3 sub(D3 a, D3 b) { return D3{a.x - b.x, a.y - b.y, a.z - b.z}; }
 __device__ __forceinline__ D3 scl(D3 a, double s) { return D3{a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ D3 dvs(D3 a, double s) { return D3{a.x / s, a.y / s, a.z / s}; }
 __device__ __forceinline__ double dot(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 __device__ __forceinline__ double len(D3 a) { return sqrt(dot(a, a)); }
 __device__ __forceinline__ D3 cross(D3 a, D3 b) {
     return D3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
 }
 
-// The pair functions of the forward in FP64 (the support test on r^2 <= h^2
-// of the FP32 forward decided which pairs are in the lists).
-__device__ __forceinline__ double d_h2(const StepConst& c) { return (double)c.h * (double)c.h; }
-__device__ __forceinline__ double d_poly6(const StepConst& c, double r2) {
-    const double h2 = d_h2(c);
+// The constants of a step in FP64: the FP32 step constants widened (FP32
+// trajectory), or PBFluids::setParams (src/fluid/PBFluids.cpp:44-101,
+// Kernels.cpp:9-27) evaluated in FP64 (FP64 trajectory).
+struct C64 {
+    double g[3], sub_dt, h, rho0, eps, wq, k_corr, n_corr, viscosity, damping, cohesion, w0_self, poly6, spiky,
+        inv_rho0, vort_eps;
+    double bmin[3], bmax[3];
+    uint32_t n, hash_mask, enable_scorr, enable_visc, enable_vort;
+};
+
+// The pair functions of CS_ComputeLambdas.glsl:74-92 / CS_ComputeDeltaP.glsl:94-104
+// in FP64, with the oracle twin's operation order.
+__device__ __forceinline__ double d_poly6(const C64& k, double r2) {
+    const double h2 = k.h * k.h;
     if (r2 > h2) return 0.0;
     const double q = h2 - r2;
-    return (double)c.poly6 * q * q * q;
+    return k.poly6 * q * q * q;
 }
-__device__ __forceinline__ D3 d_poly6_grad(const StepConst& c, D3 r, double r2) {
-    const double h2 = d_h2(c);
+__device__ __forceinline__ D3 d_poly6_grad(const C64& k, D3 r, double r2) {
+    const double h2 = k.h * k.h;
     if (r2 > h2) return dz3();
     const double q = h2 - r2;
-    return scl(r, -6.0 * (double)c.poly6 * q * q);
+    return scl(r, -6.0 * k.poly6 * q * q);
 }
-__device__ __forceinline__ D3 d_spiky(const StepConst& c, D3 r, double r2) {
-    if (r2 > d_h2(c) || r2 < 1e-10) return dz3();
+__device__ __forceinline__ D3 d_spiky(const C64& k, D3 r, double r2) {
+    if (r2 > k.h * k.h || r2 < 1e-10) return dz3();
     const double d = sqrt(r2);
-    const double hd = (double)c.h - d;
-    return scl(r, -(double)c.spiky * hd * hd / d);
+    const double hd = k.h - d;
+    return scl(r, -k.spiky * hd * hd / d);
 }
-// J(r) v for the spiky gradient field g(r) = phi(|r|) r of
-// CS_ComputeLambdas.glsl:83-92, phi(d) = -K (h - d)^2 / d:
+// J(r) v for the spiky gradient field g(r) = phi(|r|) r, phi(d) = -K (h - d)^2 / d:
 // J = phi I + phi'(d) / d r r^T (symmetric, even in r), zero where g is zero.
-__device__ __forceinline__ D3 d_spiky_jvp(const StepConst& c, D3 r, double r2, D3 v) {
-    if (r2 > d_h2(c) || r2 < 1e-10) return dz3();
+__device__ __forceinline__ D3 d_spiky_jvp(const C64& k, D3 r, double r2, D3 v) {
+    if (r2 > k.h * k.h || r2 < 1e-10) return dz3();
     const double d = sqrt(r2);
-    const double hd = (double)c.h - d;
-    const double K = (double)c.spiky;
-    const double phi = -K * hd * hd / d;
-    const double dphi = K * (2.0 * hd / d + hd * hd / (d * d));
+    const double hd = k.h - d;
+    const double phi = -k.spiky * hd * hd / d;
+    const double dphi = k.spiky * (2.0 * hd / d + hd * hd / (d * d));
     return add(scl(v, phi), scl(r, dphi / d * dot(r, v)));
 }
 __device__ __forceinline__ double d_powr(double x, double n) {
@@ -348,42 +357,72 @@ __device__ __forceinline__ double d_powr(double x, double n) {
         uint32_t e = (uint32_t)n;
         double result = 1.0, base = x;
         while (e) {
-            if (e & 1u) result *= base;
+            if (e & 1u) result = result * base;
             e >>= 1u;
-            if (e) base *= base;
+            if (e) base = base * base;
         }
         return result;
     }
     return pow(x, n);
 }
-// s_corr (CS_ComputeDeltaP.glsl:94-104) and its gradient
-// -k n (W/wq)^(n-1) / wq dW/dr.
-__device__ __forceinline__ double d_scorr(const StepConst& c, double r2) {
-    if (c.enable_scorr == 0u) return 0.0;
-    const double w = d_poly6(c, r2);
-    if (c.wq <= 1e-12f || w <= 0.0) return 0.0;
-    return -(double)c.k_corr * d_powr(w / (double)c.wq, (double)c.n_corr);
+__device__ __forceinline__ double d_scorr(const C64& k, double r2) {
+    if (k.enable_scorr == 0u) return 0.0;
+    const double w = d_poly6(k, r2);
+    if (k.wq <= 1e-12 || w <= 0.0) return 0.0;
+    const double ratio = w / k.wq;
+    if (ratio <= 0.0) return 0.0;
+    return -k.k_corr * d_powr(ratio, k.n_corr);
 }
-__device__ __forceinline__ D3 d_scorr_grad(const StepConst& c, D3 r, double r2) {
-    if (c.enable_scorr == 0u) return dz3();
-    const double w = d_poly6(c, r2);
-    if (c.wq <= 1e-12f || w <= 0.0 || c.n_corr == 0.0f) return dz3();
-    const double p = d_powr(w / (double)c.wq, (double)c.n_corr - 1.0);
-    return scl(d_poly6_grad(c, r, r2), -(double)c.k_corr * (double)c.n_corr * p / (double)c.wq);
+// -k n (W/wq)^(n-1) / wq dW/dr
+__device__ __forceinline__ D3 d_scorr_grad(const C64& k, D3 r, double r2) {
+    if (k.enable_scorr == 0u) return dz3();
+    const double w = d_poly6(k, r2);
+    if (k.wq <= 1e-12 || w <= 0.0 || k.n_corr == 0.0) return dz3();
+    const double p = d_powr(w / k.wq, k.n_corr - 1.0);
+    return scl(d_poly6_grad(k, r, r2), -k.k_corr * k.n_corr * p / k.wq);
 }
 
-__global__ __launch_bounds__(kAB) void k_f2d(uint32_t n, const float4* __restrict__ in, double4* __restrict__ out) {
+// |v.xyz| in the precision of the tape, as the forward computed it.
+__device__ __forceinline__ float slen3(float4 v) { return sqrtf(v.x * v.x + v.y * v.y + v.z * v.z); }
+__device__ __forceinline__ double slen3(double4 v) { return sqrt(v.x * v.x + v.y * v.y + v.z * v.z); }
+__device__ __forceinline__ bool memcmp_eq(float4 a, float4 b) {
+    return __float_as_uint(a.x) == __float_as_uint(b.x) && __float_as_uint(a.y) == __float_as_uint(b.y) &&
+           __float_as_uint(a.z) == __float_as_uint(b.z) && __float_as_uint(a.w) == __float_as_uint(b.w);
+}
+__device__ __forceinline__ bool memcmp_eq(double4 a, double4 b) {
+    return __double_as_longlong(a.x) == __double_as_longlong(b.x) &&
+           __double_as_longlong(a.y) == __double_as_longlong(b.y) &&
+           __double_as_longlong(a.z) == __double_as_longlong(b.z) &&
+           __double_as_longlong(a.w) == __double_as_longlong(b.w);
+}
+
+template <class T4>
+struct Prec;
+template <>
+struct Prec<float4> {
+    using S = float;
+    using T2 = float2;
+};
+template <>
+struct Prec<double4> {
+    using S = double;
+    using T2 = double2;
+};
+
+__global__ __launch_bounds__(kAB) void k_f2d(uint32_t n, const float4* __restrict__ in, double4* __restrict__ out,
+                                             int keep_w) {
     const uint32_t i = agid();
     if (i >= n) return;
     const float4 v = in[i];
-    out[i] = make_double4(v.x, v.y, v.z, 0.0);
+    out[i] = make_double4(v.x, v.y, v.z, keep_w ? (double)v.w : 0.0);
 }
 
-__global__ __launch_bounds__(kAB) void k_d2f(uint32_t n, const double4* __restrict__ in, float4* __restrict__ out) {
+__global__ __launch_bounds__(kAB) void k_d2f(uint32_t n, const double4* __restrict__ in, float4* __restrict__ out,
+                                             int keep_w) {
     const uint32_t i = agid();
     if (i >= n) return;
     const double4 v = in[i];
-    out[i] = make_float4((float)v.x, (float)v.y, (float)v.z, 0.0f);
+    out[i] = make_float4((float)v.x, (float)v.y, (float)v.z, keep_w ? (float)v.w : 0.0f);
 }
 
 __global__ __launch_bounds__(kAB) void k_zero_f64(double* __restrict__ p, size_t n) {
@@ -391,22 +430,231 @@ __global__ __launch_bounds__(kAB) void k_zero_f64(double* __restrict__ p, size_t
     if (i < n) p[i] = 0.0;
 }
 
+// ---- FP64 trajectory: forward / recording kernels ----------------------------
+// The same passes as the FP32 path (predict, neighbour lists, lambda, delta-p,
+// integrate, vorticity) on FP64 state, with the FP64 constants.
+
+// CS_PredictAndHash.glsl:88-130 (no interaction force).
+__global__ __launch_bounds__(kAB) void k_predict64(C64 k, const double4* __restrict__ pos, double4* __restrict__ vel,
+                                                   double4* __restrict__ xstar, uint32_t* __restrict__ keys) {
+    const uint32_t id = agid();
+    if (id >= k.n) return;
+    const D3 p = d3(pos[id]);
+    const double4 v4 = vel[id];
+    D3 v = d3(v4);
+    v = add(v, scl(dmk(k.g[0], k.g[1], k.g[2]), k.sub_dt));
+    const D3 xp = add(p, scl(v, k.sub_dt));
+    vel[id] = d4(v, v4.w);
+    xstar[id] = d4(xp);
+    const int32_t cx = (int32_t)floor(xp.x / k.h), cy = (int32_t)floor(xp.y / k.h), cz = (int32_t)floor(xp.z / k.h);
+    keys[id] = cell_hash(cx, cy, cz, k.hash_mask);
+}
+
+__global__ __launch_bounds__(kAB) void k_reorder64(uint32_t n, const uint32_t* __restrict__ perm,
+                                                   const double4* __restrict__ xstar, const double4* __restrict__ vel,
+                                                   double4* __restrict__ P, double4* __restrict__ vs) {
+    const uint32_t s = agid();
+    if (s >= n) return;
+    const uint32_t id = perm[s];
+    P[s] = xstar[id];
+    vs[s] = vel[id];
+}
+
+// Neighbour list over FP64 positions, in the reference's visit order.
+__global__ __launch_bounds__(kAB) void k_edges64(C64 k, const double4* __restrict__ X, const int2* __restrict__ cells,
+                                                 uint32_t* __restrict__ nbr, uint32_t* __restrict__ cnt,
+                                                 uint32_t* __restrict__ overflow) {
+    const uint32_t i = agid();
+    if (i >= k.n) return;
+    const D3 xi = d3(X[i]);
+    const double h2 = k.h * k.h;
+    const int32_t cx = (int32_t)floor(xi.x / k.h), cy = (int32_t)floor(xi.y / k.h), cz = (int32_t)floor(xi.z / k.h);
+    uint32_t m = 0;
+    for (int q = 0; q < 27; ++q) {
+        const int2 se = cells[cell_hash(cx + q % 3 - 1, cy + (q / 3) % 3 - 1, cz + q / 9 - 1, k.hash_mask)];
+        for (int32_t s = se.x; s < se.y; ++s) {
+            if ((uint32_t)s == i) continue;
+            const D3 r = sub(xi, d3(X[s]));
+            if (dot(r, r) <= h2) {
+                if (m < kEdgeCap) nbr[(size_t)m * k.n + i] = (uint32_t)s;
+                ++m;
+            }
+        }
+    }
+    atomicMax(overflow + 2, m);
+    if (m > kEdgeCap) {
+        atomicOr(overflow, 1u);
+        m = kEdgeCap;
+    }
+    cnt[i] = m;
+}
+
+__global__ __launch_bounds__(kAB) void k_rec_lambda64(C64 k, const double4* __restrict__ P,
+                                                      const uint32_t* __restrict__ nbr,
+                                                      const uint32_t* __restrict__ cnt, double4* __restrict__ PL,
+                                                      double2* __restrict__ lr, double4* __restrict__ aux) {
+    const uint32_t i = agid();
+    if (i >= k.n) return;
+    const D3 xi = d3(P[i]);
+    const double inv = k.inv_rho0;
+    double rho = k.w0_self;
+    double sum_g2 = 0.0;
+    D3 gi = dz3();
+    const uint32_t m = cnt[i];
+    for (uint32_t r = 0; r < m; ++r) {
+        const D3 d = sub(xi, d3(P[nbr[(size_t)r * k.n + i]]));
+        const double r2 = dot(d, d);
+        rho = rho + d_poly6(k, r2);
+        const D3 gw = d_spiky(k, d, r2);
+        const double gj2 = dot(gw, gw);
+        sum_g2 = sum_g2 + gj2 * (inv * inv);
+        gi = add(gi, gw);
+    }
+    gi = scl(gi, inv);
+    double C = rho / k.rho0 - 1.0;
+    const double floorc = -k.cohesion;
+    C = C > floorc ? C : floorc;
+    sum_g2 = sum_g2 + dot(gi, gi);
+    const double D = sum_g2 + k.eps;
+    const double lambda = -C / D;
+    PL[i] = d4(xi, lambda);
+    lr[i] = make_double2(lambda, rho);
+    aux[i] = d4(gi, D);
+}
+
+__global__ __launch_bounds__(kAB) void k_rec_deltap64(C64 k, const double4* __restrict__ PL,
+                                                      const uint32_t* __restrict__ nbr,
+                                                      const uint32_t* __restrict__ cnt, double4* __restrict__ P,
+                                                      double4* __restrict__ Ptape, double4* __restrict__ dptape) {
+    const uint32_t i = agid();
+    if (i >= k.n) return;
+    const double4 xi4 = PL[i];
+    const D3 xi = d3(xi4);
+    const double li = xi4.w;
+    D3 dp = dz3();
+    const uint32_t m = cnt[i];
+    for (uint32_t r = 0; r < m; ++r) {
+        const double4 a = PL[nbr[(size_t)r * k.n + i]];
+        const D3 d = sub(xi, d3(a));
+        const double r2 = dot(d, d);
+        dp = add(dp, scl(d_spiky(k, d, r2), li + a.w + d_scorr(k, r2)));
+    }
+    dp = scl(dp, k.inv_rho0);
+    dptape[i] = d4(dp);
+    const double maxdp = 0.5 * k.h;
+    const double l2 = dot(dp, dp);
+    if (l2 > maxdp * maxdp) dp = scl(dp, maxdp / sqrt(l2));
+    const D3 p = add(xi, dp);
+    P[i] = d4(p);
+    Ptape[i] = d4(p);
+}
+
+__global__ __launch_bounds__(kAB) void k_rec_integrate64(C64 k, const double4* __restrict__ P,
+                                                         const double4* __restrict__ vs,
+                                                         const uint32_t* __restrict__ perm,
+                                                         double4* __restrict__ pos, double4* __restrict__ vel,
+                                                         const uint32_t* __restrict__ nbr,
+                                                         const uint32_t* __restrict__ cnt,
+                                                         double4* __restrict__ curl_out, double4* __restrict__ posS,
+                                                         double4* __restrict__ visc_tape) {
+    const uint32_t s = agid();
+    if (s >= k.n) return;
+    const uint32_t id = perm[s];
+    const double4 op4 = pos[id];
+    const D3 old_pos = d3(op4);
+    const double4 ov4 = vs[s];
+    const D3 old_vel = d3(ov4);
+    D3 xp = d3(P[s]);
+    const double inv = k.inv_rho0;
+    const bool do_visc = k.enable_visc == 1u;
+    const bool do_vort = k.enable_vort == 1u;
+    D3 visc = dz3(), visc_delta = dz3();
+    if (do_visc || do_vort) {
+        D3 curl = dz3();
+        const uint32_t m = cnt[s];
+        for (uint32_t r = 0; r < m; ++r) {
+            const uint32_t j = nbr[(size_t)r * k.n + s];
+            const D3 d = sub(xp, d3(P[j]));
+            const double r2 = dot(d, d);
+            const D3 vij = sub(d3(vs[j]), old_vel);
+            if (do_visc) visc = add(visc, scl(scl(vij, d_poly6(k, r2)), inv));
+            if (do_vort) curl = add(curl, cross(vij, d_spiky(k, d, r2)));
+        }
+        if (do_visc) visc_delta = scl(visc, k.viscosity);
+        if (do_vort) curl_out[s] = d4(scl(curl, inv));
+    }
+    visc_tape[s] = d4(visc);
+    bool cxh = false, cyh = false, czh = false;
+    if (xp.x <= k.bmin[0]) { xp.x = k.bmin[0]; cxh = true; }
+    if (xp.x >= k.bmax[0]) { xp.x = k.bmax[0]; cxh = true; }
+    if (xp.y <= k.bmin[1]) { xp.y = k.bmin[1]; cyh = true; }
+    if (xp.y >= k.bmax[1]) { xp.y = k.bmax[1]; cyh = true; }
+    if (xp.z <= k.bmin[2]) { xp.z = k.bmin[2]; czh = true; }
+    if (xp.z >= k.bmax[2]) { xp.z = k.bmax[2]; czh = true; }
+    const double inv_dt = (k.sub_dt > 0.0) ? (1.0 / k.sub_dt) : 0.0;
+    D3 v = scl(sub(xp, old_pos), inv_dt);
+    v = add(v, visc_delta);
+    const double sp2 = dot(v, v);
+    if (sp2 > 2500.0) v = scl(v, 50.0 / sqrt(sp2));
+    if (cxh) v.x = v.x * -k.damping;
+    if (cyh) v.y = v.y * -k.damping;
+    if (czh) v.z = v.z * -k.damping;
+    pos[id] = d4(xp, op4.w);
+    vel[id] = d4(v, ov4.w);
+    posS[s] = d4(xp);
+}
+
+__global__ __launch_bounds__(kAB) void k_rec_vorticity64(C64 k, const double4* __restrict__ posS,
+                                                         const double4* __restrict__ curl,
+                                                         const uint32_t* __restrict__ perm,
+                                                         double4* __restrict__ vel, const uint32_t* __restrict__ nbr,
+                                                         const uint32_t* __restrict__ cnt,
+                                                         double4* __restrict__ eta_tape) {
+    const uint32_t s = agid();
+    if (s >= k.n) return;
+    const D3 pi = d3(posS[s]);
+    D3 eta = dz3();
+    const uint32_t m = cnt[s];
+    for (uint32_t r = 0; r < m; ++r) {
+        const uint32_t j = nbr[(size_t)r * k.n + s];
+        const D3 d = sub(pi, d3(posS[j]));
+        eta = add(eta, scl(d_spiky(k, d, dot(d, d)), len(d3(curl[j]))));
+    }
+    eta = scl(eta, k.inv_rho0);
+    eta_tape[s] = d4(eta);
+    const double el = len(eta);
+    if (el < 1e-6) return;
+    const D3 nn = dvs(eta, el);
+    const D3 dv = scl(scl(cross(nn, d3(curl[s])), k.vort_eps), k.sub_dt);
+    const uint32_t id = perm[s];
+    const double4 v4 = vel[id];
+    vel[id] = make_double4(v4.x + dv.x, v4.y + dv.y, v4.z + dv.z, v4.w);
+}
+
+// ---- reverse passes -----------------------------------------------------------
+// Templates over the tape type: float4 (FP32 trajectory) or double4 (FP64).
+// Branch decisions (wall flags, speed clamp, delta-p clamp, cohesion floor,
+// vorticity threshold) are re-evaluated in the tape's precision exactly as
+// the forward took them; the derivative is that of the taken branch.
+
 // Vorticity confinement, slot i's own terms: v_i += a (n x w_i), a = eps_v dt,
 // n = eta / |eta|, eta = inv sum_j |w_j| g(p_i - p_j).
-__global__ __launch_bounds__(kAB) void k_rev_vort(StepConst c, const float4* __restrict__ posS,
-                                                  const float4* __restrict__ curl, const float4* __restrict__ eta_tape,
-                                                  const uint32_t* __restrict__ perm, const double4* __restrict__ gvel,
-                                                  const uint32_t* __restrict__ nbr, const uint32_t* __restrict__ cnt,
-                                                  double4* __restrict__ gS_own, double4* __restrict__ gw_own,
-                                                  double4* __restrict__ ep, double* __restrict__ es) {
+template <class T4>
+__global__ __launch_bounds__(kAB) void k_rev_vort(C64 k, const T4* __restrict__ posS, const T4* __restrict__ curl,
+                                                  const T4* __restrict__ eta_tape, const uint32_t* __restrict__ perm,
+                                                  const double4* __restrict__ gvel, const uint32_t* __restrict__ nbr,
+                                                  const uint32_t* __restrict__ cnt, double4* __restrict__ gS_own,
+                                                  double4* __restrict__ gw_own, double4* __restrict__ ep,
+                                                  double* __restrict__ es) {
+    using S = typename Prec<T4>::S;
     const uint32_t i = agid();
-    if (i >= c.n) return;
+    if (i >= k.n) return;
     const uint32_t m = cnt[i];
-    const float4 e4 = eta_tape[i];
-    if (len(xyz(e4)) < 1e-6f) {  // the forward's test, in FP32
+    const T4 e4 = eta_tape[i];
+    if (slen3(e4) < S(1e-6)) {  // the forward's test
         for (uint32_t r = 0; r < m; ++r) {
-            ep[(size_t)r * c.n + i] = make_double4(0.0, 0.0, 0.0, 0.0);
-            es[(size_t)r * c.n + i] = 0.0;
+            ep[(size_t)r * k.n + i] = make_double4(0.0, 0.0, 0.0, 0.0);
+            es[(size_t)r * k.n + i] = 0.0;
         }
         gS_own[i] = make_double4(0.0, 0.0, 0.0, 0.0);
         gw_own[i] = make_double4(0.0, 0.0, 0.0, 0.0);
@@ -414,8 +662,8 @@ __global__ __launch_bounds__(kAB) void k_rev_vort(StepConst c, const float4* __r
     }
     const D3 eta = d3(e4);
     const double el = len(eta);
-    const double inv = (double)c.inv_rho0;
-    const double a = (double)c.vort_eps * (double)c.sub_dt;
+    const double inv = k.inv_rho0;
+    const double a = k.vort_eps * k.sub_dt;
     const D3 nn = scl(eta, 1.0 / el);
     const D3 om = d3(curl[i]);
     const D3 vb = d3(gvel[perm[i]]);
@@ -425,22 +673,23 @@ __global__ __launch_bounds__(kAB) void k_rev_vort(StepConst c, const float4* __r
     const D3 pi = d3(posS[i]);
     D3 gs = dz3();
     for (uint32_t r = 0; r < m; ++r) {
-        const size_t e = (size_t)r * c.n + i;
+        const size_t e = (size_t)r * k.n + i;
         const uint32_t j = nbr[e];
         const D3 d = sub(pi, d3(posS[j]));
         const double r2 = dot(d, d);
         const double mj = len(d3(curl[j]));
-        const D3 rb = scl(d_spiky_jvp(c, d, r2, etab), inv * mj);
+        const D3 rb = scl(d_spiky_jvp(k, d, r2, etab), inv * mj);
         gs = add(gs, rb);
         ep[e] = d4(rb);
-        es[e] = inv * dot(d_spiky(c, d, r2), etab);
+        es[e] = inv * dot(d_spiky(k, d, r2), etab);
     }
     gS_own[i] = d4(gs);
 }
 
 // Vorticity, the terms of the pairs reaching j: position adjoint into the
 // id-order state adjoint, |w_j| adjoint into w_j.
-__global__ __launch_bounds__(kAB) void k_rev_vort_t(uint32_t n, const float4* __restrict__ curl,
+template <class T4>
+__global__ __launch_bounds__(kAB) void k_rev_vort_t(uint32_t n, const T4* __restrict__ curl,
                                                     const uint32_t* __restrict__ perm, const uint32_t* __restrict__ tin,
                                                     const uint32_t* __restrict__ tcnt, const double4* __restrict__ ep,
                                                     const double* __restrict__ es, const double4* __restrict__ gS_own,
@@ -466,11 +715,11 @@ __global__ __launch_bounds__(kAB) void k_rev_vort_t(uint32_t n, const float4* __
     gw[j] = d4(w);
 }
 
-// Integrate, slot k's own terms.  gpos / gvel: adjoint of the substep's output
+// Integrate, slot s's own terms.  gpos / gvel: adjoint of the substep's output
 // state (id order); gw: adjoint of the curl (null when no later pass reads it).
-__global__ __launch_bounds__(kAB) void k_rev_integrate(StepConst c, const float4* __restrict__ xold,
-                                                       const float4* __restrict__ vs, const float4* __restrict__ P,
-                                                       const float4* __restrict__ visc_tape,
+template <class T4>
+__global__ __launch_bounds__(kAB) void k_rev_integrate(C64 k, const T4* __restrict__ xold, const T4* __restrict__ vs,
+                                                       const T4* __restrict__ P, const T4* __restrict__ visc_tape,
                                                        const uint32_t* __restrict__ perm,
                                                        const uint32_t* __restrict__ nbr,
                                                        const uint32_t* __restrict__ cnt,
@@ -480,56 +729,60 @@ __global__ __launch_bounds__(kAB) void k_rev_integrate(StepConst c, const float4
                                                        double4* __restrict__ gvs_own, double4* __restrict__ gxold,
                                                        double* __restrict__ acc_visc, double4* __restrict__ ep,
                                                        double4* __restrict__ ev) {
-    const uint32_t k = agid();
-    if (k >= c.n) return;
-    const uint32_t id = perm[k];
-    // the forward's FP32 decisions: wall flags and the speed clamp
-    V3 xp = xyz(P[k]);
-    const Domain& dom = c.dom;
+    using S = typename Prec<T4>::S;
+    const uint32_t s = agid();
+    if (s >= k.n) return;
+    const uint32_t id = perm[s];
+    // the forward's decisions, in its precision: wall flags and the speed clamp
+    const T4 p4 = P[s];
+    S xpx = p4.x, xpy = p4.y, xpz = p4.z;
     bool cxh = false, cyh = false, czh = false;
-    if (xp.x <= dom.bmin[0]) { xp.x = dom.bmin[0]; cxh = true; }
-    if (xp.x >= dom.bmax[0]) { xp.x = dom.bmax[0]; cxh = true; }
-    if (xp.y <= dom.bmin[1]) { xp.y = dom.bmin[1]; cyh = true; }
-    if (xp.y >= dom.bmax[1]) { xp.y = dom.bmax[1]; cyh = true; }
-    if (xp.z <= dom.bmin[2]) { xp.z = dom.bmin[2]; czh = true; }
-    if (xp.z >= dom.bmax[2]) { xp.z = dom.bmax[2]; czh = true; }
-    const bool do_visc = c.enable_visc == 1u;
-    const bool do_vort = c.enable_vort == 1u;
-    const float inv_dt_f = (c.sub_dt > 0.0f) ? (1.0f / c.sub_dt) : 0.0f;
-    const V3 visc_f = xyz(visc_tape[k]);
-    V3 vf = scl(sub(xp, xyz(xold[id])), inv_dt_f);
-    if (do_visc) vf = add(vf, scl(visc_f, c.viscosity));
-    const float sp2 = dot(vf, vf);
-    const double inv_dt = (double)inv_dt_f;
+    if (xpx <= S(k.bmin[0])) { xpx = S(k.bmin[0]); cxh = true; }
+    if (xpx >= S(k.bmax[0])) { xpx = S(k.bmax[0]); cxh = true; }
+    if (xpy <= S(k.bmin[1])) { xpy = S(k.bmin[1]); cyh = true; }
+    if (xpy >= S(k.bmax[1])) { xpy = S(k.bmax[1]); cyh = true; }
+    if (xpz <= S(k.bmin[2])) { xpz = S(k.bmin[2]); czh = true; }
+    if (xpz >= S(k.bmax[2])) { xpz = S(k.bmax[2]); czh = true; }
+    const bool do_visc = k.enable_visc == 1u;
+    const bool do_vort = k.enable_vort == 1u;
+    const S inv_dt_s = (S(k.sub_dt) > S(0)) ? (S(1) / S(k.sub_dt)) : S(0);
+    const T4 xo = xold[id], vt = visc_tape[s];
+    S vx = (xpx - xo.x) * inv_dt_s, vy = (xpy - xo.y) * inv_dt_s, vz = (xpz - xo.z) * inv_dt_s;
+    if (do_visc) {
+        vx = vx + vt.x * S(k.viscosity);
+        vy = vy + vt.y * S(k.viscosity);
+        vz = vz + vt.z * S(k.viscosity);
+    }
+    const S sp2 = vx * vx + vy * vy + vz * vz;
+    const double inv_dt = (double)inv_dt_s;
     D3 gv = d3(gvel[id]);
-    const double damp = (double)c.damping;
-    if (cxh) gv.x = gv.x * -damp;
-    if (cyh) gv.y = gv.y * -damp;
-    if (czh) gv.z = gv.z * -damp;
-    if (sp2 > 2500.0f) {
-        const D3 v = dmk(vf.x, vf.y, vf.z);
+    if (cxh) gv.x = gv.x * -k.damping;
+    if (cyh) gv.y = gv.y * -k.damping;
+    if (czh) gv.z = gv.z * -k.damping;
+    if (sp2 > S(2500)) {
+        const D3 v = dmk(vx, vy, vz);
         const double l = len(v);
         const D3 vh = scl(v, 1.0 / l);
         gv = scl(sub(gv, scl(vh, dot(vh, gv))), 50.0 / l);
     }
-    const D3 visc = dmk(visc_f.x, visc_f.y, visc_f.z);
+    const D3 visc = d3(vt);
     D3 vb = dz3();
     if (do_visc) {
         acc_visc[id] += dot(gv, visc);
-        vb = scl(gv, (double)c.viscosity);
+        vb = scl(gv, k.viscosity);
     }
     const D3 gxp = add(d3(gpos[id]), scl(gv, inv_dt));
     gxold[id] = d4(scl(gv, -inv_dt));
     D3 gp = dmk(cxh ? 0.0 : gxp.x, cyh ? 0.0 : gxp.y, czh ? 0.0 : gxp.z);
-    const D3 wb = (do_vort && gw) ? d3(gw[k]) : dz3();
+    const D3 wb = (do_vort && gw) ? d3(gw[s]) : dz3();
     D3 gvi = dz3();
-    const double inv = (double)c.inv_rho0;
-    const uint32_t m = cnt[k];
+    const double inv = k.inv_rho0;
+    const uint32_t m = cnt[s];
     if (do_visc || do_vort) {
-        const D3 p = d3(P[k]);
-        const D3 vi = d3(vs[k]);
+        const D3 p = d3(p4);
+        const D3 vi = d3(vs[s]);
         for (uint32_t r = 0; r < m; ++r) {
-            const size_t e = (size_t)r * c.n + k;
+            const size_t e = (size_t)r * k.n + s;
             const uint32_t j = nbr[e];
             const D3 d = sub(p, d3(P[j]));
             const double r2 = dot(d, d);
@@ -537,12 +790,12 @@ __global__ __launch_bounds__(kAB) void k_rev_integrate(StepConst c, const float4
             D3 dvb = dz3();
             D3 rb = dz3();
             if (do_visc) {
-                dvb = scl(vb, inv * d_poly6(c, r2));
-                rb = scl(d_poly6_grad(c, d, r2), inv * dot(dv, vb));
+                dvb = scl(vb, inv * d_poly6(k, r2));
+                rb = scl(d_poly6_grad(k, d, r2), inv * dot(dv, vb));
             }
             if (do_vort) {
-                dvb = add(dvb, scl(cross(d_spiky(c, d, r2), wb), inv));
-                rb = add(rb, d_spiky_jvp(c, d, r2, scl(cross(wb, dv), inv)));
+                dvb = add(dvb, scl(cross(d_spiky(k, d, r2), wb), inv));
+                rb = add(rb, d_spiky_jvp(k, d, r2, scl(cross(wb, dv), inv)));
             }
             gvi = sub(gvi, dvb);
             gp = add(gp, rb);
@@ -551,12 +804,12 @@ __global__ __launch_bounds__(kAB) void k_rev_integrate(StepConst c, const float4
         }
     } else {
         for (uint32_t r = 0; r < m; ++r) {
-            ep[(size_t)r * c.n + k] = make_double4(0.0, 0.0, 0.0, 0.0);
-            ev[(size_t)r * c.n + k] = make_double4(0.0, 0.0, 0.0, 0.0);
+            ep[(size_t)r * k.n + s] = make_double4(0.0, 0.0, 0.0, 0.0);
+            ev[(size_t)r * k.n + s] = make_double4(0.0, 0.0, 0.0, 0.0);
         }
     }
-    gP_own[k] = d4(gp);
-    gvs_own[k] = d4(gvi);
+    gP_own[s] = d4(gp);
+    gvs_own[s] = d4(gvi);
 }
 
 __global__ __launch_bounds__(kAB) void k_rev_pairs_vec2(uint32_t n, const uint32_t* __restrict__ tin,
@@ -582,39 +835,41 @@ __global__ __launch_bounds__(kAB) void k_rev_pairs_vec2(uint32_t n, const uint32
 // (CS_ComputeDeltaP.glsl:120-166) of one iteration, slot i's own terms.
 // gP: adjoint of the corrected positions; out: adjoint of the positions the
 // iteration started from (own terms) and of lambda.
-__global__ __launch_bounds__(kAB) void k_rev_deltap(StepConst c, const float4* __restrict__ Pprev,
-                                                    const float2* __restrict__ lr, const float4* __restrict__ dptape,
-                                                    const uint32_t* __restrict__ nbr, const uint32_t* __restrict__ cnt,
-                                                    const double4* __restrict__ gP, double4* __restrict__ gQ_own,
-                                                    double* __restrict__ gl_own, double4* __restrict__ ep,
-                                                    double* __restrict__ es) {
+template <class T4>
+__global__ __launch_bounds__(kAB) void k_rev_deltap(C64 k, const T4* __restrict__ Pprev,
+                                                    const typename Prec<T4>::T2* __restrict__ lr,
+                                                    const T4* __restrict__ dptape, const uint32_t* __restrict__ nbr,
+                                                    const uint32_t* __restrict__ cnt, const double4* __restrict__ gP,
+                                                    double4* __restrict__ gQ_own, double* __restrict__ gl_own,
+                                                    double4* __restrict__ ep, double* __restrict__ es) {
+    using S = typename Prec<T4>::S;
     const uint32_t i = agid();
-    if (i >= c.n) return;
+    if (i >= k.n) return;
     const D3 g = d3(gP[i]);
-    const V3 dpf = xyz(dptape[i]);
-    const float maxdp = 0.5f * c.h;
+    const T4 dp4 = dptape[i];
+    const S maxdp = S(0.5) * S(k.h);
     D3 gdp = g;
-    if (dot(dpf, dpf) > maxdp * maxdp) {  // the forward's clamp decision
-        const D3 dp = d3(dptape[i]);
+    if (dp4.x * dp4.x + dp4.y * dp4.y + dp4.z * dp4.z > maxdp * maxdp) {  // the forward's clamp decision
+        const D3 dp = d3(dp4);
         const double l = len(dp);
         const D3 dh = scl(dp, 1.0 / l);
         gdp = scl(sub(g, scl(dh, dot(dh, g))), (double)maxdp / l);
     }
-    const double inv = (double)c.inv_rho0;
+    const double inv = k.inv_rho0;
     const D3 xi = d3(Pprev[i]);
     const double li = (double)lr[i].x;
     D3 gq = g;
     double gl = 0.0;
     const uint32_t m = cnt[i];
     for (uint32_t r = 0; r < m; ++r) {
-        const size_t e = (size_t)r * c.n + i;
+        const size_t e = (size_t)r * k.n + i;
         const uint32_t j = nbr[e];
         const D3 d = sub(xi, d3(Pprev[j]));
         const double r2 = dot(d, d);
-        const double a = li + (double)lr[j].x + d_scorr(c, r2);
-        const double t = inv * dot(d_spiky(c, d, r2), gdp);
+        const double a = li + (double)lr[j].x + d_scorr(k, r2);
+        const double t = inv * dot(d_spiky(k, d, r2), gdp);
         gl += t;
-        const D3 rb = add(scl(d_spiky_jvp(c, d, r2, gdp), inv * a), scl(d_scorr_grad(c, d, r2), t));
+        const D3 rb = add(scl(d_spiky_jvp(k, d, r2, gdp), inv * a), scl(d_scorr_grad(k, d, r2), t));
         gq = add(gq, rb);
         ep[e] = d4(rb);
         es[e] = t;
@@ -645,40 +900,42 @@ __global__ __launch_bounds__(kAB) void k_rev_pairs_vec_scalar(uint32_t n, const 
 }
 
 // Lambda (CS_ComputeLambdas.glsl:108-162), slot i's own terms, added to gQ.
-__global__ __launch_bounds__(kAB) void k_rev_lambda(StepConst c, const float4* __restrict__ Pprev,
-                                                    const float2* __restrict__ lr, const float4* __restrict__ aux,
-                                                    const uint32_t* __restrict__ perm,
+template <class T4>
+__global__ __launch_bounds__(kAB) void k_rev_lambda(C64 k, const T4* __restrict__ Pprev,
+                                                    const typename Prec<T4>::T2* __restrict__ lr,
+                                                    const T4* __restrict__ aux, const uint32_t* __restrict__ perm,
                                                     const uint32_t* __restrict__ nbr, const uint32_t* __restrict__ cnt,
                                                     const double* __restrict__ gl, double4* __restrict__ gQ,
                                                     double* __restrict__ acc_eps, double4* __restrict__ ep) {
+    using S = typename Prec<T4>::S;
     const uint32_t i = agid();
-    if (i >= c.n) return;
+    if (i >= k.n) return;
     const double lb = gl[i];
-    const float rho_f = lr[i].y;
-    const float4 a4 = aux[i];
+    const S rho_s = lr[i].y;
+    const T4 a4 = aux[i];
     const D3 gi = d3(a4);
     const double D = (double)a4.w;
-    // the forward's FP32 cohesion-floor decision, its clamped value
-    const float craw_f = rho_f / c.rho0 - 1.0f;
-    const bool active = craw_f > -c.cohesion;
-    const double C = active ? (double)craw_f : -(double)c.cohesion;
+    // the forward's cohesion-floor decision and clamped value, in its precision
+    const S craw = rho_s / S(k.rho0) - S(1);
+    const bool active = craw > -S(k.cohesion);
+    const double C = active ? (double)craw : -(double)S(k.cohesion);
     const double Cb = -lb / D;
     const double Db = lb * C / (D * D);
     acc_eps[perm[i]] += Db;
-    const double inv = (double)c.inv_rho0;
-    const double rhob = active ? Cb / (double)c.rho0 : 0.0;
+    const double inv = k.inv_rho0;
+    const double rhob = active ? Cb / k.rho0 : 0.0;
     const double sb = 2.0 * inv * inv * Db;
     const D3 Gb = scl(gi, 2.0 * Db * inv);
     const D3 xi = d3(Pprev[i]);
     D3 acc = dz3();
     const uint32_t m = cnt[i];
     for (uint32_t r = 0; r < m; ++r) {
-        const size_t e = (size_t)r * c.n + i;
+        const size_t e = (size_t)r * k.n + i;
         const uint32_t j = nbr[e];
         const D3 d = sub(xi, d3(Pprev[j]));
         const double r2 = dot(d, d);
-        const D3 gw = d_spiky(c, d, r2);
-        const D3 rb = add(scl(d_poly6_grad(c, d, r2), rhob), d_spiky_jvp(c, d, r2, add(scl(gw, sb), Gb)));
+        const D3 gw = d_spiky(k, d, r2);
+        const D3 rb = add(scl(d_poly6_grad(k, d, r2), rhob), d_spiky_jvp(k, d, r2, add(scl(gw, sb), Gb)));
         acc = add(acc, rb);
         ep[e] = d4(rb);
     }
@@ -698,16 +955,16 @@ __global__ __launch_bounds__(kAB) void k_rev_pairs_sub(uint32_t n, const uint32_
 
 // Predict (CS_PredictAndHash.glsl:88-130 without the interaction force):
 // x* = x + v* dt, v* = v + g dt.
-__global__ __launch_bounds__(kAB) void k_rev_predict(StepConst c, const uint32_t* __restrict__ perm,
+__global__ __launch_bounds__(kAB) void k_rev_predict(C64 k, const uint32_t* __restrict__ perm,
                                                      const double4* __restrict__ gP0, const double4* __restrict__ gvs,
                                                      const double4* __restrict__ gxold, double4* __restrict__ gpos,
                                                      double4* __restrict__ gvel) {
-    const uint32_t k = agid();
-    if (k >= c.n) return;
-    const uint32_t id = perm[k];
-    const D3 gx = d3(gP0[k]);
+    const uint32_t s = agid();
+    if (s >= k.n) return;
+    const uint32_t id = perm[s];
+    const D3 gx = d3(gP0[s]);
     gpos[id] = d4(add(gx, d3(gxold[id])));
-    gvel[id] = d4(add(d3(gvs[k]), scl(gx, (double)c.sub_dt)));
+    gvel[id] = d4(add(d3(gvs[s]), scl(gx, k.sub_dt)));
 }
 
 __global__ __launch_bounds__(kAB) void k_reduce2(uint32_t n, const double* __restrict__ a,
@@ -751,13 +1008,13 @@ __global__ __launch_bounds__(kAB) void k_absmax(uint32_t n, const double4* __res
 }
 
 // Bitwise comparison of the replayed state with the forward's checkpoint.
-__global__ __launch_bounds__(kAB) void k_compare(uint32_t n, const float4* __restrict__ a,
-                                                 const float4* __restrict__ b, uint32_t* __restrict__ count) {
+template <class T4>
+__global__ __launch_bounds__(kAB) void k_compare(uint32_t n, const T4* __restrict__ a, const T4* __restrict__ b,
+                                                 uint32_t* __restrict__ count) {
     const uint32_t i = agid();
     if (i >= n) return;
-    const float4 x = a[i], y = b[i];
-    const bool same = __float_as_uint(x.x) == __float_as_uint(y.x) && __float_as_uint(x.y) == __float_as_uint(y.y) &&
-                      __float_as_uint(x.z) == __float_as_uint(y.z) && __float_as_uint(x.w) == __float_as_uint(y.w);
+    const T4 x = a[i], y = b[i];
+    const bool same = memcmp_eq(x, y);
     if (!same) atomicAdd(count, 1u);
 }
 
@@ -768,6 +1025,55 @@ struct EdgeList {
     uint32_t* tcnt = nullptr;  // n
 };
 
+// The recorded trajectory in one precision: checkpoints of every frame start
+// and the tape of the frame being reversed.  The FP64 trajectory also keeps
+// its working state here (the FP32 one runs in the core's buffers).
+template <class T4>
+struct Tape {
+    using T2 = typename Prec<T4>::T2;
+    T4* ck_pos = nullptr;  // (frames + 1) x n; the last is the forward's result
+    T4* ck_vel = nullptr;
+    T4* xold = nullptr;    // S x n, id order
+    T4* vs = nullptr;      // S x n
+    T4* P = nullptr;       // S x (I + 1) x n
+    T2* lr = nullptr;      // S x I x n
+    T4* aux = nullptr;     // S x I x n
+    T4* dp = nullptr;      // S x I x n
+    T4* visc = nullptr;    // S x n
+    T4* eta = nullptr;     // n
+    T4 *pos = nullptr, *vel = nullptr, *xstar = nullptr, *Pw = nullptr, *PL = nullptr, *vsw = nullptr,
+       *curl = nullptr, *posS = nullptr;
+    bool ready = false;
+
+    hipError_t alloc(size_t n, size_t frames, size_t S, size_t I, bool work) {
+        hipError_t e = hipSuccess;
+#define ALLOC(p, cnt) \
+    if ((e = dev_alloc(&(p), (cnt))) != hipSuccess) return e;
+        ALLOC(ck_pos, (frames + 1) * n);
+        ALLOC(ck_vel, (frames + 1) * n);
+        ALLOC(xold, S * n);
+        ALLOC(vs, S * n);
+        ALLOC(P, S * (I + 1) * n);
+        ALLOC(lr, S * I * n);
+        ALLOC(aux, S * I * n);
+        ALLOC(dp, S * I * n);
+        ALLOC(visc, S * n);
+        ALLOC(eta, n);
+        if (work)
+            for (T4** p : {&pos, &vel, &xstar, &Pw, &PL, &vsw, &curl, &posS}) ALLOC(*p, n);
+#undef ALLOC
+        ready = true;
+        return hipSuccess;
+    }
+    void release() {
+        for (T4** p : {&ck_pos, &ck_vel, &xold, &vs, &P, &aux, &dp, &visc, &eta, &pos, &vel, &xstar, &Pw, &PL, &vsw,
+                       &curl, &posS})
+            dev_free(p);
+        dev_free(&lr);
+        ready = false;
+    }
+};
+
 }  // namespace
 
 struct fsi_adjoint {
@@ -775,20 +1081,12 @@ struct fsi_adjoint {
     fsi_fluid_config cfg{};
     fsi_pbf_constants k{};
     uint32_t n = 0, frames = 0, S = 0, I = 0;
+    uint32_t precision = 32;  // of the trajectory: 32 (the solver's kernels) or 64
+    double visc64 = 0.0, eps64 = 0.0;  // the parameters of the FP64 trajectory
     bool forward_done = false;
-    // checkpoints: frames + 1 states (the last is the forward's result)
-    float4* ck_pos = nullptr;
-    float4* ck_vel = nullptr;
-    // tape of one frame
-    float4* xold = nullptr;   // S x n, id order
-    uint32_t* perm = nullptr; // S x n
-    float4* vs = nullptr;     // S x n
-    float4* P = nullptr;      // S x (I + 1) x n
-    float2* lr = nullptr;     // S x I x n
-    float4* aux = nullptr;    // S x I x n
-    float4* dp = nullptr;     // S x I x n
-    float4* visc = nullptr;   // S x n
-    float4* eta = nullptr;    // n
+    Tape<float4> t32;
+    Tape<double4> t64;
+    uint32_t* perm = nullptr;  // S x n
     std::vector<EdgeList> lists;  // S x (I + 1) + 1
     uint32_t* list_mem = nullptr;
     // adjoint work (FP64)
@@ -817,17 +1115,9 @@ int bind(fsi_adjoint* a) {
 }
 
 void free_all(fsi_adjoint* a) {
-    dev_free(&a->ck_pos);
-    dev_free(&a->ck_vel);
-    dev_free(&a->xold);
+    a->t32.release();
+    a->t64.release();
     dev_free(&a->perm);
-    dev_free(&a->vs);
-    dev_free(&a->P);
-    dev_free(&a->lr);
-    dev_free(&a->aux);
-    dev_free(&a->dp);
-    dev_free(&a->visc);
-    dev_free(&a->eta);
     dev_free(&a->list_mem);
     for (double4** p : {&a->gpos, &a->gvel, &a->gxold, &a->gP, &a->gQ, &a->gown, &a->gvs, &a->gvs_own, &a->gw,
                         &a->gw_own, &a->ep, &a->ev})
@@ -852,22 +1142,67 @@ StepConst step_const(const fsi_adjoint* a) {
     return c;
 }
 
-hipError_t alloc_all(fsi_adjoint* a) {
+// The step constants in FP64.  FP32 trajectory: the FP32 constants widened,
+// so that every decision of the reverse passes is the forward's.  FP64
+// trajectory: the derived coefficients of PBFluids::setParams
+// (src/fluid/PBFluids.cpp:44-101; wq from Kernels.cpp:9-27) evaluated in FP64
+// from the configuration.
+C64 c64_of(const fsi_adjoint* a, const StepConst& c) {
+    C64 k{};
+    for (int x = 0; x < 3; ++x) {
+        k.g[x] = c.g[x];
+        k.bmin[x] = c.dom.bmin[x];
+        k.bmax[x] = c.dom.bmax[x];
+    }
+    k.sub_dt = c.sub_dt;
+    k.h = c.h;
+    k.rho0 = c.rho0;
+    k.eps = c.eps;
+    k.wq = c.wq;
+    k.k_corr = c.k_corr;
+    k.n_corr = c.n_corr;
+    k.viscosity = c.viscosity;
+    k.damping = c.damping;
+    k.cohesion = c.cohesion;
+    k.w0_self = c.w0_self;
+    k.poly6 = c.poly6;
+    k.spiky = c.spiky;
+    k.inv_rho0 = c.inv_rho0;
+    k.vort_eps = c.vort_eps;
+    k.n = c.n;
+    k.hash_mask = c.dom.hash_mask;
+    k.enable_scorr = c.enable_scorr;
+    k.enable_visc = c.enable_visc;
+    k.enable_vort = c.enable_vort;
+    if (a->precision == 64) {
+        k.viscosity = a->visc64;
+        k.eps = a->eps64;
+        const double PI = 3.14159265358979323846;
+        const double h = a->cfg.h;
+        const double h2 = h * h, h3 = h2 * h, h6 = h3 * h3, h9 = h6 * h3;
+        k.sub_dt = (double)a->cfg.dt / (double)a->cfg.substep_iterations;
+        k.w0_self = (315.0 / (64.0 * PI)) / h3;
+        k.poly6 = 315.0 / (64.0 * PI * h9);
+        k.spiky = 45.0 / (PI * h6);
+        k.inv_rho0 = 1.0 / (double)a->cfg.rho0;
+        const double dq = (double)a->cfg.delta_q * h;
+        const double r2 = dq * dq;
+        if (r2 > h2) {
+            k.wq = 0.0;
+        } else {
+            const double h4 = h2 * h2, diff = h2 - r2;
+            k.wq = (315.0 / (64.0 * PI) / (h4 * h4 * h)) * (diff * diff * diff);
+        }
+    }
+    return k;
+}
+
+hipError_t alloc_common(fsi_adjoint* a) {
     const size_t n = a->n, S = a->S, I = a->I;
     hipError_t e = hipSuccess;
 #define ALLOC(p, cnt) \
     if ((e = dev_alloc(&(p), (cnt))) != hipSuccess) return e;
-    ALLOC(a->ck_pos, (a->frames + 1) * n);
-    ALLOC(a->ck_vel, (a->frames + 1) * n);
-    ALLOC(a->xold, S * n);
     ALLOC(a->perm, S * n);
-    ALLOC(a->vs, S * n);
-    ALLOC(a->P, S * (I + 1) * n);
-    ALLOC(a->lr, S * I * n);
-    ALLOC(a->aux, S * I * n);
-    ALLOC(a->dp, S * I * n);
-    ALLOC(a->visc, S * n);
-    ALLOC(a->eta, n);
     const size_t nl = S * (I + 1) + 1;
     const size_t per = ((size_t)kEdgeCap + kInCap + 2) * n;
     ALLOC(a->list_mem, nl * per);
@@ -907,56 +1242,108 @@ hipError_t join_out(fsi_adjoint* a, void* stream) {
     return hipStreamWaitEvent((hipStream_t)stream, a->ev_out, 0);
 }
 
-hipError_t build_list(fsi_adjoint* a, const StepConst& c, const EdgeList& L, const float4* X) {
+hipError_t finish_list(fsi_adjoint* a, const EdgeList& L) {
     hipStream_t st = a->core.stream;
     const uint32_t n = a->n;
-    hipLaunchKernelGGL(k_edges, agrid(n), dim3(kAB), 0, st, c, X, a->core.d.cells, L.nbr, L.cnt, a->flags);
     hipLaunchKernelGGL(k_zero_u32, agrid(n), dim3(kAB), 0, st, L.tcnt, (size_t)n);
     hipLaunchKernelGGL(k_transpose, agrid(n), dim3(kAB), 0, st, n, L.nbr, L.cnt, L.tin, L.tcnt, a->flags);
     hipLaunchKernelGGL(k_tsort, agrid(n), dim3(kAB), 0, st, n, L.tin, L.tcnt, a->flags);
     return hipGetLastError();
 }
 
+hipError_t build_list(fsi_adjoint* a, const StepConst& c, const EdgeList& L, const float4* X) {
+    hipLaunchKernelGGL(k_edges, agrid(a->n), dim3(kAB), 0, a->core.stream, c, X, a->core.d.cells, L.nbr, L.cnt,
+                       a->flags);
+    return finish_list(a, L);
+}
+
+hipError_t build_list(fsi_adjoint* a, const C64& k, const EdgeList& L, const double4* X) {
+    hipLaunchKernelGGL(k_edges64, agrid(a->n), dim3(kAB), 0, a->core.stream, k, X, a->core.d.cells, L.nbr, L.cnt,
+                       a->flags);
+    return finish_list(a, L);
+}
+
 const EdgeList& list_of(const fsi_adjoint* a, uint32_t s, uint32_t k) { return a->lists[(size_t)s * (a->I + 1) + k]; }
 
-// Replays one frame from the state in d.pos / d.vel, writing the tape.
+// Replays one frame of the FP32 trajectory from the state in d.pos / d.vel,
+// writing the tape.
 int record_frame(fsi_adjoint* a, const StepConst& c) {
     DeviceState& d = a->core.d;
+    Tape<float4>& t = a->t32;
     hipStream_t st = a->core.stream;
     const size_t n = a->n;
     const uint32_t I = a->I;
     const int key_bits = log2u(a->cfg.hash_size);
     for (uint32_t s = 0; s < a->S; ++s) {
-        FSI_HIP_TRY(hipMemcpyAsync(a->xold + s * n, d.pos, sizeof(float4) * n, hipMemcpyDeviceToDevice, st));
+        FSI_HIP_TRY(hipMemcpyAsync(t.xold + s * n, d.pos, sizeof(float4) * n, hipMemcpyDeviceToDevice, st));
         FSI_HIP_TRY(launch_predict(c, d, st));
         FSI_HIP_TRY(radix_sort_pairs(d.keys, d.n, key_bits, a->core.sort_digit_bits, d.sort, &d.skeys, &d.perm, st));
         FSI_HIP_TRY(launch_cells(c, d, st));
         FSI_HIP_TRY(launch_reorder(c, d, st));
-        uint32_t* perm = a->perm + s * n;
-        float4* vs = a->vs + s * n;
-        float4* Ps = a->P + (size_t)s * (I + 1) * n;
-        FSI_HIP_TRY(hipMemcpyAsync(perm, d.perm, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, st));
-        FSI_HIP_TRY(hipMemcpyAsync(vs, d.vs, sizeof(float4) * n, hipMemcpyDeviceToDevice, st));
+        float4* Ps = t.P + (size_t)s * (I + 1) * n;
+        FSI_HIP_TRY(hipMemcpyAsync(a->perm + s * n, d.perm, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, st));
+        FSI_HIP_TRY(hipMemcpyAsync(t.vs + s * n, d.vs, sizeof(float4) * n, hipMemcpyDeviceToDevice, st));
         FSI_HIP_TRY(hipMemcpyAsync(Ps, d.P, sizeof(float4) * n, hipMemcpyDeviceToDevice, st));
         for (uint32_t it = 0; it < I; ++it) {
             const EdgeList& L = list_of(a, s, it);
             FSI_HIP_TRY(build_list(a, c, L, d.P));
-            const size_t t = ((size_t)s * I + it) * n;
-            hipLaunchKernelGGL(k_rec_lambda, agrid(a->n), dim3(kAB), 0, st, c, d.P, L.nbr, L.cnt, d.PL, a->lr + t,
-                               a->aux + t);
+            const size_t q = ((size_t)s * I + it) * n;
+            hipLaunchKernelGGL(k_rec_lambda, agrid(a->n), dim3(kAB), 0, st, c, d.P, L.nbr, L.cnt, d.PL, t.lr + q,
+                               t.aux + q);
             hipLaunchKernelGGL(k_rec_deltap, agrid(a->n), dim3(kAB), 0, st, c, d.PL, L.nbr, L.cnt, d.P,
-                               Ps + (it + 1) * n, a->dp + t);
+                               Ps + (it + 1) * n, t.dp + q);
         }
         const EdgeList& L = list_of(a, s, I);
         FSI_HIP_TRY(build_list(a, c, L, d.P));
         hipLaunchKernelGGL(k_rec_integrate, agrid(a->n), dim3(kAB), 0, st, c, d.P, d.vs, d.perm, d.pos, d.vel, L.nbr,
-                           L.cnt, d.curl, d.posS, a->visc + s * n);
+                           L.cnt, d.curl, d.posS, t.visc + s * n);
     }
     if (c.enable_vort) {
         const EdgeList& L = a->lists.back();
         FSI_HIP_TRY(build_list(a, c, L, d.posS));
         hipLaunchKernelGGL(k_rec_vorticity, agrid(a->n), dim3(kAB), 0, st, c, d.posS, d.curl, d.perm, d.vel, L.nbr,
-                           L.cnt, a->eta);
+                           L.cnt, t.eta);
+    }
+    FSI_HIP_TRY(hipGetLastError());
+    return FSI_OK;
+}
+
+// One frame of the FP64 trajectory from the state in t64.pos / t64.vel,
+// writing the tape (the forward runs this too: the lists are the search).
+int record_frame64(fsi_adjoint* a, const StepConst& c, const C64& k) {
+    DeviceState& d = a->core.d;
+    Tape<double4>& t = a->t64;
+    hipStream_t st = a->core.stream;
+    const size_t n = a->n;
+    const uint32_t I = a->I;
+    const int key_bits = log2u(a->cfg.hash_size);
+    const dim3 g = agrid(a->n), b(kAB);
+    for (uint32_t s = 0; s < a->S; ++s) {
+        FSI_HIP_TRY(hipMemcpyAsync(t.xold + s * n, t.pos, sizeof(double4) * n, hipMemcpyDeviceToDevice, st));
+        hipLaunchKernelGGL(k_predict64, g, b, 0, st, k, t.pos, t.vel, t.xstar, d.keys);
+        FSI_HIP_TRY(radix_sort_pairs(d.keys, d.n, key_bits, a->core.sort_digit_bits, d.sort, &d.skeys, &d.perm, st));
+        FSI_HIP_TRY(launch_cells(c, d, st));
+        hipLaunchKernelGGL(k_reorder64, g, b, 0, st, a->n, d.perm, t.xstar, t.vel, t.Pw, t.vsw);
+        double4* Ps = t.P + (size_t)s * (I + 1) * n;
+        FSI_HIP_TRY(hipMemcpyAsync(a->perm + s * n, d.perm, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, st));
+        FSI_HIP_TRY(hipMemcpyAsync(t.vs + s * n, t.vsw, sizeof(double4) * n, hipMemcpyDeviceToDevice, st));
+        FSI_HIP_TRY(hipMemcpyAsync(Ps, t.Pw, sizeof(double4) * n, hipMemcpyDeviceToDevice, st));
+        for (uint32_t it = 0; it < I; ++it) {
+            const EdgeList& L = list_of(a, s, it);
+            FSI_HIP_TRY(build_list(a, k, L, t.Pw));
+            const size_t q = ((size_t)s * I + it) * n;
+            hipLaunchKernelGGL(k_rec_lambda64, g, b, 0, st, k, t.Pw, L.nbr, L.cnt, t.PL, t.lr + q, t.aux + q);
+            hipLaunchKernelGGL(k_rec_deltap64, g, b, 0, st, k, t.PL, L.nbr, L.cnt, t.Pw, Ps + (it + 1) * n, t.dp + q);
+        }
+        const EdgeList& L = list_of(a, s, I);
+        FSI_HIP_TRY(build_list(a, k, L, t.Pw));
+        hipLaunchKernelGGL(k_rec_integrate64, g, b, 0, st, k, t.Pw, t.vsw, d.perm, t.pos, t.vel, L.nbr, L.cnt, t.curl,
+                           t.posS, t.visc + s * n);
+    }
+    if (k.enable_vort) {
+        const EdgeList& L = a->lists.back();
+        FSI_HIP_TRY(build_list(a, k, L, t.posS));
+        hipLaunchKernelGGL(k_rec_vorticity64, g, b, 0, st, k, t.posS, t.curl, d.perm, t.vel, L.nbr, L.cnt, t.eta);
     }
     FSI_HIP_TRY(hipGetLastError());
     return FSI_OK;
@@ -964,49 +1351,221 @@ int record_frame(fsi_adjoint* a, const StepConst& c) {
 
 // Runs the recorded frame backwards: gpos / gvel (id order) hold the adjoint
 // of the frame's final state on entry and of its initial state on exit.
-int reverse_frame(fsi_adjoint* a, const StepConst& c) {
-    DeviceState& d = a->core.d;
+// posS / curl: the last substep's positions and curl (sorted slots).
+template <class T4>
+int reverse_frame(fsi_adjoint* a, const C64& k, const Tape<T4>& t, const T4* posS, const T4* curl) {
     hipStream_t st = a->core.stream;
     const uint32_t n = a->n;
     const size_t ns = n;
     const uint32_t I = a->I, S = a->S;
     const dim3 g = agrid(n), b(kAB);
     bool have_gw = false;
-    if (c.enable_vort) {
+    if (k.enable_vort) {
         const EdgeList& L = a->lists.back();
         const uint32_t* perm = a->perm + (S - 1) * ns;
-        hipLaunchKernelGGL(k_rev_vort, g, b, 0, st, c, d.posS, d.curl, a->eta, perm, a->gvel, L.nbr, L.cnt, a->gown,
+        hipLaunchKernelGGL((k_rev_vort<T4>), g, b, 0, st, k, posS, curl, t.eta, perm, a->gvel, L.nbr, L.cnt, a->gown,
                            a->gw_own, a->ep, a->es);
-        hipLaunchKernelGGL(k_rev_vort_t, g, b, 0, st, n, d.curl, perm, L.tin, L.tcnt, a->ep, a->es, a->gown, a->gw_own,
-                           a->gpos, a->gw);
+        hipLaunchKernelGGL((k_rev_vort_t<T4>), g, b, 0, st, n, curl, perm, L.tin, L.tcnt, a->ep, a->es, a->gown,
+                           a->gw_own, a->gpos, a->gw);
         have_gw = true;
     }
     for (uint32_t s = S; s-- > 0;) {
         const uint32_t* perm = a->perm + s * ns;
-        const float4* Ps = a->P + (size_t)s * (I + 1) * ns;
+        const T4* Ps = t.P + (size_t)s * (I + 1) * ns;
         const EdgeList& Li = list_of(a, s, I);
-        hipLaunchKernelGGL(k_rev_integrate, g, b, 0, st, c, a->xold + s * ns, a->vs + s * ns, Ps + I * ns,
-                           a->visc + s * ns, perm, Li.nbr, Li.cnt, a->gpos, a->gvel,
+        hipLaunchKernelGGL((k_rev_integrate<T4>), g, b, 0, st, k, t.xold + s * ns, t.vs + s * ns, Ps + I * ns,
+                           t.visc + s * ns, perm, Li.nbr, Li.cnt, a->gpos, a->gvel,
                            (have_gw && s == S - 1) ? a->gw : nullptr, a->gown, a->gvs_own, a->gxold, a->acc_visc,
                            a->ep, a->ev);
         hipLaunchKernelGGL(k_rev_pairs_vec2, g, b, 0, st, n, Li.tin, Li.tcnt, a->ep, a->ev, a->gown, a->gvs_own, a->gP,
                            a->gvs);
         for (uint32_t it = I; it-- > 0;) {
             const EdgeList& L = list_of(a, s, it);
-            const size_t t = ((size_t)s * I + it) * ns;
-            const float4* Pprev = Ps + it * ns;
-            hipLaunchKernelGGL(k_rev_deltap, g, b, 0, st, c, Pprev, a->lr + t, a->dp + t, L.nbr, L.cnt, a->gP, a->gown,
-                               a->gl_own, a->ep, a->es);
+            const size_t q = ((size_t)s * I + it) * ns;
+            const T4* Pprev = Ps + it * ns;
+            hipLaunchKernelGGL((k_rev_deltap<T4>), g, b, 0, st, k, Pprev, t.lr + q, t.dp + q, L.nbr, L.cnt, a->gP,
+                               a->gown, a->gl_own, a->ep, a->es);
             hipLaunchKernelGGL(k_rev_pairs_vec_scalar, g, b, 0, st, n, L.tin, L.tcnt, a->ep, a->es, a->gown, a->gl_own,
                                a->gQ, a->gl);
-            hipLaunchKernelGGL(k_rev_lambda, g, b, 0, st, c, Pprev, a->lr + t, a->aux + t, perm, L.nbr, L.cnt, a->gl,
-                               a->gQ, a->acc_eps, a->ep);
+            hipLaunchKernelGGL((k_rev_lambda<T4>), g, b, 0, st, k, Pprev, t.lr + q, t.aux + q, perm, L.nbr, L.cnt,
+                               a->gl, a->gQ, a->acc_eps, a->ep);
             hipLaunchKernelGGL(k_rev_pairs_sub, g, b, 0, st, n, L.tin, L.tcnt, a->ep, a->gQ);
             std::swap(a->gP, a->gQ);
         }
-        hipLaunchKernelGGL(k_rev_predict, g, b, 0, st, c, perm, a->gP, a->gvs, a->gxold, a->gpos, a->gvel);
+        hipLaunchKernelGGL(k_rev_predict, g, b, 0, st, k, perm, a->gP, a->gvs, a->gxold, a->gpos, a->gvel);
     }
     FSI_HIP_TRY(hipGetLastError());
+    return FSI_OK;
+}
+
+// Sets the constants for the given parameters; returns the step constants.
+StepConst prepare(fsi_adjoint* a) {
+    pack_constants(a->cfg, a->cfg.bounds_min, a->cfg.bounds_max, std::clamp(a->cfg.bound_damping, 0.0f, 1.0f), a->n,
+                   a->k);
+    return step_const(a);
+}
+
+// Forward of either precision.  f64_io: states are n double4 (FP64
+// trajectory only), else n float4.
+int forward_impl(fsi_adjoint* a, const void* pos0, const void* vel0, bool f64_io, double viscosity, double eps,
+                 void* pos_out, void* vel_out, void* stream) {
+    int rc = bind(a);
+    if (rc) return rc;
+    if (!pos0 || !vel0) return fail(FSI_ERR_INVALID_ARG, "NULL state");
+    if (!std::isfinite(viscosity) || !std::isfinite(eps)) return fail(FSI_ERR_INVALID_ARG, "non-finite parameter");
+    if (f64_io && a->precision != 64) return fail(FSI_ERR_STATE, "FP64 states need the FP64 trajectory");
+    if (!(a->precision == 32 ? a->t32.ready : a->t64.ready)) return fail(FSI_ERR_STATE, "handle has no tape");
+    a->forward_done = false;
+    a->cfg.viscosity = (float)viscosity;
+    a->cfg.eps = (float)eps;
+    a->visc64 = f64_io ? viscosity : (double)a->cfg.viscosity;
+    a->eps64 = f64_io ? eps : (double)a->cfg.eps;
+    const StepConst c = prepare(a);
+    Core& core = a->core;
+    DeviceState& d = core.d;
+    hipStream_t st = core.stream;
+    const size_t n = a->n;
+    FSI_HIP_TRY(join_in(a, stream));
+    if (a->precision == 64) {
+        const C64 k = c64_of(a, c);
+        Tape<double4>& t = a->t64;
+        const dim3 g = agrid(a->n), b(kAB);
+        if (f64_io) {
+            FSI_HIP_TRY(hipMemcpyAsync(t.pos, pos0, sizeof(double4) * n, hipMemcpyDeviceToDevice, st));
+            FSI_HIP_TRY(hipMemcpyAsync(t.vel, vel0, sizeof(double4) * n, hipMemcpyDeviceToDevice, st));
+        } else {
+            hipLaunchKernelGGL(k_f2d, g, b, 0, st, a->n, (const float4*)pos0, t.pos, 1);
+            hipLaunchKernelGGL(k_f2d, g, b, 0, st, a->n, (const float4*)vel0, t.vel, 1);
+        }
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(kAB), 0, st, a->flags, (size_t)4);
+        for (uint32_t f = 0; f < a->frames; ++f) {
+            FSI_HIP_TRY(hipMemcpyAsync(t.ck_pos + f * n, t.pos, sizeof(double4) * n, hipMemcpyDeviceToDevice, st));
+            FSI_HIP_TRY(hipMemcpyAsync(t.ck_vel + f * n, t.vel, sizeof(double4) * n, hipMemcpyDeviceToDevice, st));
+            if ((rc = record_frame64(a, c, k))) return rc;
+        }
+        FSI_HIP_TRY(hipMemcpyAsync(t.ck_pos + a->frames * n, t.pos, sizeof(double4) * n, hipMemcpyDeviceToDevice, st));
+        FSI_HIP_TRY(hipMemcpyAsync(t.ck_vel + a->frames * n, t.vel, sizeof(double4) * n, hipMemcpyDeviceToDevice, st));
+        if (f64_io) {
+            if (pos_out) FSI_HIP_TRY(hipMemcpyAsync(pos_out, t.pos, sizeof(double4) * n, hipMemcpyDeviceToDevice, st));
+            if (vel_out) FSI_HIP_TRY(hipMemcpyAsync(vel_out, t.vel, sizeof(double4) * n, hipMemcpyDeviceToDevice, st));
+        } else {
+            if (pos_out) hipLaunchKernelGGL(k_d2f, g, b, 0, st, a->n, t.pos, (float4*)pos_out, 1);
+            if (vel_out) hipLaunchKernelGGL(k_d2f, g, b, 0, st, a->n, t.vel, (float4*)vel_out, 1);
+        }
+        FSI_HIP_TRY(hipGetLastError());
+        uint32_t flags[4];
+        FSI_HIP_TRY(hipMemcpyAsync(flags, a->flags, sizeof(flags), hipMemcpyDeviceToHost, st));
+        FSI_HIP_TRY(join_out(a, stream));
+        FSI_HIP_TRY(hipStreamSynchronize(st));
+        if (flags[0])
+            return fail(FSI_ERR_CAPACITY, "more than 96 neighbours (" + std::to_string(flags[2]) +
+                                              ") or 128 pairs into one particle (" + std::to_string(flags[3]) +
+                                              ") in a pass");
+        a->forward_done = true;
+        return FSI_OK;
+    }
+    Tape<float4>& t = a->t32;
+    d.tiled = c.dom.dshift == kTileShift;
+    FramePlan plan;
+    plan.substeps = a->S;
+    plan.max_iter = a->I;
+    plan.key_bits = log2u(a->cfg.hash_size);
+    plan.vorticity = a->cfg.enable_vorticity != 0;
+    FSI_HIP_TRY(hipMemcpyAsync(d.pos, pos0, sizeof(float4) * n, hipMemcpyDeviceToDevice, st));
+    FSI_HIP_TRY(hipMemcpyAsync(d.vel, vel0, sizeof(float4) * n, hipMemcpyDeviceToDevice, st));
+    for (uint32_t f = 0; f < a->frames; ++f) {
+        FSI_HIP_TRY(hipMemcpyAsync(t.ck_pos + f * n, d.pos, sizeof(float4) * n, hipMemcpyDeviceToDevice, st));
+        FSI_HIP_TRY(hipMemcpyAsync(t.ck_vel + f * n, d.vel, sizeof(float4) * n, hipMemcpyDeviceToDevice, st));
+        if ((rc = core_frame(core, c, plan))) return rc;
+    }
+    FSI_HIP_TRY(hipMemcpyAsync(t.ck_pos + a->frames * n, d.pos, sizeof(float4) * n, hipMemcpyDeviceToDevice, st));
+    FSI_HIP_TRY(hipMemcpyAsync(t.ck_vel + a->frames * n, d.vel, sizeof(float4) * n, hipMemcpyDeviceToDevice, st));
+    if (pos_out) FSI_HIP_TRY(hipMemcpyAsync(pos_out, d.pos, sizeof(float4) * n, hipMemcpyDeviceToDevice, st));
+    if (vel_out) FSI_HIP_TRY(hipMemcpyAsync(vel_out, d.vel, sizeof(float4) * n, hipMemcpyDeviceToDevice, st));
+    FSI_HIP_TRY(join_out(a, stream));
+    a->forward_done = true;
+    return FSI_OK;
+}
+
+// Backward of either precision; f64_io as in forward_impl.
+int backward_impl(fsi_adjoint* a, const void* grad_pos, const void* grad_vel, bool f64_io, void* grad_pos0,
+                  void* grad_vel0, double* grad_params, void* stream) {
+    int rc = bind(a);
+    if (rc) return rc;
+    if (!a->forward_done) return fail(FSI_ERR_STATE, "backward before forward");
+    if (f64_io && a->precision != 64) return fail(FSI_ERR_STATE, "FP64 states need the FP64 trajectory");
+    const StepConst c = step_const(a);
+    const C64 k = c64_of(a, c);
+    Core& core = a->core;
+    DeviceState& d = core.d;
+    hipStream_t st = core.stream;
+    const size_t n = a->n;
+    const dim3 g = agrid(a->n), b(kAB);
+    FSI_HIP_TRY(join_in(a, stream));
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(kAB), 0, st, a->flags, (size_t)4);
+    hipLaunchKernelGGL(k_zero_u32, agrid(2 * a->frames), dim3(kAB), 0, st, a->norms, 2 * (size_t)a->frames);
+    hipLaunchKernelGGL(k_zero_f64, g, b, 0, st, a->acc_visc, n);
+    hipLaunchKernelGGL(k_zero_f64, g, b, 0, st, a->acc_eps, n);
+    for (int q = 0; q < 2; ++q) {
+        const void* src = q == 0 ? grad_pos : grad_vel;
+        double4* dst = q == 0 ? a->gpos : a->gvel;
+        if (!src)
+            hipLaunchKernelGGL(k_zero_f64, agrid(4 * a->n), b, 0, st, (double*)dst, 4 * n);
+        else if (f64_io)
+            FSI_HIP_TRY(hipMemcpyAsync(dst, src, sizeof(double4) * n, hipMemcpyDeviceToDevice, st));
+        else
+            hipLaunchKernelGGL(k_f2d, g, b, 0, st, a->n, (const float4*)src, dst, 0);
+    }
+    for (uint32_t f = a->frames; f-- > 0;) {
+        if (a->precision == 64) {
+            Tape<double4>& t = a->t64;
+            FSI_HIP_TRY(hipMemcpyAsync(t.pos, t.ck_pos + f * n, sizeof(double4) * n, hipMemcpyDeviceToDevice, st));
+            FSI_HIP_TRY(hipMemcpyAsync(t.vel, t.ck_vel + f * n, sizeof(double4) * n, hipMemcpyDeviceToDevice, st));
+            if ((rc = record_frame64(a, c, k))) return rc;
+            hipLaunchKernelGGL((k_compare<double4>), g, b, 0, st, a->n, t.pos, t.ck_pos + (f + 1) * n, a->flags + 1);
+            hipLaunchKernelGGL((k_compare<double4>), g, b, 0, st, a->n, t.vel, t.ck_vel + (f + 1) * n, a->flags + 1);
+            if ((rc = reverse_frame(a, k, t, (const double4*)t.posS, (const double4*)t.curl))) return rc;
+        } else {
+            Tape<float4>& t = a->t32;
+            FSI_HIP_TRY(hipMemcpyAsync(d.pos, t.ck_pos + f * n, sizeof(float4) * n, hipMemcpyDeviceToDevice, st));
+            FSI_HIP_TRY(hipMemcpyAsync(d.vel, t.ck_vel + f * n, sizeof(float4) * n, hipMemcpyDeviceToDevice, st));
+            if ((rc = record_frame(a, c))) return rc;
+            hipLaunchKernelGGL((k_compare<float4>), g, b, 0, st, a->n, d.pos, t.ck_pos + (f + 1) * n, a->flags + 1);
+            hipLaunchKernelGGL((k_compare<float4>), g, b, 0, st, a->n, d.vel, t.ck_vel + (f + 1) * n, a->flags + 1);
+            if ((rc = reverse_frame(a, k, t, (const float4*)d.posS, (const float4*)d.curl))) return rc;
+        }
+        hipLaunchKernelGGL(k_absmax, g, b, 0, st, a->n, a->gpos, a->gvel, a->norms + 2 * (size_t)f);
+    }
+    hipLaunchKernelGGL(k_reduce2, dim3(1), dim3(kAB), 0, st, a->n, a->acc_visc, a->acc_eps, a->dred);
+    for (int q = 0; q < 2; ++q) {
+        void* dst = q == 0 ? grad_pos0 : grad_vel0;
+        const double4* src = q == 0 ? a->gpos : a->gvel;
+        if (!dst) continue;
+        if (f64_io)
+            FSI_HIP_TRY(hipMemcpyAsync(dst, src, sizeof(double4) * n, hipMemcpyDeviceToDevice, st));
+        else
+            hipLaunchKernelGGL(k_d2f, g, b, 0, st, a->n, src, (float4*)dst, 0);
+    }
+    FSI_HIP_TRY(hipGetLastError());
+    double red[2] = {0.0, 0.0};
+    uint32_t* flags = a->stats;
+    FSI_HIP_TRY(hipMemcpyAsync(red, a->dred, sizeof(red), hipMemcpyDeviceToHost, st));
+    FSI_HIP_TRY(hipMemcpyAsync(flags, a->flags, sizeof(a->stats), hipMemcpyDeviceToHost, st));
+    a->host_norms.resize(2 * (size_t)a->frames);
+    FSI_HIP_TRY(hipMemcpyAsync(a->host_norms.data(), a->norms, sizeof(uint32_t) * a->host_norms.size(),
+                               hipMemcpyDeviceToHost, st));
+    FSI_HIP_TRY(join_out(a, stream));
+    FSI_HIP_TRY(hipStreamSynchronize(st));
+    if (flags[1])
+        return fail(FSI_ERR_STATE, "replay differs from the forward (" + std::to_string(flags[1]) + " records)");
+    if (flags[0])
+        return fail(FSI_ERR_CAPACITY, "more than 96 neighbours (" + std::to_string(flags[2]) +
+                                          ") or 128 pairs into one particle (" + std::to_string(flags[3]) +
+                                          ") in a pass");
+    if (grad_params) {
+        grad_params[0] = red[0];
+        grad_params[1] = red[1];
+    }
     return FSI_OK;
 }
 
@@ -1045,7 +1604,8 @@ int fsi_adj_create(const fsi_fluid_config* cfg, uint32_t n, uint32_t frames, fsi
     a->I = cfg->solver_iterations;
     hipError_t e = core_alloc_particles(a->core, n);
     if (e == hipSuccess) e = core_alloc_cells(a->core, cfg->hash_size);
-    if (e == hipSuccess) e = alloc_all(a);
+    if (e == hipSuccess) e = alloc_common(a);
+    if (e == hipSuccess) e = a->t32.alloc(n, frames, a->S, a->I, false);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&a->ev_in, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&a->ev_out, hipEventDisableTiming);
     if (e != hipSuccess) return bail(fail(FSI_ERR_HIP, std::string("allocation: ") + hipGetErrorString(e)));
@@ -1067,100 +1627,51 @@ int fsi_adj_destroy(fsi_adjoint* a) {
     return FSI_OK;
 }
 
-int fsi_adj_forward(fsi_adjoint* a, const float* pos0, const float* vel0, float viscosity, float eps, float* pos_out,
-                    float* vel_out, void* stream) {
+int fsi_adj_set_precision(fsi_adjoint* a, uint32_t bits) {
     int rc = bind(a);
     if (rc) return rc;
-    if (!pos0 || !vel0) return fail(FSI_ERR_INVALID_ARG, "NULL state");
-    if (!std::isfinite(viscosity) || !std::isfinite(eps)) return fail(FSI_ERR_INVALID_ARG, "non-finite parameter");
-    a->cfg.viscosity = viscosity;
-    a->cfg.eps = eps;
-    pack_constants(a->cfg, a->cfg.bounds_min, a->cfg.bounds_max, std::clamp(a->cfg.bound_damping, 0.0f, 1.0f), a->n,
-                   a->k);
-    const StepConst c = step_const(a);
-    a->core.d.tiled = c.dom.dshift == kTileShift;
-    FramePlan plan;
-    plan.substeps = a->S;
-    plan.max_iter = a->I;
-    plan.key_bits = log2u(a->cfg.hash_size);
-    plan.vorticity = a->cfg.enable_vorticity != 0;
-    Core& core = a->core;
-    DeviceState& d = core.d;
-    hipStream_t st = core.stream;
-    const size_t n = a->n;
-    FSI_HIP_TRY(join_in(a, stream));
-    FSI_HIP_TRY(hipMemcpyAsync(d.pos, pos0, sizeof(float4) * n, hipMemcpyDeviceToDevice, st));
-    FSI_HIP_TRY(hipMemcpyAsync(d.vel, vel0, sizeof(float4) * n, hipMemcpyDeviceToDevice, st));
-    for (uint32_t f = 0; f < a->frames; ++f) {
-        FSI_HIP_TRY(hipMemcpyAsync(a->ck_pos + f * n, d.pos, sizeof(float4) * n, hipMemcpyDeviceToDevice, st));
-        FSI_HIP_TRY(hipMemcpyAsync(a->ck_vel + f * n, d.vel, sizeof(float4) * n, hipMemcpyDeviceToDevice, st));
-        if ((rc = core_frame(core, c, plan))) return rc;
+    if (bits != 32 && bits != 64) return fail(FSI_ERR_INVALID_ARG, "precision must be 32 or 64");
+    if (bits == a->precision) return FSI_OK;
+    FSI_HIP_TRY(hipStreamSynchronize(a->core.stream));
+    Tape<float4>& t32 = a->t32;
+    Tape<double4>& t64 = a->t64;
+    hipError_t e = hipSuccess;
+    if (bits == 64) {
+        t32.release();
+        e = t64.alloc(a->n, a->frames, a->S, a->I, true);
+    } else {
+        t64.release();
+        e = t32.alloc(a->n, a->frames, a->S, a->I, false);
     }
-    FSI_HIP_TRY(hipMemcpyAsync(a->ck_pos + a->frames * n, d.pos, sizeof(float4) * n, hipMemcpyDeviceToDevice, st));
-    FSI_HIP_TRY(hipMemcpyAsync(a->ck_vel + a->frames * n, d.vel, sizeof(float4) * n, hipMemcpyDeviceToDevice, st));
-    if (pos_out) FSI_HIP_TRY(hipMemcpyAsync(pos_out, d.pos, sizeof(float4) * n, hipMemcpyDeviceToDevice, st));
-    if (vel_out) FSI_HIP_TRY(hipMemcpyAsync(vel_out, d.vel, sizeof(float4) * n, hipMemcpyDeviceToDevice, st));
-    FSI_HIP_TRY(join_out(a, stream));
-    a->forward_done = true;
+    if (e != hipSuccess) {
+        t32.release();
+        t64.release();
+        a->forward_done = false;
+        return fail(FSI_ERR_HIP, std::string("allocation: ") + hipGetErrorString(e) + " (handle has no tape now)");
+    }
+    a->precision = bits;
+    a->forward_done = false;
     return FSI_OK;
+}
+
+int fsi_adj_forward(fsi_adjoint* a, const float* pos0, const float* vel0, float viscosity, float eps, float* pos_out,
+                    float* vel_out, void* stream) {
+    return forward_impl(a, pos0, vel0, false, viscosity, eps, pos_out, vel_out, stream);
+}
+
+int fsi_adj_forward64(fsi_adjoint* a, const double* pos0, const double* vel0, double viscosity, double eps,
+                      double* pos_out, double* vel_out, void* stream) {
+    return forward_impl(a, pos0, vel0, true, viscosity, eps, pos_out, vel_out, stream);
 }
 
 int fsi_adj_backward(fsi_adjoint* a, const float* grad_pos, const float* grad_vel, float* grad_pos0,
                      float* grad_vel0, double* grad_params, void* stream) {
-    int rc = bind(a);
-    if (rc) return rc;
-    if (!a->forward_done) return fail(FSI_ERR_STATE, "backward before forward");
-    const StepConst c = step_const(a);
-    Core& core = a->core;
-    DeviceState& d = core.d;
-    hipStream_t st = core.stream;
-    const size_t n = a->n;
-    FSI_HIP_TRY(join_in(a, stream));
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(kAB), 0, st, a->flags, (size_t)4);
-    hipLaunchKernelGGL(k_zero_u32, agrid(2 * a->frames), dim3(kAB), 0, st, a->norms, 2 * (size_t)a->frames);
-    hipLaunchKernelGGL(k_zero_f64, agrid(a->n), dim3(kAB), 0, st, a->acc_visc, n);
-    hipLaunchKernelGGL(k_zero_f64, agrid(a->n), dim3(kAB), 0, st, a->acc_eps, n);
-    if (grad_pos)
-        hipLaunchKernelGGL(k_f2d, agrid(a->n), dim3(kAB), 0, st, a->n, (const float4*)grad_pos, a->gpos);
-    else
-        hipLaunchKernelGGL(k_zero_f64, agrid(4 * a->n), dim3(kAB), 0, st, (double*)a->gpos, 4 * n);
-    if (grad_vel)
-        hipLaunchKernelGGL(k_f2d, agrid(a->n), dim3(kAB), 0, st, a->n, (const float4*)grad_vel, a->gvel);
-    else
-        hipLaunchKernelGGL(k_zero_f64, agrid(4 * a->n), dim3(kAB), 0, st, (double*)a->gvel, 4 * n);
-    for (uint32_t f = a->frames; f-- > 0;) {
-        FSI_HIP_TRY(hipMemcpyAsync(d.pos, a->ck_pos + f * n, sizeof(float4) * n, hipMemcpyDeviceToDevice, st));
-        FSI_HIP_TRY(hipMemcpyAsync(d.vel, a->ck_vel + f * n, sizeof(float4) * n, hipMemcpyDeviceToDevice, st));
-        if ((rc = record_frame(a, c))) return rc;
-        hipLaunchKernelGGL(k_compare, agrid(a->n), dim3(kAB), 0, st, a->n, d.pos, a->ck_pos + (f + 1) * n, a->flags + 1);
-        hipLaunchKernelGGL(k_compare, agrid(a->n), dim3(kAB), 0, st, a->n, d.vel, a->ck_vel + (f + 1) * n, a->flags + 1);
-        if ((rc = reverse_frame(a, c))) return rc;
-        hipLaunchKernelGGL(k_absmax, agrid(a->n), dim3(kAB), 0, st, a->n, a->gpos, a->gvel, a->norms + 2 * (size_t)f);
-    }
-    hipLaunchKernelGGL(k_reduce2, dim3(1), dim3(kAB), 0, st, a->n, a->acc_visc, a->acc_eps, a->dred);
-    if (grad_pos0) hipLaunchKernelGGL(k_d2f, agrid(a->n), dim3(kAB), 0, st, a->n, a->gpos, (float4*)grad_pos0);
-    if (grad_vel0) hipLaunchKernelGGL(k_d2f, agrid(a->n), dim3(kAB), 0, st, a->n, a->gvel, (float4*)grad_vel0);
-    FSI_HIP_TRY(hipGetLastError());
-    double red[2] = {0.0, 0.0};
-    uint32_t* flags = a->stats;
-    FSI_HIP_TRY(hipMemcpyAsync(red, a->dred, sizeof(red), hipMemcpyDeviceToHost, st));
-    FSI_HIP_TRY(hipMemcpyAsync(flags, a->flags, sizeof(a->stats), hipMemcpyDeviceToHost, st));
-    a->host_norms.resize(2 * (size_t)a->frames);
-    FSI_HIP_TRY(hipMemcpyAsync(a->host_norms.data(), a->norms, sizeof(uint32_t) * a->host_norms.size(),
-                               hipMemcpyDeviceToHost, st));
-    FSI_HIP_TRY(join_out(a, stream));
-    FSI_HIP_TRY(hipStreamSynchronize(st));
-    if (flags[1])
-        return fail(FSI_ERR_STATE, "replay differs from the forward (" + std::to_string(flags[1]) + " records)");
-    if (flags[0])
-        return fail(FSI_ERR_CAPACITY, "more than 96 neighbours (" + std::to_string(flags[2]) +
-                                          ") or 128 pairs into one particle (" + std::to_string(flags[3]) +
-                                          ") in a pass");
-    if (grad_params) {
-        grad_params[0] = red[0];
-        grad_params[1] = red[1];
-    }
-    return FSI_OK;
+    return backward_impl(a, grad_pos, grad_vel, false, grad_pos0, grad_vel0, grad_params, stream);
+}
+
+int fsi_adj_backward64(fsi_adjoint* a, const double* grad_pos, const double* grad_vel, double* grad_pos0,
+                       double* grad_vel0, double* grad_params, void* stream) {
+    return backward_impl(a, grad_pos, grad_vel, true, grad_pos0, grad_vel0, grad_params, stream);
 }
 
 int fsi_adj_frame_norms(fsi_adjoint* a, float* out) {

@@ -16,6 +16,7 @@
 
 #include "../../include/fsi_pbf.h"
 #include "pbf_core.h"
+#include "pbf_mesh.h"
 
 using namespace fsi;
 
@@ -421,6 +422,47 @@ int fsi_pbf_read_contacts(fsi_pbf* h, fsi_contact_candidate* out, uint32_t cap, 
         FSI_HIP_TRY(hipMemcpy(out, h->contacts, sizeof(fsi_contact_candidate) * m, hipMemcpyDeviceToHost));
     }
     return FSI_OK;
+}
+
+int fsi_pbf_set_sdf_from_mesh(fsi_pbf* h, fsi_mesh* m) {
+    int rc = bind(h);
+    if (rc) return rc;
+    if (!m) return fail(FSI_ERR_INVALID_ARG, "mesh handle is NULL");
+    if (mesh_device(m) != h->core.device) return fail(FSI_ERR_INVALID_ARG, "mesh and solver on different devices");
+    const float* grid = nullptr;
+    int32_t dims[3];
+    double origin[3], cell = 0.0;
+    hipEvent_t ready = nullptr;
+    if ((rc = mesh_grid_for_solver(m, &grid, dims, origin, &cell, &ready))) return rc;
+    const size_t count = (size_t)dims[0] * (size_t)dims[1] * (size_t)dims[2];
+    const bool same =
+        h->sdf_grid && dims[0] == h->sdf_dims[0] && dims[1] == h->sdf_dims[1] && dims[2] == h->sdf_dims[2];
+    if (!same) {
+        FSI_HIP_TRY(hipStreamSynchronize(h->core.stream));
+        FSI_HIP_TRY(dev_alloc(&h->sdf_grid, count));
+    }
+    FSI_HIP_TRY(hipStreamWaitEvent(h->core.stream, ready, 0));
+    FSI_HIP_TRY(hipMemcpyAsync(h->sdf_grid, grid, count * sizeof(float), hipMemcpyDeviceToDevice, h->core.stream));
+    if ((rc = mesh_wait_for(m, h->core.stream))) return rc;
+    for (int a = 0; a < 3; ++a) {
+        h->sdf_dims[a] = dims[a];
+        h->sdf_origin[a] = (float)origin[a];
+    }
+    h->sdf_cell = (float)cell;
+    h->sdf_valid = true;
+    return FSI_OK;
+}
+
+int fsi_pbf_contact_forces(fsi_pbf* h, fsi_mesh* m, float band, int use_vel_magnitude, float control_threshold,
+                           float* forces, uint32_t counts[2]) {
+    int rc = bind(h);
+    if (rc) return rc;
+    if (!m) return fail(FSI_ERR_INVALID_ARG, "mesh handle is NULL");
+    if (mesh_device(m) != h->core.device) return fail(FSI_ERR_INVALID_ARG, "mesh and solver on different devices");
+    if (!h->contacts_valid || h->core.d.n == 0)
+        return fail(FSI_ERR_STATE, "no contact candidates (fsi_pbf_filter_contacts first)");
+    return mesh_contact_forces(m, h->contacts, h->contact_count, h->core.d.n, h->core.stream, band,
+                               use_vel_magnitude, control_threshold, forces, counts);
 }
 
 int fsi_pbf_read_grid(fsi_pbf* h, uint32_t* keys, uint32_t* ids, int32_t* cells) {

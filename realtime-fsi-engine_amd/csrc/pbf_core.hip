@@ -198,6 +198,7 @@ void core_destroy(Core& c) {
     dev_free(&d.vs);
     dev_free(&d.curl);
     dev_free(&d.posS);
+    dev_free(&d.vdelta);
     dev_free(&d.rho);
     dev_free(&d.lod);
     dev_free(&d.dp_tap);
@@ -260,6 +261,7 @@ hipError_t core_alloc_particles(Core& c, uint32_t n) {
     ALLOC(d.vs, n);
     ALLOC(d.curl, n);
     ALLOC(d.posS, n);
+    ALLOC(d.vdelta, n);
     ALLOC(d.rho, n);
     ALLOC(d.lod, n);
     ALLOC(d.dp_tap, n);

@@ -424,52 +424,73 @@ __global__ __launch_bounds__(kBlock) void k_apply_sdf(StepConst c, const uint32_
     if (sdf_project(dom, p, nrm)) P[k] = make_float4(p.x, p.y, p.z, 0.0f);
 }
 
-// CS_Integrate.glsl:135-280.  Neighbour velocities come from the sorted
-// post-predict snapshot vs (the Jacobi form of the racy read at :171).
+// CS_Integrate.glsl:135-280 in two kernels.  The gather: XSPH sum and curl
+// over the neighbours (CS_Integrate.glsl:160-205); neighbour velocities come
+// from the sorted post-predict snapshot vs (the Jacobi form of the racy read
+// at :171).  It writes the XSPH velocity change and the curl.
 template <int V, bool B>
-__global__ __launch_bounds__(kBlock) void k_integrate(StepConst c, const uint32_t* __restrict__ order, const float4* __restrict__ P,
-                                                      const float4* __restrict__ vs,
-                                                      const uint32_t* __restrict__ perm,
-                                                      float4* __restrict__ pos, float4* __restrict__ vel,
-                                                      const int2* __restrict__ cells, float4* __restrict__ curl_out,
-                                                      float4* __restrict__ posS) {
+__global__ __launch_bounds__(kBlock) void k_integrate(StepConst c, const uint32_t* __restrict__ order,
+                                                      const float4* __restrict__ P, const float4* __restrict__ vs,
+                                                      const int2* __restrict__ cells, float4* __restrict__ vdelta,
+                                                      float4* __restrict__ curl_out) {
+    const uint32_t t = xcd_gid();
+    if (t >= c.n) return;
+    const uint32_t k = order[t];
+    const V3 old_vel = xyz(vs[k]);
+    const V3 xp = xyz(P[k]);
+    const float inv = c.inv_rho0;
+    const bool do_visc = c.enable_visc == 1u;
+    const bool do_vort = c.enable_vort == 1u;
+    V3 visc = mk(0.0f, 0.0f, 0.0f);
+    V3 curl = mk(0.0f, 0.0f, 0.0f);
+    uint32_t ncand = 0, nnbr = 0;
+    __shared__ uint32_t lst[kLdsCap * kBlock];
+    gather<V, false>(xp, c.h, domain_of<B>(c, k), cells, P, k, &lst[threadIdx.x], [&](uint32_t j, float4, V3 r, float r2) {
+        const V3 vij = sub(xyz(vs[j]), old_vel);
+        if (do_visc) {
+            const float w = poly6_r2(c, r2);
+            visc = add(visc, scl(scl(vij, w), inv));
+        }
+        if (do_vort) {
+            const V3 gw = spiky(c, r, r2);
+            curl = add(curl, cross(vij, gw));
+        }
+    }, ncand, nnbr);
+    if (do_visc) {
+        const V3 vd = scl(visc, c.viscosity);
+        vdelta[k] = make_float4(vd.x, vd.y, vd.z, 0.0f);
+    }
+    if (do_vort) {
+        const V3 cw = scl(curl, inv);
+        curl_out[k] = make_float4(cw.x, cw.y, cw.z, 0.0f);
+    }
+}
+
+// The rest of CS_Integrate.glsl:206-280 per particle: AABB clamp, SDF
+// projection, velocity from the displacement plus the XSPH change, speed
+// clamp, wall damping, removal of the normal velocity at an SDF contact; the
+// committed state goes to pos / vel (id order) and posS (sorted).  A kernel
+// of its own so that the trilinear SDF samples do not raise the register
+// count of the gather.
+template <bool B>
+__global__ __launch_bounds__(kBlock) void k_integrate_finish(StepConst c, const uint32_t* __restrict__ order,
+                                                             const float4* __restrict__ P,
+                                                             const float4* __restrict__ vs,
+                                                             const float4* __restrict__ vdelta,
+                                                             const uint32_t* __restrict__ perm,
+                                                             float4* __restrict__ pos, float4* __restrict__ vel,
+                                                             float4* __restrict__ posS) {
     const uint32_t t = xcd_gid();
     if (t >= c.n) return;
     const uint32_t k = order[t];
     const Domain& dom = domain_of<B>(c, k);
-    
     const uint32_t id = perm[k];
     const float4 op4 = pos[id];
     const V3 old_pos = xyz(op4);
-    const float4 ov4 = vs[k];
-    const V3 old_vel = xyz(ov4);
+    const float vw = vs[k].w;
     V3 xp = xyz(P[k]);
-    const float inv = c.inv_rho0;
-    const bool do_visc = c.enable_visc == 1u;
-    const bool do_vort = c.enable_vort == 1u;
     V3 visc_delta = mk(0.0f, 0.0f, 0.0f);
-    if (do_visc || do_vort) {
-        V3 visc = mk(0.0f, 0.0f, 0.0f);
-        V3 curl = mk(0.0f, 0.0f, 0.0f);
-        uint32_t ncand = 0, nnbr = 0;
-        __shared__ uint32_t lst[kLdsCap * kBlock];
-        gather<V, false>(xp, c.h, dom, cells, P, k, &lst[threadIdx.x], [&](uint32_t j, float4, V3 r, float r2) {
-            const V3 vij = sub(xyz(vs[j]), old_vel);
-            if (do_visc) {
-                const float w = poly6_r2(c, r2);
-                visc = add(visc, scl(scl(vij, w), inv));
-            }
-            if (do_vort) {
-                const V3 gw = spiky(c, r, r2);
-                curl = add(curl, cross(vij, gw));
-            }
-        }, ncand, nnbr);
-        if (do_visc) visc_delta = scl(visc, c.viscosity);
-        if (do_vort) {
-            const V3 cw = scl(curl, inv);
-            curl_out[k] = make_float4(cw.x, cw.y, cw.z, 0.0f);
-        }
-    }
+    if (c.enable_visc == 1u) visc_delta = xyz(vdelta[k]);
     bool cxh = false, cyh = false, czh = false;
     if (xp.x <= dom.bmin[0]) { xp.x = dom.bmin[0]; cxh = true; }
     if (xp.x >= dom.bmax[0]) { xp.x = dom.bmax[0]; cxh = true; }
@@ -493,7 +514,7 @@ __global__ __launch_bounds__(kBlock) void k_integrate(StepConst c, const uint32_
         v = sub(v, scl(nrm, vn));
     }
     pos[id] = make_float4(xp.x, xp.y, xp.z, op4.w);
-    vel[id] = make_float4(v.x, v.y, v.z, ov4.w);
+    vel[id] = make_float4(v.x, v.y, v.z, vw);
     posS[k] = make_float4(xp.x, xp.y, xp.z, 0.0f);
 }
 
@@ -1043,10 +1064,17 @@ hipError_t launch_deltap(const StepConst& c, DeviceState& d, uint32_t iter, bool
 }
 
 hipError_t launch_integrate(const StepConst& c, DeviceState& d, hipStream_t st) {
-#define L(V, B)                                                                                                   \
-    hipLaunchKernelGGL((k_integrate<V, B>), grid_for(c.n), dim3(kBlock), 0, st, c, d.order, d.P, d.vs, d.perm, d.pos, \
-                       d.vel, d.cells, d.curl, d.posS);
-    FSI_GATHER_SWITCH(L)
+    if (c.enable_visc == 1u || c.enable_vort == 1u) {
+#define L(V, B)                                                                                                     \
+    hipLaunchKernelGGL((k_integrate<V, B>), grid_for(c.n), dim3(kBlock), 0, st, c, d.order, d.P, d.vs, d.cells, \
+                       d.vdelta, d.curl);
+        FSI_GATHER_SWITCH(L)
+#undef L
+    }
+#define L(B)                                                                                                   \
+    hipLaunchKernelGGL((k_integrate_finish<B>), grid_for(c.n), dim3(kBlock), 0, st, c, d.order, d.P, d.vs, d.vdelta, \
+                       d.perm, d.pos, d.vel, d.posS);
+    FSI_B_SWITCH(d.batched, L)
 #undef L
     return hipGetLastError();
 }

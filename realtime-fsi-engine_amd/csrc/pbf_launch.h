@@ -12,9 +12,11 @@ namespace fsi {
 
 // Keys per radix tile = 256 threads x kSortIPT.
 constexpr int kSortIPT = 8;
-// Neighbour-list capacity: rows of the lambda -> delta-p list in HBM, and the
-// per-lane LDS list of the other gather passes.
-constexpr int kNbrCap = 32;
+// Neighbour-list capacity: rows of the lambda -> delta-p list in HBM.  A
+// particle of a fluid at rest density has about 32 neighbours, more under
+// compression and where two of its 27 cells hash to one bucket (visited
+// twice); a longer row makes delta-p take the bucket walk instead.
+constexpr int kNbrCap = 64;
 // Widest radix digit the solver plans with (LDS: 44 KiB per workgroup at 10).
 constexpr int kSortMaxDigitBits = 10;
 
@@ -52,6 +54,7 @@ struct DeviceState {
     float4* vs = nullptr;      // sorted: post-predict velocity (v*, pad2)
     float4* curl = nullptr;    // sorted: curl / rho0 of the last Integrate
     float4* posS = nullptr;    // sorted: committed positions of the last substep
+    float4* vdelta = nullptr;  // sorted: XSPH velocity change of the last Integrate
     float* rho = nullptr;      // sorted: density of the last lambda pass
     uint32_t* lod = nullptr;   // sorted: APBF level of detail
     float4* dp_tap = nullptr;  // sorted: last delta-p (taps only)

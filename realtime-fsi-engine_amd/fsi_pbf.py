@@ -203,14 +203,26 @@ def load() -> C.CDLL:
         "fsi_batch_read_timing": ([P, C.POINTER(CTiming)], C.c_int),
         "fsi_adj_create": ([C.POINTER(CFluidConfig), C.c_uint32, C.c_uint32, C.POINTER(P)], C.c_int),
         "fsi_adj_destroy": ([P], C.c_int),
+        "fsi_adj_set_precision": ([P, C.c_uint32], C.c_int),
         "fsi_adj_forward": ([P, P, P, C.c_float, C.c_float, P, P, P], C.c_int),
         "fsi_adj_backward": ([P, P, P, P, P, f64p, P], C.c_int),
+        "fsi_adj_forward64": ([P, P, P, C.c_double, C.c_double, P, P, P], C.c_int),
+        "fsi_adj_backward64": ([P, P, P, P, P, f64p, P], C.c_int),
         "fsi_adj_stats": ([P, u32p], C.c_int),
         "fsi_adj_frame_norms": ([P, f32p], C.c_int),
         "fsi_scene_lattice": ([C.c_uint32, C.c_uint32, C.c_uint32, f32p, C.c_float, C.c_float, C.c_uint32, f32p, P],
                               C.c_int),
         "fsi_mesh_icosphere": ([C.c_uint32, f64p, C.c_double, f64p, i32p], C.c_int),
         "fsi_sdf_build": ([f64p, C.c_uint32, i32p, C.c_uint32, C.c_double, f64p, i32p, f32p], C.c_int),
+        "fsi_mesh_create": ([f64p, C.c_uint32, i32p, C.c_uint32, C.c_double, C.POINTER(P)], C.c_int),
+        "fsi_mesh_destroy": ([P], C.c_int),
+        "fsi_mesh_grid_info": ([P, f64p, f64p, i32p], C.c_int),
+        "fsi_mesh_read_sdf": ([P, f32p], C.c_int),
+        "fsi_mesh_refit": ([P, f64p], C.c_int),
+        "fsi_mesh_refit_dirty": ([P, f64p, C.c_double, C.c_double, i32p, i32p, u32p], C.c_int),
+        "fsi_mesh_read_forces": ([P, f32p], C.c_int),
+        "fsi_pbf_set_sdf_from_mesh": ([P, P], C.c_int),
+        "fsi_pbf_contact_forces": ([P, P, C.c_float, C.c_int, C.c_float, f32p, u32p], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -280,6 +292,74 @@ def sdf_build(verts: np.ndarray, faces: np.ndarray, cell: float) -> SdfGrid:
     grid = np.zeros((int(dims[2]), int(dims[1]), int(dims[0])), dtype=np.float32)
     _check(lib.fsi_sdf_build(*args, _ptr(grid, C.c_float)))
     return SdfGrid(grid=grid, origin=origin, cell=float(cell), dims=(int(dims[0]), int(dims[1]), int(dims[2])))
+
+
+class Mesh:
+    """The obstacle mesh on the device (SDFBoundary with its grid in HBM):
+    signed-distance grid, refit after a deformation, contact forces."""
+
+    def __init__(self, verts: np.ndarray, faces: np.ndarray, cell: float):
+        self.lib = load()
+        self.verts = np.ascontiguousarray(verts, dtype=np.float64).reshape(-1, 3)
+        self.faces = np.ascontiguousarray(faces, dtype=np.int32).reshape(-1, 3)
+        h = C.c_void_p()
+        _check(self.lib.fsi_mesh_create(_ptr(self.verts, C.c_double), len(self.verts), _ptr(self.faces, C.c_int32),
+                                        len(self.faces), float(cell), C.byref(h)))
+        self.h = h
+        origin = np.zeros(3, dtype=np.float64)
+        c = np.zeros(1, dtype=np.float64)
+        dims = np.zeros(3, dtype=np.int32)
+        _check(self.lib.fsi_mesh_grid_info(h, _ptr(origin, C.c_double), _ptr(c, C.c_double), _ptr(dims, C.c_int32)))
+        self.origin = origin
+        self.cell = float(c[0])
+        self.dims = (int(dims[0]), int(dims[1]), int(dims[2]))
+
+    def read_sdf(self) -> SdfGrid:
+        grid = np.zeros((self.dims[2], self.dims[1], self.dims[0]), dtype=np.float32)
+        _check(self.lib.fsi_mesh_read_sdf(self.h, _ptr(grid, C.c_float)))
+        return SdfGrid(grid=grid, origin=self.origin.copy(), cell=self.cell, dims=self.dims)
+
+    def refit(self, verts: np.ndarray):
+        v = np.ascontiguousarray(verts, dtype=np.float64).reshape(-1, 3)
+        if v.shape != self.verts.shape:
+            raise ValueError("refit keeps the vertex count")
+        _check(self.lib.fsi_mesh_refit(self.h, _ptr(v, C.c_double)))
+        self.verts = v.copy()
+
+    def refit_dirty(self, verts: np.ndarray, band_radius: float, move_eps: float):
+        """Returns (recomputed voxels, box min xyz, box max xyz)."""
+        v = np.ascontiguousarray(verts, dtype=np.float64).reshape(-1, 3)
+        if v.shape != self.verts.shape:
+            raise ValueError("refit keeps the vertex count")
+        lo = np.zeros(3, dtype=np.int32)
+        hi = np.zeros(3, dtype=np.int32)
+        n = C.c_uint32(0)
+        _check(self.lib.fsi_mesh_refit_dirty(self.h, _ptr(v, C.c_double), float(band_radius), float(move_eps),
+                                             _ptr(lo, C.c_int32), _ptr(hi, C.c_int32), C.byref(n)))
+        self.verts = v.copy()
+        return n.value, lo, hi
+
+    def read_forces(self) -> np.ndarray:
+        out = np.zeros((len(self.verts), 3), dtype=np.float32)
+        _check(self.lib.fsi_mesh_read_forces(self.h, _ptr(out, C.c_float)))
+        return out
+
+    def close(self):
+        if self.h:
+            self.lib.fsi_mesh_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # ---------------------------------------------------------------- solver
@@ -366,6 +446,21 @@ class Solver:
 
     def filter_contacts(self, band: float):
         _check(self.lib.fsi_pbf_filter_contacts(self.h, float(np.float32(band))))
+
+    def set_sdf_from_mesh(self, mesh: "Mesh", padding: float = 0.0, containment: bool = False):
+        _check(self.lib.fsi_pbf_set_sdf_from_mesh(self.h, mesh.h))
+        _check(self.lib.fsi_pbf_set_sdf_padding(self.h, float(np.float32(padding))))
+        _check(self.lib.fsi_pbf_set_sdf_containment(self.h, 1 if containment else 0))
+
+    def contact_forces(self, mesh: "Mesh", band: float, use_vel_magnitude: bool = True,
+                       control_threshold: float = 0.0):
+        """computeContactForces on the device: ((nv, 3) forces, (contacts, control points))."""
+        out = np.zeros((len(mesh.verts), 3), dtype=np.float32)
+        counts = np.zeros(2, dtype=np.uint32)
+        _check(self.lib.fsi_pbf_contact_forces(self.h, mesh.h, float(np.float32(band)), 1 if use_vel_magnitude else 0,
+                                               float(np.float32(control_threshold)), _ptr(out, C.c_float),
+                                               _ptr(counts, C.c_uint32)))
+        return out, (int(counts[0]), int(counts[1]))
 
     def read_contacts(self) -> np.ndarray:
         count = C.c_uint32(0)

@@ -30,9 +30,10 @@ def _stream(device):
 
 class Frames:
     """`frames` frames of PBFluids::step for n particles under cfg (viscosity
-    and eps are inputs of each run)."""
+    and eps are inputs of each run).  precision 32 differentiates the solver's
+    FP32 trajectory, 64 an FP64 run of the same passes."""
 
-    def __init__(self, cfg: fsi_pbf.FluidConfig, n: int, frames: int):
+    def __init__(self, cfg: fsi_pbf.FluidConfig, n: int, frames: int, precision: int = 32):
         self.lib = fsi_pbf.load()
         self.cfg = cfg
         self.n = int(n)
@@ -42,6 +43,15 @@ class Frames:
         fsi_pbf._check(self.lib.fsi_adj_create(C.byref(self._c), self.n, self.frames, C.byref(h)))
         self.h = h
         self.generation = 0  # bumped by every forward: a backward needs its own forward's checkpoints
+        self.precision = 32
+        if precision != 32:
+            self.set_precision(precision)
+
+    def set_precision(self, bits: int):
+        """Trajectory precision, 32 or 64; drops the last forward's checkpoints."""
+        fsi_pbf._check(self.lib.fsi_adj_set_precision(self.h, int(bits)))
+        self.precision = int(bits)
+        self.generation += 1
 
     def stats(self) -> dict:
         """Neighbour statistics and the replay check of the last backward."""
@@ -75,18 +85,23 @@ class _Run(torch.autograd.Function):
     def forward(ctx, frames: Frames, pos0, vel0, viscosity, eps):
         pos0 = pos0.detach().contiguous()
         vel0 = vel0.detach().contiguous()
-        if pos0.shape != (frames.n, 4) or vel0.shape != (frames.n, 4) or pos0.dtype != torch.float32:
-            raise ValueError("pos0 / vel0 must be float32 tensors of shape (n, 4)")
+        if pos0.shape != (frames.n, 4) or vel0.shape != (frames.n, 4):
+            raise ValueError("pos0 / vel0 must have shape (n, 4)")
+        if pos0.dtype not in (torch.float32, torch.float64) or vel0.dtype != pos0.dtype:
+            raise ValueError("pos0 / vel0 must both be float32, or both float64")
         if not pos0.is_cuda or not vel0.is_cuda:
             raise ValueError("pos0 / vel0 must be on the GPU")
+        f64 = pos0.dtype == torch.float64
         pos = torch.empty_like(pos0)
         vel = torch.empty_like(vel0)
-        fsi_pbf._check(frames.lib.fsi_adj_forward(frames.h, _ptr(pos0), _ptr(vel0), float(viscosity), float(eps),
-                                                  _ptr(pos), _ptr(vel), _stream(pos0.device)))
+        fwd = frames.lib.fsi_adj_forward64 if f64 else frames.lib.fsi_adj_forward
+        fsi_pbf._check(fwd(frames.h, _ptr(pos0), _ptr(vel0), float(viscosity), float(eps), _ptr(pos), _ptr(vel),
+                           _stream(pos0.device)))
         frames.generation += 1
         ctx.frames = frames
         ctx.generation = frames.generation
         ctx.param_like = (viscosity, eps)
+        ctx.f64 = f64
         return pos, vel
 
     @staticmethod
@@ -94,14 +109,15 @@ class _Run(torch.autograd.Function):
         frames = ctx.frames
         if frames.generation != ctx.generation:
             raise RuntimeError("the Frames handle ran another forward since this graph was built")
-        gpos = gpos.contiguous() if gpos is not None else None
-        gvel = gvel.contiguous() if gvel is not None else None
+        dt = torch.float64 if ctx.f64 else torch.float32
+        gpos = gpos.to(dt).contiguous() if gpos is not None else None
+        gvel = gvel.to(dt).contiguous() if gvel is not None else None
         like = gpos if gpos is not None else gvel
-        gp0 = torch.empty((frames.n, 4), dtype=torch.float32, device=like.device)
+        gp0 = torch.empty((frames.n, 4), dtype=dt, device=like.device)
         gv0 = torch.empty_like(gp0)
         params = (C.c_double * 2)()
-        fsi_pbf._check(frames.lib.fsi_adj_backward(frames.h, _ptr(gpos), _ptr(gvel), _ptr(gp0), _ptr(gv0), params,
-                                                   _stream(like.device)))
+        bwd = frames.lib.fsi_adj_backward64 if ctx.f64 else frames.lib.fsi_adj_backward
+        fsi_pbf._check(bwd(frames.h, _ptr(gpos), _ptr(gvel), _ptr(gp0), _ptr(gv0), params, _stream(like.device)))
         out = []
         for val, ref in zip(params, ctx.param_like):
             out.append(torch.as_tensor(val, dtype=ref.dtype, device=ref.device) if torch.is_tensor(ref) else None)
@@ -111,7 +127,8 @@ class _Run(torch.autograd.Function):
 def simulate(frames: Frames, pos0, vel0, viscosity, eps):
     """Final (pos, vel), each (n, 3), after the frames from (pos0, vel0) (n, 3).
 
-    viscosity and eps are 0-d tensors (differentiable) or floats."""
+    viscosity and eps are 0-d tensors (differentiable) or floats.  float64
+    states (FP64 trajectory only) keep the whole chain in FP64."""
     pad = torch.zeros((pos0.shape[0], 1), dtype=pos0.dtype, device=pos0.device)
     p4 = torch.cat([pos0, pad], dim=1)
     v4 = torch.cat([vel0, pad], dim=1)
