@@ -59,6 +59,10 @@ N_SIDE = (128, 64, 128)
 ITERS, SUBSTEPS, HASH = 6, 2, 1 << 20
 
 
+def progress(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def make_config(fsi):
     return fsi.FluidConfig(dt=1.0 / 120.0, gravity=(0.0, -9.81, 0.0), h=0.1, rho0=6378.0, solver_iterations=ITERS,
                            substep_iterations=SUBSTEPS, eps=600.0, bounds_min=(0.0, 0.0, 0.0),
@@ -245,13 +249,35 @@ def adjoint_leg(fsi, reps=3):
         tf.append(t1 - t0)
         tb.append(t2 - t1)
     st = fh.stats()
+    # the same frames as an FP64 trajectory (states and adjoints in FP64)
+    fh.set_precision(64)
+    p64, v64 = pos0.double(), vel0.double()
+    po64, vo64 = torch.empty_like(p64), torch.empty_like(v64)
+    g64, gp64, gv64 = gpos.double(), torch.empty_like(p64), torch.empty_like(p64)
+    t64f, t64b = [], []
+    for _ in range(reps + 1):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fsi._check(lib.fsi_adj_forward64(h, ptr(p64), ptr(v64), 0.01, 600.0, ptr(po64), ptr(vo64), stream))
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        fsi._check(lib.fsi_adj_backward64(h, ptr(g64), None, ptr(gp64), ptr(gv64), params, stream))
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        t64f.append(t1 - t0)
+        t64b.append(t2 - t1)
     fh.close()
     f_ms = float(np.median(tf)) * 1e3
     b_ms = float(np.median(tb)) * 1e3
+    f64_ms = float(np.median(t64f[1:])) * 1e3
+    b64_ms = float(np.median(t64b[1:])) * 1e3
     return {"particles": n, "frames": frames, "iterations": 4, "substeps": 2, "forward_ms": round(f_ms, 3),
             "backward_ms": round(b_ms, 3), "adjoint_forward_ratio": round(b_ms / f_ms, 3), "reps": reps,
             "loss": float(loss.item()), "grad_viscosity": params[0], "grad_eps": params[1],
-            "replay_mismatches": st["replay_mismatches"], "max_neighbours": st["max_neighbours"]}
+            "replay_mismatches": st["replay_mismatches"], "max_neighbours": st["max_neighbours"],
+            "fp64_trajectory": {"forward_ms": round(f64_ms, 3), "backward_ms": round(b64_ms, 3),
+                                "backward_over_fp32_forward": round(b64_ms / f_ms, 3),
+                                "backward_over_fp64_forward": round(b64_ms / f64_ms, 3)}}
 
 
 def plate_mesh(nx=128, nz=32, size=(4.0, 1.0), origin=(1.2, 1.0, 2.7)):
@@ -461,14 +487,19 @@ def main():
         "passes": passes,
         "passes_note": "separate run of min(steps, 5) frames with events around every pass",
     }
+    progress(f"plain leg: {out['ms_per_step']:.3f} ms/frame")
     if not args.no_batched:
         out["batched"] = batched_leg(fsi, args.batch_steps, 1)
+        progress(f"batched leg: speed-up {out['batched']['speedup']:.2f}")
     if not args.no_adjoint:
         out["adjoint"] = adjoint_leg(fsi)
+        progress(f"adjoint leg: ratio {out['adjoint']['adjoint_forward_ratio']:.2f}")
     if not args.no_fsi:
         out["fsi"] = fsi_leg(fsi, args.fsi_steps)
+        progress(f"fsi leg: {out['fsi']['ms_per_frame']:.3f} ms/frame")
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(fsi, cfg, parts, sdf)
+        progress("cpu baseline done")
     print(json.dumps(out))
 
 

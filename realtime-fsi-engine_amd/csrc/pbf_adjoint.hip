@@ -101,14 +101,35 @@ __global__ __launch_bounds__(kAB) void k_transpose(uint32_t n, const uint32_t* _
     }
 }
 
-// Ascending pair order inside every transposed row (insertion sort).
+// Ascending pair order inside every transposed row: an insertion sort of the
+// row staged in LDS (one column per thread), written back in order.  Rows
+// longer than the stage are sorted in place in global memory.
+constexpr uint32_t kSortStage = 48;
 __global__ __launch_bounds__(kAB) void k_tsort(uint32_t n, uint32_t* __restrict__ tin, uint32_t* __restrict__ tcnt,
                                                uint32_t* __restrict__ stats) {
+    __shared__ uint32_t stage[kSortStage * kAB];
     const uint32_t j = agid();
     if (j >= n) return;
-    atomicMax(stats + 3, tcnt[j]);
-    const uint32_t m = min(tcnt[j], kInCap);
+    const uint32_t full = tcnt[j];
+    atomicMax(stats + 3, full);
+    const uint32_t m = min(full, kInCap);
     tcnt[j] = m;
+    if (m <= kSortStage) {
+        uint32_t* row = &stage[threadIdx.x];
+        for (uint32_t a = 0; a < m; ++a) {
+            const uint32_t v = tin[(size_t)a * n + j];
+            uint32_t b = a;
+            while (b > 0) {
+                const uint32_t u = row[(b - 1) * kAB];
+                if (u <= v) break;
+                row[b * kAB] = u;
+                --b;
+            }
+            row[b * kAB] = v;
+        }
+        for (uint32_t a = 0; a < m; ++a) tin[(size_t)a * n + j] = row[a * kAB];
+        return;
+    }
     for (uint32_t a = 1; a < m; ++a) {
         const uint32_t v = tin[(size_t)a * n + j];
         uint32_t b = a;

@@ -42,8 +42,8 @@ inline dim3 grid_for(uint32_t n) { return dim3((n + kBlock - 1) / kBlock); }
 
 // Per-lane LDS list of the two-phase form (entries), and the fill level at
 // which the whole wave flushes.
-constexpr uint32_t kLdsCap = 16;
-constexpr uint32_t kFlushAt = 8;
+constexpr uint32_t kLdsCap = 24;
+constexpr uint32_t kFlushAt = 16;
 
 // Phase 1 of the two-phase gather, with the phase-2 work interleaved in
 // wave-uniform flushes.  The 27 buckets are scanned as in scan_buckets; the

@@ -2,6 +2,8 @@
 
   python rocprof_summary.py kernels <trace_dir> --cmd "<command>"
       per-kernel calls / average / share from <trace_dir>/run_kernel_stats.csv
+  python rocprof_summary.py counters <pmc_dir> --cmd "<command>"
+      per-kernel average of every counter of one --pmc run (+ VALU per wave)
   python rocprof_summary.py traffic <fetch_dir> <write_dir> --calib-fetch <dir> --calib-write <dir>
                             --calib-bytes B --cmd "<command>" [--json out.json]
       HBM bytes per dispatch of each kernel: FETCH_SIZE and WRITE_SIZE (KiB per
@@ -95,6 +97,27 @@ def traffic(args):
             json.dump(out, f, indent=1)
 
 
+def counters(args):
+    """Per-kernel averages of every counter of one --pmc run, and VALU
+    instructions per wave when SQ_WAVES and SQ_INSTS_VALU are among them."""
+    acc = collections.defaultdict(lambda: collections.defaultdict(list))
+    for r in csv.DictReader(open(os.path.join(args.dir, "run_counter_collection.csv"))):
+        acc[short_name(r["Kernel_Name"])][r["Counter_Name"]].append(float(r["Counter_Value"]))
+    names = sorted({c for k in acc.values() for c in k})
+    print(f"# Hardware counters per dispatch\n\nCommand:\n\n    {args.cmd}\n")
+    extra = "SQ_WAVES" in names and "SQ_INSTS_VALU" in names
+    print("| kernel | dispatches | " + " | ".join(names) + (" | VALU inst / wave |" if extra else " |"))
+    print("|---|---|" + "---|" * len(names) + ("---|" if extra else ""))
+    rows = sorted(acc.items(), key=lambda kv: -sum(kv[1].get("SQ_INSTS_VALU", [0.0])))
+    for k, cs in rows:
+        means = {c: sum(v) / len(v) for c, v in cs.items()}
+        line = f"| `{k}` | {len(next(iter(cs.values())))} | " + " | ".join(f"{means.get(c, 0.0):.4g}" for c in names)
+        if extra:
+            w = means.get("SQ_WAVES", 0.0)
+            line += f" | {means.get('SQ_INSTS_VALU', 0.0) / w:.0f}" if w else " | -"
+        print(line + " |")
+
+
 def main():
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="what", required=True)
@@ -110,8 +133,11 @@ def main():
     t.add_argument("--calib-kernel", default="elementwise|Functor|copy")
     t.add_argument("--cmd", required=True)
     t.add_argument("--json")
+    c = sub.add_parser("counters")
+    c.add_argument("dir")
+    c.add_argument("--cmd", required=True)
     a = ap.parse_args()
-    kernels(a) if a.what == "kernels" else traffic(a)
+    {"kernels": kernels, "traffic": traffic, "counters": counters}[a.what](a)
 
 
 if __name__ == "__main__":
