@@ -15,9 +15,10 @@
 // forward saw.  The reverse kernels then run the passes backwards.  A pass
 // that sums over the pairs (i, j) of slot i has a partial derivative for slot
 // j too; each reverse pass writes those per pair, and a second kernel sums
-// them into slot j through the transposed list (the pairs that reach j, in
-// ascending pair order), so every sum has a fixed order and the gradient is
-// reproducible bit for bit.
+// them into slot j through the transposed list (the pairs that reach j).  The
+// rows of that list fill in no fixed order, so their sums are formed in 64-bit
+// integer units of the row's largest term (RowSum), which do not depend on
+// the order: the gradient is reproducible bit for bit.
 //
 // Scope: the substep of PBFluids::step (src/fluid/PBFluids.cpp:323-517) with
 // s_corr, cohesion floor, XSPH viscosity, curl, AABB clamp with damping,
@@ -53,6 +54,121 @@ constexpr uint32_t kInCap = 128;
 __device__ __forceinline__ uint32_t agid() { return blockIdx.x * kAB + threadIdx.x; }
 inline dim3 agrid(uint32_t n) { return dim3((n + kAB - 1) / kAB); }
 
+// Quads: at 65,536 particles one thread per slot is one wave per SIMD, and
+// the FP64 reverse passes stall on their loads and long dependency chains.
+// The reverse kernels give each slot four adjacent lanes (lane = thread & 3)
+// that take every fourth pair; per-slot sums are reduced across the quad.
+constexpr uint32_t kQ = 4;
+inline dim3 qgrid(uint32_t n) { return dim3((uint32_t)(((size_t)n * kQ + kAB - 1) / kAB)); }
+__device__ __forceinline__ double qsum(double v) {
+    v += __shfl_xor(v, 1, (int)kQ);
+    v += __shfl_xor(v, 2, (int)kQ);
+    return v;
+}
+
+// Order-independent sums over a transposed row.  The atomic appends of
+// k_transpose fill a row in whatever order they run, so its terms are added
+// in a way that does not depend on the order: every term is rounded to a
+// multiple of 2^(E - 55), E the binary exponent of the row's largest |term|,
+// and the multiples are added as 64-bit integers (exact, associative).  A
+// row holds at most kInCap = 128 terms of at most 2^55 units: no overflow.
+// The rounding error, 2^-56 of the largest term per term, is below that of a
+// sequential FP64 sum.  The same row always gives the same bits.
+struct RowSum {
+    int e = 0;  // exponent of the row's largest |term| (0 when all are zero)
+    __device__ __forceinline__ void set(double maxabs) {
+        e = 0;
+        if (maxabs > 0.0) (void)frexp(maxabs, &e);
+    }
+    __device__ __forceinline__ long long units(double v) const { return (long long)rint(ldexp(v, 55 - e)); }
+    __device__ __forceinline__ double value(long long u) const { return ldexp((double)u, e - 55); }
+};
+
+// Lanes per transposed row in the kernels that sum rows, and the terms each
+// lane keeps in registers between the two passes of row_sum (rows of up to
+// kR * kRKeep terms are read once).
+constexpr uint32_t kR = 8, kRKeep = 4;
+inline dim3 rgrid(uint32_t n) { return dim3((uint32_t)(((size_t)n * kR + kAB - 1) / kAB)); }
+__device__ __forceinline__ double rmax(double v) {
+#pragma unroll
+    for (int o = 1; o < (int)kR; o <<= 1) v = fmax(v, __shfl_xor(v, o, (int)kR));
+    return v;
+}
+__device__ __forceinline__ long long rsum_i64(long long v) {
+#pragma unroll
+    for (int o = 1; o < (int)kR; o <<= 1) v += __shfl_xor(v, o, (int)kR);
+    return v;
+}
+__device__ __forceinline__ int ror(int v) {
+#pragma unroll
+    for (int o = 1; o < (int)kR; o <<= 1) v |= __shfl_xor(v, o, (int)kR);
+    return v;
+}
+
+// The C order-independent sums of transposed row j (m terms) over the kR
+// lanes of its group: get(e, v) gives the C terms of pair e.  Lane l takes
+// the row's entries l, l + kR, ...; the first kRKeep of them stay in
+// registers for the second pass, later ones are read again.  A non-finite
+// term makes the sums NaN.
+template <int C, class Get>
+__device__ __forceinline__ void row_sum(const uint32_t* __restrict__ tin, uint32_t n, uint32_t j, uint32_t m,
+                                        uint32_t lane, Get&& get, double out[C]) {
+    double keep[kRKeep][C];
+    double mx[C];
+    int bad = 0;
+#pragma unroll
+    for (int c = 0; c < C; ++c) mx[c] = 0.0;
+    auto scan = [&](const double* v) {
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            const double a = fabs(v[c]);
+            bad |= (a <= 1.7976931348623157e308) ? 0 : 1;
+            mx[c] = fmax(mx[c], a);
+        }
+    };
+#pragma unroll
+    for (uint32_t t = 0; t < kRKeep; ++t) {
+        const uint32_t q = lane + t * kR;
+        if (q < m) {
+            get(tin[(size_t)q * n + j], keep[t]);
+        } else {
+#pragma unroll
+            for (int c = 0; c < C; ++c) keep[t][c] = 0.0;
+        }
+        scan(keep[t]);
+    }
+    for (uint32_t q = lane + kRKeep * kR; q < m; q += kR) {
+        double v[C];
+        get(tin[(size_t)q * n + j], v);
+        scan(v);
+    }
+    bad = ror(bad);
+    RowSum rs[C];
+    long long u[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        rs[c].set(rmax(mx[c]));
+        u[c] = 0;
+    }
+    if (!bad) {
+#pragma unroll
+        for (uint32_t t = 0; t < kRKeep; ++t)
+#pragma unroll
+            for (int c = 0; c < C; ++c) u[c] += rs[c].units(keep[t][c]);
+        for (uint32_t q = lane + kRKeep * kR; q < m; q += kR) {
+            double v[C];
+            get(tin[(size_t)q * n + j], v);
+#pragma unroll
+            for (int c = 0; c < C; ++c) u[c] += rs[c].units(v[c]);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        u[c] = rsum_i64(u[c]);
+        out[c] = bad ? __longlong_as_double(0x7ff8000000000000ll) : rs[c].value(u[c]);
+    }
+}
+
 __device__ __forceinline__ V3 zero3() { return mk(0.0f, 0.0f, 0.0f); }
 __device__ __forceinline__ float4 f4(V3 v, float w = 0.0f) { return make_float4(v.x, v.y, v.z, w); }
 
@@ -61,86 +177,136 @@ __global__ __launch_bounds__(kAB) void k_zero_u32(uint32_t* __restrict__ p, size
     if (i < n) p[i] = 0u;
 }
 
+// Statistics kept as maxima over kSpread slots: block b folds its maximum
+// into slot b mod kSpread and the reader takes the maximum over the slots.
+// Atomics from many waves on one address queue at the cache; spread over
+// kSpread addresses, with one atomic per block, they do not.
+constexpr uint32_t kSpread = 64;
+// flag words: 0 overflow bits, 1 replay mismatches, then the slots of the
+// largest out-degree and of the largest in-degree of the lists
+constexpr uint32_t kOutMax = 2, kInMax = 2 + kSpread, kFlagWords = 2 + 2 * kSpread;
+
+// The block's maxima of v[0..C) folded into slots[c] + (block mod kSpread).
+// Every thread of the block calls it (it holds a barrier).  Values are
+// compared as unsigned integers (the bits of a non-negative float order the
+// same way).
+template <int C>
+__device__ __forceinline__ void block_max(const uint32_t (&v)[C], uint32_t* const (&slots)[C]) {
+    __shared__ uint32_t part[C][kAB / 64];
+    uint32_t m[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        m[c] = v[c];
+        for (int o = 32; o > 0; o >>= 1) m[c] = max(m[c], (uint32_t)__shfl_xor((int)m[c], o));
+        if (__lane_id() == 0u) part[c][threadIdx.x / 64u] = m[c];
+    }
+    __syncthreads();
+    if (threadIdx.x < (uint32_t)C) {
+        uint32_t b = 0;
+        for (uint32_t w = 0; w < kAB / 64; ++w) b = max(b, part[threadIdx.x][w]);
+        if (b > 0u) atomicMax(slots[threadIdx.x] + blockIdx.x % kSpread, b);
+    }
+}
+
 // ---- neighbour lists ------------------------------------------------------
 // Pair (i, r) -- the r-th neighbour of slot i -- is stored at r * n + i; its
 // number in the transposed list is that same index.
 
+// The neighbour list of slot i, built by the four lanes of a quad in the
+// reference's visit order (CS_ComputeLambdas.glsl:126-150): the lanes walk the
+// same 27 buckets and test every fourth candidate (lane l takes the bucket's
+// slots first + l, first + 4 + l, ...), and a ballot over the quad gives each
+// neighbour its rank in visit order.  Four lanes per slot keep four times as
+// many waves in flight as one lane per slot, and the candidate loads of a
+// quad are contiguous.  inside(xk) is the support test r^2 <= h^2.  Returns
+// the slot's neighbour count (entries past kEdgeCap are not stored).
+template <class T4, class Inside>
+__device__ __forceinline__ uint32_t quad_list(const T4* __restrict__ X, const int2* __restrict__ cells,
+                                              const BucketWalk& bw, uint32_t n, uint32_t i, uint32_t lane,
+                                              uint32_t* __restrict__ nbr, Inside&& inside) {
+    const uint32_t qbase = __lane_id() & ~(kQ - 1u), below = (1u << lane) - 1u;
+    uint32_t m = 0;
+    auto place = [&](bool in, uint32_t s) {
+        const uint32_t qm = (uint32_t)(__ballot(in) >> qbase) & 0xfu;
+        if (in) {
+            const uint32_t r = m + (uint32_t)__popc(qm & below);
+            if (r < kEdgeCap) nbr[(size_t)r * n + i] = s;
+        }
+        m += (uint32_t)__popc(qm);
+    };
+    int2 se = cells[bw.bucket(0)];
+#pragma unroll 1
+    for (int q = 0; q < 27; ++q) {
+        int2 nxt = make_int2(-1, -1);
+        if (q < 26) nxt = cells[bw.bucket(q + 1)];
+        // an empty bucket is (-1, -1): no iterations
+        const uint32_t end = (uint32_t)se.y, last = end - 1u;
+        for (uint32_t b = (uint32_t)se.x; b < end; b += 2u * kQ) {
+            const uint32_t sa = b + lane, sb = b + kQ + lane;
+            const T4 xa = X[min(sa, last)], xb = X[min(sb, last)];
+            place(sa <= last && sa != i && inside(xa), sa);
+            place(sb <= last && sb != i && inside(xb), sb);
+        }
+        se = nxt;
+    }
+    return m;
+}
+
+// Stores the count of a quad-built list (lane 0), flags a list longer than
+// kEdgeCap and keeps the largest count.  Every thread of the block calls it;
+// m is 0 for lanes past the last slot.
+__device__ __forceinline__ void store_count(uint32_t m, bool live, uint32_t i, uint32_t lane,
+                                            uint32_t* __restrict__ cnt, uint32_t* __restrict__ overflow) {
+    block_max<1>({m}, {overflow + kOutMax});
+    if (live && lane == 0u) {
+        if (m > kEdgeCap) {
+            atomicOr(overflow, 1u);
+            m = kEdgeCap;
+        }
+        cnt[i] = m;
+    }
+}
+
 __global__ __launch_bounds__(kAB) void k_edges(StepConst c, const float4* __restrict__ X,
                                                const int2* __restrict__ cells, uint32_t* __restrict__ nbr,
                                                uint32_t* __restrict__ cnt, uint32_t* __restrict__ overflow) {
-    const uint32_t i = agid();
-    if (i >= c.n) return;
-    const float h2 = c.h * c.h;
+    const uint32_t g = agid(), i = g / kQ, lane = g % kQ;
+    const bool live = i < c.n;
     uint32_t m = 0;
-    scan_buckets(xyz(X[i]), c.h, c.dom, cells, X, i, [&](uint32_t k, float4, V3, float r2) {
-        if (r2 <= h2) {
-            if (m < kEdgeCap) nbr[(size_t)m * c.n + i] = k;
-            ++m;
-        }
-    });
-    atomicMax(overflow + 2, m);
-    if (m > kEdgeCap) {
-        atomicOr(overflow, 1u);
-        m = kEdgeCap;
+    if (live) {
+        const V3 x = xyz(X[i]);
+        const float h2 = c.h * c.h;
+        int32_t cx, cy, cz;
+        cell_of(x, c.h, cx, cy, cz);
+        m = quad_list(X, cells, BucketWalk(cx, cy, cz, c.dom), c.n, i, lane, nbr, [&](const float4& xk) {
+            const V3 r = sub(x, xyz(xk));
+            return dot(r, r) <= h2;
+        });
     }
-    cnt[i] = m;
+    store_count(m, live, i, lane, cnt, overflow);
 }
 
 __global__ __launch_bounds__(kAB) void k_transpose(uint32_t n, const uint32_t* __restrict__ nbr,
                                                    const uint32_t* __restrict__ cnt, uint32_t* __restrict__ tin,
                                                    uint32_t* __restrict__ tcnt, uint32_t* __restrict__ overflow) {
-    const uint32_t i = agid();
+    const uint32_t g = agid(), i = g / kQ, lane = g % kQ;
     if (i >= n) return;
     const uint32_t m = cnt[i];
-    for (uint32_t r = 0; r < m; ++r) {
+    for (uint32_t r = lane; r < m; r += kQ) {
         const size_t e = (size_t)r * n + i;
         const uint32_t j = nbr[e];
         const uint32_t q = atomicAdd(&tcnt[j], 1u);
         if (q < kInCap) tin[(size_t)q * n + j] = (uint32_t)e;
-        else atomicOr(overflow, 2u);
+        else atomicOr(overflow, 2u);  // rare: the list is refused
     }
 }
 
-// Ascending pair order inside every transposed row: an insertion sort of the
-// row staged in LDS (one column per thread), written back in order.  Rows
-// longer than the stage are sorted in place in global memory.
-constexpr uint32_t kSortStage = 48;
-__global__ __launch_bounds__(kAB) void k_tsort(uint32_t n, uint32_t* __restrict__ tin, uint32_t* __restrict__ tcnt,
-                                               uint32_t* __restrict__ stats) {
-    __shared__ uint32_t stage[kSortStage * kAB];
+// The largest in-degree of a list (a statistic of the backward); rows longer
+// than kInCap were truncated by k_transpose, which flags it.
+__global__ __launch_bounds__(kAB) void k_tstats(uint32_t n, const uint32_t* __restrict__ tcnt,
+                                                uint32_t* __restrict__ flags) {
     const uint32_t j = agid();
-    if (j >= n) return;
-    const uint32_t full = tcnt[j];
-    atomicMax(stats + 3, full);
-    const uint32_t m = min(full, kInCap);
-    tcnt[j] = m;
-    if (m <= kSortStage) {
-        uint32_t* row = &stage[threadIdx.x];
-        for (uint32_t a = 0; a < m; ++a) {
-            const uint32_t v = tin[(size_t)a * n + j];
-            uint32_t b = a;
-            while (b > 0) {
-                const uint32_t u = row[(b - 1) * kAB];
-                if (u <= v) break;
-                row[b * kAB] = u;
-                --b;
-            }
-            row[b * kAB] = v;
-        }
-        for (uint32_t a = 0; a < m; ++a) tin[(size_t)a * n + j] = row[a * kAB];
-        return;
-    }
-    for (uint32_t a = 1; a < m; ++a) {
-        const uint32_t v = tin[(size_t)a * n + j];
-        uint32_t b = a;
-        while (b > 0) {
-            const uint32_t u = tin[(size_t)(b - 1) * n + j];
-            if (u <= v) break;
-            tin[(size_t)b * n + j] = u;
-            --b;
-        }
-        tin[(size_t)b * n + j] = v;
-    }
+    block_max<1>({j < n ? tcnt[j] : 0u}, {flags + kInMax});
 }
 
 // ---- recording forward ----------------------------------------------------
@@ -332,6 +498,7 @@ __device__ __forceinline__ double len(D3 a) { return sqrt(dot(a, a)); }
 __device__ __forceinline__ D3 cross(D3 a, D3 b) {
     return D3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
 }
+__device__ __forceinline__ D3 qsum3(D3 v) { return D3{qsum(v.x), qsum(v.y), qsum(v.z)}; }
 
 // The constants of a step in FP64: the FP32 step constants widened (FP32
 // trajectory), or PBFluids::setParams (src/fluid/PBFluids.cpp:44-101,
@@ -485,29 +652,20 @@ __global__ __launch_bounds__(kAB) void k_reorder64(uint32_t n, const uint32_t* _
 __global__ __launch_bounds__(kAB) void k_edges64(C64 k, const double4* __restrict__ X, const int2* __restrict__ cells,
                                                  uint32_t* __restrict__ nbr, uint32_t* __restrict__ cnt,
                                                  uint32_t* __restrict__ overflow) {
-    const uint32_t i = agid();
-    if (i >= k.n) return;
-    const D3 xi = d3(X[i]);
-    const double h2 = k.h * k.h;
-    const int32_t cx = (int32_t)floor(xi.x / k.h), cy = (int32_t)floor(xi.y / k.h), cz = (int32_t)floor(xi.z / k.h);
+    const uint32_t g = agid(), i = g / kQ, lane = g % kQ;
+    const bool live = i < k.n;
     uint32_t m = 0;
-    for (int q = 0; q < 27; ++q) {
-        const int2 se = cells[cell_hash(cx + q % 3 - 1, cy + (q / 3) % 3 - 1, cz + q / 9 - 1, k.hash_mask)];
-        for (int32_t s = se.x; s < se.y; ++s) {
-            if ((uint32_t)s == i) continue;
-            const D3 r = sub(xi, d3(X[s]));
-            if (dot(r, r) <= h2) {
-                if (m < kEdgeCap) nbr[(size_t)m * k.n + i] = (uint32_t)s;
-                ++m;
-            }
-        }
+    if (live) {
+        const D3 xi = d3(X[i]);
+        const double h2 = k.h * k.h;
+        const int32_t cx = (int32_t)floor(xi.x / k.h), cy = (int32_t)floor(xi.y / k.h),
+                      cz = (int32_t)floor(xi.z / k.h);
+        m = quad_list(X, cells, BucketWalk(cx, cy, cz, k.hash_mask, 0u), k.n, i, lane, nbr, [&](const double4& xk) {
+            const D3 r = sub(xi, d3(xk));
+            return dot(r, r) <= h2;
+        });
     }
-    atomicMax(overflow + 2, m);
-    if (m > kEdgeCap) {
-        atomicOr(overflow, 1u);
-        m = kEdgeCap;
-    }
-    cnt[i] = m;
+    store_count(m, live, i, lane, cnt, overflow);
 }
 
 __global__ __launch_bounds__(kAB) void k_rec_lambda64(C64 k, const double4* __restrict__ P,
@@ -668,17 +826,19 @@ __global__ __launch_bounds__(kAB) void k_rev_vort(C64 k, const T4* __restrict__ 
                                                   double4* __restrict__ gw_own, double4* __restrict__ ep,
                                                   double* __restrict__ es) {
     using S = typename Prec<T4>::S;
-    const uint32_t i = agid();
+    const uint32_t g = agid(), i = g / kQ, lane = g % kQ;
     if (i >= k.n) return;
     const uint32_t m = cnt[i];
     const T4 e4 = eta_tape[i];
     if (slen3(e4) < S(1e-6)) {  // the forward's test
-        for (uint32_t r = 0; r < m; ++r) {
+        for (uint32_t r = lane; r < m; r += kQ) {
             ep[(size_t)r * k.n + i] = make_double4(0.0, 0.0, 0.0, 0.0);
             es[(size_t)r * k.n + i] = 0.0;
         }
-        gS_own[i] = make_double4(0.0, 0.0, 0.0, 0.0);
-        gw_own[i] = make_double4(0.0, 0.0, 0.0, 0.0);
+        if (lane == 0) {
+            gS_own[i] = make_double4(0.0, 0.0, 0.0, 0.0);
+            gw_own[i] = make_double4(0.0, 0.0, 0.0, 0.0);
+        }
         return;
     }
     const D3 eta = d3(e4);
@@ -689,11 +849,11 @@ __global__ __launch_bounds__(kAB) void k_rev_vort(C64 k, const T4* __restrict__ 
     const D3 om = d3(curl[i]);
     const D3 vb = d3(gvel[perm[i]]);
     const D3 nb = scl(cross(om, vb), a);
-    gw_own[i] = d4(scl(cross(vb, nn), a));
+    if (lane == 0) gw_own[i] = d4(scl(cross(vb, nn), a));
     const D3 etab = scl(sub(nb, scl(nn, dot(nn, nb))), 1.0 / el);
     const D3 pi = d3(posS[i]);
     D3 gs = dz3();
-    for (uint32_t r = 0; r < m; ++r) {
+    for (uint32_t r = lane; r < m; r += kQ) {
         const size_t e = (size_t)r * k.n + i;
         const uint32_t j = nbr[e];
         const D3 d = sub(pi, d3(posS[j]));
@@ -704,7 +864,8 @@ __global__ __launch_bounds__(kAB) void k_rev_vort(C64 k, const T4* __restrict__ 
         ep[e] = d4(rb);
         es[e] = inv * dot(d_spiky(k, d, r2), etab);
     }
-    gS_own[i] = d4(gs);
+    gs = qsum3(gs);
+    if (lane == 0) gS_own[i] = d4(gs);
 }
 
 // Vorticity, the terms of the pairs reaching j: position adjoint into the
@@ -716,23 +877,24 @@ __global__ __launch_bounds__(kAB) void k_rev_vort_t(uint32_t n, const T4* __rest
                                                     const double* __restrict__ es, const double4* __restrict__ gS_own,
                                                     const double4* __restrict__ gw_own, double4* __restrict__ gpos,
                                                     double4* __restrict__ gw) {
-    const uint32_t j = agid();
+    const uint32_t g = agid(), j = g / kR, lane = g % kR;
     if (j >= n) return;
-    D3 sp = dz3();
-    double sm = 0.0;
-    const uint32_t m = tcnt[j];
-    for (uint32_t q = 0; q < m; ++q) {
-        const uint32_t e = tin[(size_t)q * n + j];
-        sp = add(sp, d3(ep[e]));
-        sm += es[e];
-    }
-    const D3 g = sub(d3(gS_own[j]), sp);
+    double r[4];
+    row_sum<4>(tin, n, j, min(tcnt[j], kInCap), lane, [&](uint32_t e, double* v) {
+        const double4 a = ep[e];
+        v[0] = a.x;
+        v[1] = a.y;
+        v[2] = a.z;
+        v[3] = es[e];
+    }, r);
+    if (lane != 0) return;
+    const D3 gj = sub(d3(gS_own[j]), dmk(r[0], r[1], r[2]));
     const uint32_t id = perm[j];
-    gpos[id] = d4(add(d3(gpos[id]), g));
+    gpos[id] = d4(add(d3(gpos[id]), gj));
     const D3 wj = d3(curl[j]);
     const double mj = len(wj);
     D3 w = d3(gw_own[j]);
-    if (mj > 0.0) w = add(w, scl(wj, sm / mj));
+    if (mj > 0.0) w = add(w, scl(wj, r[3] / mj));
     gw[j] = d4(w);
 }
 
@@ -751,7 +913,7 @@ __global__ __launch_bounds__(kAB) void k_rev_integrate(C64 k, const T4* __restri
                                                        double* __restrict__ acc_visc, double4* __restrict__ ep,
                                                        double4* __restrict__ ev) {
     using S = typename Prec<T4>::S;
-    const uint32_t s = agid();
+    const uint32_t g = agid(), s = g / kQ, lane = g % kQ;
     if (s >= k.n) return;
     const uint32_t id = perm[s];
     // the forward's decisions, in its precision: wall flags and the speed clamp
@@ -789,12 +951,13 @@ __global__ __launch_bounds__(kAB) void k_rev_integrate(C64 k, const T4* __restri
     const D3 visc = d3(vt);
     D3 vb = dz3();
     if (do_visc) {
-        acc_visc[id] += dot(gv, visc);
+        if (lane == 0) acc_visc[id] += dot(gv, visc);
         vb = scl(gv, k.viscosity);
     }
     const D3 gxp = add(d3(gpos[id]), scl(gv, inv_dt));
-    gxold[id] = d4(scl(gv, -inv_dt));
-    D3 gp = dmk(cxh ? 0.0 : gxp.x, cyh ? 0.0 : gxp.y, czh ? 0.0 : gxp.z);
+    if (lane == 0) gxold[id] = d4(scl(gv, -inv_dt));
+    // the slot's own term (wall-masked gxp) enters the quad's sum once, in lane 0
+    D3 gp = lane == 0 ? dmk(cxh ? 0.0 : gxp.x, cyh ? 0.0 : gxp.y, czh ? 0.0 : gxp.z) : dz3();
     const D3 wb = (do_vort && gw) ? d3(gw[s]) : dz3();
     D3 gvi = dz3();
     const double inv = k.inv_rho0;
@@ -802,7 +965,7 @@ __global__ __launch_bounds__(kAB) void k_rev_integrate(C64 k, const T4* __restri
     if (do_visc || do_vort) {
         const D3 p = d3(p4);
         const D3 vi = d3(vs[s]);
-        for (uint32_t r = 0; r < m; ++r) {
+        for (uint32_t r = lane; r < m; r += kQ) {
             const size_t e = (size_t)r * k.n + s;
             const uint32_t j = nbr[e];
             const D3 d = sub(p, d3(P[j]));
@@ -824,13 +987,17 @@ __global__ __launch_bounds__(kAB) void k_rev_integrate(C64 k, const T4* __restri
             ep[e] = d4(rb);
         }
     } else {
-        for (uint32_t r = 0; r < m; ++r) {
+        for (uint32_t r = lane; r < m; r += kQ) {
             ep[(size_t)r * k.n + s] = make_double4(0.0, 0.0, 0.0, 0.0);
             ev[(size_t)r * k.n + s] = make_double4(0.0, 0.0, 0.0, 0.0);
         }
     }
-    gP_own[s] = d4(gp);
-    gvs_own[s] = d4(gvi);
+    gp = qsum3(gp);
+    gvi = qsum3(gvi);
+    if (lane == 0) {
+        gP_own[s] = d4(gp);
+        gvs_own[s] = d4(gvi);
+    }
 }
 
 __global__ __launch_bounds__(kAB) void k_rev_pairs_vec2(uint32_t n, const uint32_t* __restrict__ tin,
@@ -839,17 +1006,21 @@ __global__ __launch_bounds__(kAB) void k_rev_pairs_vec2(uint32_t n, const uint32
                                                         const double4* __restrict__ gP_own,
                                                         const double4* __restrict__ gvs_own, double4* __restrict__ gP,
                                                         double4* __restrict__ gvs) {
-    const uint32_t j = agid();
+    const uint32_t g = agid(), j = g / kR, lane = g % kR;
     if (j >= n) return;
-    D3 sp = dz3(), sv = dz3();
-    const uint32_t m = tcnt[j];
-    for (uint32_t q = 0; q < m; ++q) {
-        const uint32_t e = tin[(size_t)q * n + j];
-        sp = add(sp, d3(ep[e]));
-        sv = add(sv, d3(ev[e]));
-    }
-    gP[j] = d4(sub(d3(gP_own[j]), sp));
-    gvs[j] = d4(add(d3(gvs_own[j]), sv));
+    double r[6];
+    row_sum<6>(tin, n, j, min(tcnt[j], kInCap), lane, [&](uint32_t e, double* v) {
+        const double4 a = ep[e], b = ev[e];
+        v[0] = a.x;
+        v[1] = a.y;
+        v[2] = a.z;
+        v[3] = b.x;
+        v[4] = b.y;
+        v[5] = b.z;
+    }, r);
+    if (lane != 0) return;
+    gP[j] = d4(sub(d3(gP_own[j]), dmk(r[0], r[1], r[2])));
+    gvs[j] = d4(add(d3(gvs_own[j]), dmk(r[3], r[4], r[5])));
 }
 
 // Clamp-and-add of CS_ApplyDeltaP.glsl:80-106 and delta-p
@@ -864,7 +1035,7 @@ __global__ __launch_bounds__(kAB) void k_rev_deltap(C64 k, const T4* __restrict_
                                                     double4* __restrict__ gQ_own, double* __restrict__ gl_own,
                                                     double4* __restrict__ ep, double* __restrict__ es) {
     using S = typename Prec<T4>::S;
-    const uint32_t i = agid();
+    const uint32_t gid = agid(), i = gid / kQ, lane = gid % kQ;
     if (i >= k.n) return;
     const D3 g = d3(gP[i]);
     const T4 dp4 = dptape[i];
@@ -879,10 +1050,10 @@ __global__ __launch_bounds__(kAB) void k_rev_deltap(C64 k, const T4* __restrict_
     const double inv = k.inv_rho0;
     const D3 xi = d3(Pprev[i]);
     const double li = (double)lr[i].x;
-    D3 gq = g;
+    D3 gq = lane == 0 ? g : dz3();  // the identity term once per quad
     double gl = 0.0;
     const uint32_t m = cnt[i];
-    for (uint32_t r = 0; r < m; ++r) {
+    for (uint32_t r = lane; r < m; r += kQ) {
         const size_t e = (size_t)r * k.n + i;
         const uint32_t j = nbr[e];
         const D3 d = sub(xi, d3(Pprev[j]));
@@ -895,6 +1066,9 @@ __global__ __launch_bounds__(kAB) void k_rev_deltap(C64 k, const T4* __restrict_
         ep[e] = d4(rb);
         es[e] = t;
     }
+    gq = qsum3(gq);
+    gl = qsum(gl);
+    if (lane != 0) return;
     gQ_own[i] = d4(gq);
     gl_own[i] = gl;
 }
@@ -906,18 +1080,19 @@ __global__ __launch_bounds__(kAB) void k_rev_pairs_vec_scalar(uint32_t n, const 
                                                               const double4* __restrict__ gQ_own,
                                                               const double* __restrict__ gl_own,
                                                               double4* __restrict__ gQ, double* __restrict__ gl) {
-    const uint32_t j = agid();
+    const uint32_t g = agid(), j = g / kR, lane = g % kR;
     if (j >= n) return;
-    D3 sp = dz3();
-    double ss = 0.0;
-    const uint32_t m = tcnt[j];
-    for (uint32_t q = 0; q < m; ++q) {
-        const uint32_t e = tin[(size_t)q * n + j];
-        sp = add(sp, d3(ep[e]));
-        ss += es[e];
-    }
-    gQ[j] = d4(sub(d3(gQ_own[j]), sp));
-    gl[j] = gl_own[j] + ss;
+    double r[4];
+    row_sum<4>(tin, n, j, min(tcnt[j], kInCap), lane, [&](uint32_t e, double* v) {
+        const double4 a = ep[e];
+        v[0] = a.x;
+        v[1] = a.y;
+        v[2] = a.z;
+        v[3] = es[e];
+    }, r);
+    if (lane != 0) return;
+    gQ[j] = d4(sub(d3(gQ_own[j]), dmk(r[0], r[1], r[2])));
+    gl[j] = gl_own[j] + r[3];
 }
 
 // Lambda (CS_ComputeLambdas.glsl:108-162), slot i's own terms, added to gQ.
@@ -929,7 +1104,7 @@ __global__ __launch_bounds__(kAB) void k_rev_lambda(C64 k, const T4* __restrict_
                                                     const double* __restrict__ gl, double4* __restrict__ gQ,
                                                     double* __restrict__ acc_eps, double4* __restrict__ ep) {
     using S = typename Prec<T4>::S;
-    const uint32_t i = agid();
+    const uint32_t g = agid(), i = g / kQ, lane = g % kQ;
     if (i >= k.n) return;
     const double lb = gl[i];
     const S rho_s = lr[i].y;
@@ -942,7 +1117,7 @@ __global__ __launch_bounds__(kAB) void k_rev_lambda(C64 k, const T4* __restrict_
     const double C = active ? (double)craw : -(double)S(k.cohesion);
     const double Cb = -lb / D;
     const double Db = lb * C / (D * D);
-    acc_eps[perm[i]] += Db;
+    if (lane == 0) acc_eps[perm[i]] += Db;
     const double inv = k.inv_rho0;
     const double rhob = active ? Cb / k.rho0 : 0.0;
     const double sb = 2.0 * inv * inv * Db;
@@ -950,7 +1125,7 @@ __global__ __launch_bounds__(kAB) void k_rev_lambda(C64 k, const T4* __restrict_
     const D3 xi = d3(Pprev[i]);
     D3 acc = dz3();
     const uint32_t m = cnt[i];
-    for (uint32_t r = 0; r < m; ++r) {
+    for (uint32_t r = lane; r < m; r += kQ) {
         const size_t e = (size_t)r * k.n + i;
         const uint32_t j = nbr[e];
         const D3 d = sub(xi, d3(Pprev[j]));
@@ -960,18 +1135,23 @@ __global__ __launch_bounds__(kAB) void k_rev_lambda(C64 k, const T4* __restrict_
         acc = add(acc, rb);
         ep[e] = d4(rb);
     }
-    gQ[i] = d4(add(d3(gQ[i]), acc));
+    acc = qsum3(acc);
+    if (lane == 0) gQ[i] = d4(add(d3(gQ[i]), acc));
 }
 
 __global__ __launch_bounds__(kAB) void k_rev_pairs_sub(uint32_t n, const uint32_t* __restrict__ tin,
                                                        const uint32_t* __restrict__ tcnt,
                                                        const double4* __restrict__ ep, double4* __restrict__ gQ) {
-    const uint32_t j = agid();
+    const uint32_t g = agid(), j = g / kR, lane = g % kR;
     if (j >= n) return;
-    D3 sp = dz3();
-    const uint32_t m = tcnt[j];
-    for (uint32_t q = 0; q < m; ++q) sp = add(sp, d3(ep[tin[(size_t)q * n + j]]));
-    gQ[j] = d4(sub(d3(gQ[j]), sp));
+    double r[3];
+    row_sum<3>(tin, n, j, min(tcnt[j], kInCap), lane, [&](uint32_t e, double* v) {
+        const double4 a = ep[e];
+        v[0] = a.x;
+        v[1] = a.y;
+        v[2] = a.z;
+    }, r);
+    if (lane == 0) gQ[j] = d4(sub(d3(gQ[j]), dmk(r[0], r[1], r[2])));
 }
 
 // Predict (CS_PredictAndHash.glsl:88-130 without the interaction force):
@@ -1013,19 +1193,22 @@ __global__ __launch_bounds__(kAB) void k_reduce2(uint32_t n, const double* __res
 }
 
 // Largest |component| of the state adjoint, rounded to FP32 (as ordered bits
-// of non-negative floats; a NaN orders above +inf).
+// of non-negative floats; a NaN orders above +inf), into the kSpread slots of
+// out (position) and out + kSpread (velocity).
 __global__ __launch_bounds__(kAB) void k_absmax(uint32_t n, const double4* __restrict__ gpos,
                                                 const double4* __restrict__ gvel, uint32_t* __restrict__ out) {
     const uint32_t i = agid();
-    if (i >= n) return;
-    const double4 p = gpos[i], v = gvel[i];
-    const float m = (float)fmax(fmax(fabs(p.x), fabs(p.y)), fabs(p.z));
-    const float q = (float)fmax(fmax(fabs(v.x), fabs(v.y)), fabs(v.z));
-    uint32_t bp = __float_as_uint(m), bq = __float_as_uint(q);
-    if (p.x != p.x || p.y != p.y || p.z != p.z) bp = 0x7fc00000u;
-    if (v.x != v.x || v.y != v.y || v.z != v.z) bq = 0x7fc00000u;
-    atomicMax(out, bp);
-    atomicMax(out + 1, bq);
+    uint32_t bp = 0u, bq = 0u;
+    if (i < n) {
+        const double4 p = gpos[i], v = gvel[i];
+        const float m = (float)fmax(fmax(fabs(p.x), fabs(p.y)), fabs(p.z));
+        const float q = (float)fmax(fmax(fabs(v.x), fabs(v.y)), fabs(v.z));
+        bp = __float_as_uint(m);
+        bq = __float_as_uint(q);
+        if (p.x != p.x || p.y != p.y || p.z != p.z) bp = 0x7fc00000u;
+        if (v.x != v.x || v.y != v.y || v.z != v.z) bq = 0x7fc00000u;
+    }
+    block_max<2>({bp, bq}, {out, out + kSpread});
 }
 
 // Bitwise comparison of the replayed state with the forward's checkpoint.
@@ -1117,12 +1300,14 @@ struct fsi_adjoint {
     double *gl = nullptr, *gl_own = nullptr, *acc_visc = nullptr, *acc_eps = nullptr;
     double4 *ep = nullptr, *ev = nullptr;
     double* es = nullptr;
-    // [0] list overflow bits, [1] replayed records that differ from the
-    // forward, [2] / [3] largest neighbour count / in-degree of any list
+    // kFlagWords: [0] list overflow bits, [1] replayed records that differ
+    // from the forward, then the slots of the largest neighbour count and of
+    // the largest in-degree of any list
     uint32_t* flags = nullptr;
-    uint32_t stats[4] = {0, 0, 0, 0};  // flags of the last backward
-    uint32_t* norms = nullptr;          // frames x 2: max |adjoint| of pos, vel at each frame start
-    std::vector<uint32_t> host_norms;
+    uint32_t stats[4] = {0, 0, 0, 0};  // flags of the last backward (reduce_flags)
+    uint32_t* norms = nullptr;          // frames x 2 x kSpread: max |adjoint| of pos, vel at each frame start
+    std::vector<uint32_t> host_norms;   // frames x 2
+    std::vector<uint32_t> host_words;   // staging for flags and norms
     double* dred = nullptr;
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
 };
@@ -1242,11 +1427,23 @@ hipError_t alloc_common(fsi_adjoint* a) {
     ALLOC(a->ep, (size_t)kEdgeCap * n);
     ALLOC(a->ev, (size_t)kEdgeCap * n);
     ALLOC(a->es, (size_t)kEdgeCap * n);
-    ALLOC(a->flags, 4);
-    ALLOC(a->norms, 2 * (size_t)a->frames);
+    ALLOC(a->flags, kFlagWords);
+    ALLOC(a->norms, 2 * kSpread * (size_t)a->frames);
     ALLOC(a->dred, 2);
 #undef ALLOC
     return hipSuccess;
+}
+
+// The flag words as (overflow bits, replay mismatches, largest out-degree,
+// largest in-degree), the maxima reduced over their slots.
+void reduce_flags(const uint32_t* w, uint32_t out[4]) {
+    out[0] = w[0];
+    out[1] = w[1];
+    out[2] = out[3] = 0u;
+    for (uint32_t s = 0; s < kSpread; ++s) {
+        out[2] = std::max(out[2], w[kOutMax + s]);
+        out[3] = std::max(out[3], w[kInMax + s]);
+    }
 }
 
 // Stream order with the caller: our work waits for the caller's stream, and
@@ -1263,23 +1460,38 @@ hipError_t join_out(fsi_adjoint* a, void* stream) {
     return hipStreamWaitEvent((hipStream_t)stream, a->ev_out, 0);
 }
 
+// The transposed list of L (its in-degree counters were cleared for the
+// frame by zero_counters).
 hipError_t finish_list(fsi_adjoint* a, const EdgeList& L) {
     hipStream_t st = a->core.stream;
     const uint32_t n = a->n;
-    hipLaunchKernelGGL(k_zero_u32, agrid(n), dim3(kAB), 0, st, L.tcnt, (size_t)n);
-    hipLaunchKernelGGL(k_transpose, agrid(n), dim3(kAB), 0, st, n, L.nbr, L.cnt, L.tin, L.tcnt, a->flags);
-    hipLaunchKernelGGL(k_tsort, agrid(n), dim3(kAB), 0, st, n, L.tin, L.tcnt, a->flags);
+    hipLaunchKernelGGL(k_transpose, qgrid(n), dim3(kAB), 0, st, n, L.nbr, L.cnt, L.tin, L.tcnt, a->flags);
+    hipLaunchKernelGGL(k_tstats, agrid(n), dim3(kAB), 0, st, n, L.tcnt, a->flags);
+    return hipGetLastError();
+}
+
+// The in-degree counters of every list, before a frame's list builds.
+__global__ __launch_bounds__(kAB) void k_zero_tcnt(uint32_t* __restrict__ base, size_t stride, uint32_t n) {
+    const uint32_t i = agid();
+    if (i < n) base[(size_t)blockIdx.y * stride + i] = 0u;
+}
+
+hipError_t zero_counters(fsi_adjoint* a) {
+    const size_t per = ((size_t)kEdgeCap + kInCap + 2) * a->n;
+    uint32_t* base = a->list_mem + ((size_t)kEdgeCap + kInCap + 1) * a->n;
+    hipLaunchKernelGGL(k_zero_tcnt, dim3((a->n + kAB - 1) / kAB, (unsigned)a->lists.size()), dim3(kAB), 0,
+                       a->core.stream, base, per, a->n);
     return hipGetLastError();
 }
 
 hipError_t build_list(fsi_adjoint* a, const StepConst& c, const EdgeList& L, const float4* X) {
-    hipLaunchKernelGGL(k_edges, agrid(a->n), dim3(kAB), 0, a->core.stream, c, X, a->core.d.cells, L.nbr, L.cnt,
+    hipLaunchKernelGGL(k_edges, qgrid(a->n), dim3(kAB), 0, a->core.stream, c, X, a->core.d.cells, L.nbr, L.cnt,
                        a->flags);
     return finish_list(a, L);
 }
 
 hipError_t build_list(fsi_adjoint* a, const C64& k, const EdgeList& L, const double4* X) {
-    hipLaunchKernelGGL(k_edges64, agrid(a->n), dim3(kAB), 0, a->core.stream, k, X, a->core.d.cells, L.nbr, L.cnt,
+    hipLaunchKernelGGL(k_edges64, qgrid(a->n), dim3(kAB), 0, a->core.stream, k, X, a->core.d.cells, L.nbr, L.cnt,
                        a->flags);
     return finish_list(a, L);
 }
@@ -1295,6 +1507,7 @@ int record_frame(fsi_adjoint* a, const StepConst& c) {
     const size_t n = a->n;
     const uint32_t I = a->I;
     const int key_bits = log2u(a->cfg.hash_size);
+    FSI_HIP_TRY(zero_counters(a));
     for (uint32_t s = 0; s < a->S; ++s) {
         FSI_HIP_TRY(hipMemcpyAsync(t.xold + s * n, d.pos, sizeof(float4) * n, hipMemcpyDeviceToDevice, st));
         FSI_HIP_TRY(launch_predict(c, d, st));
@@ -1339,6 +1552,7 @@ int record_frame64(fsi_adjoint* a, const StepConst& c, const C64& k) {
     const uint32_t I = a->I;
     const int key_bits = log2u(a->cfg.hash_size);
     const dim3 g = agrid(a->n), b(kAB);
+    FSI_HIP_TRY(zero_counters(a));
     for (uint32_t s = 0; s < a->S; ++s) {
         FSI_HIP_TRY(hipMemcpyAsync(t.xold + s * n, t.pos, sizeof(double4) * n, hipMemcpyDeviceToDevice, st));
         hipLaunchKernelGGL(k_predict64, g, b, 0, st, k, t.pos, t.vel, t.xstar, d.keys);
@@ -1379,14 +1593,14 @@ int reverse_frame(fsi_adjoint* a, const C64& k, const Tape<T4>& t, const T4* pos
     const uint32_t n = a->n;
     const size_t ns = n;
     const uint32_t I = a->I, S = a->S;
-    const dim3 g = agrid(n), b(kAB);
+    const dim3 g = agrid(n), gq = qgrid(n), gr = rgrid(n), b(kAB);
     bool have_gw = false;
     if (k.enable_vort) {
         const EdgeList& L = a->lists.back();
         const uint32_t* perm = a->perm + (S - 1) * ns;
-        hipLaunchKernelGGL((k_rev_vort<T4>), g, b, 0, st, k, posS, curl, t.eta, perm, a->gvel, L.nbr, L.cnt, a->gown,
+        hipLaunchKernelGGL((k_rev_vort<T4>), gq, b, 0, st, k, posS, curl, t.eta, perm, a->gvel, L.nbr, L.cnt, a->gown,
                            a->gw_own, a->ep, a->es);
-        hipLaunchKernelGGL((k_rev_vort_t<T4>), g, b, 0, st, n, curl, perm, L.tin, L.tcnt, a->ep, a->es, a->gown,
+        hipLaunchKernelGGL((k_rev_vort_t<T4>), gr, b, 0, st, n, curl, perm, L.tin, L.tcnt, a->ep, a->es, a->gown,
                            a->gw_own, a->gpos, a->gw);
         have_gw = true;
     }
@@ -1394,23 +1608,23 @@ int reverse_frame(fsi_adjoint* a, const C64& k, const Tape<T4>& t, const T4* pos
         const uint32_t* perm = a->perm + s * ns;
         const T4* Ps = t.P + (size_t)s * (I + 1) * ns;
         const EdgeList& Li = list_of(a, s, I);
-        hipLaunchKernelGGL((k_rev_integrate<T4>), g, b, 0, st, k, t.xold + s * ns, t.vs + s * ns, Ps + I * ns,
+        hipLaunchKernelGGL((k_rev_integrate<T4>), gq, b, 0, st, k, t.xold + s * ns, t.vs + s * ns, Ps + I * ns,
                            t.visc + s * ns, perm, Li.nbr, Li.cnt, a->gpos, a->gvel,
                            (have_gw && s == S - 1) ? a->gw : nullptr, a->gown, a->gvs_own, a->gxold, a->acc_visc,
                            a->ep, a->ev);
-        hipLaunchKernelGGL(k_rev_pairs_vec2, g, b, 0, st, n, Li.tin, Li.tcnt, a->ep, a->ev, a->gown, a->gvs_own, a->gP,
+        hipLaunchKernelGGL(k_rev_pairs_vec2, gr, b, 0, st, n, Li.tin, Li.tcnt, a->ep, a->ev, a->gown, a->gvs_own, a->gP,
                            a->gvs);
         for (uint32_t it = I; it-- > 0;) {
             const EdgeList& L = list_of(a, s, it);
             const size_t q = ((size_t)s * I + it) * ns;
             const T4* Pprev = Ps + it * ns;
-            hipLaunchKernelGGL((k_rev_deltap<T4>), g, b, 0, st, k, Pprev, t.lr + q, t.dp + q, L.nbr, L.cnt, a->gP,
+            hipLaunchKernelGGL((k_rev_deltap<T4>), gq, b, 0, st, k, Pprev, t.lr + q, t.dp + q, L.nbr, L.cnt, a->gP,
                                a->gown, a->gl_own, a->ep, a->es);
-            hipLaunchKernelGGL(k_rev_pairs_vec_scalar, g, b, 0, st, n, L.tin, L.tcnt, a->ep, a->es, a->gown, a->gl_own,
+            hipLaunchKernelGGL(k_rev_pairs_vec_scalar, gr, b, 0, st, n, L.tin, L.tcnt, a->ep, a->es, a->gown, a->gl_own,
                                a->gQ, a->gl);
-            hipLaunchKernelGGL((k_rev_lambda<T4>), g, b, 0, st, k, Pprev, t.lr + q, t.aux + q, perm, L.nbr, L.cnt,
+            hipLaunchKernelGGL((k_rev_lambda<T4>), gq, b, 0, st, k, Pprev, t.lr + q, t.aux + q, perm, L.nbr, L.cnt,
                                a->gl, a->gQ, a->acc_eps, a->ep);
-            hipLaunchKernelGGL(k_rev_pairs_sub, g, b, 0, st, n, L.tin, L.tcnt, a->ep, a->gQ);
+            hipLaunchKernelGGL(k_rev_pairs_sub, gr, b, 0, st, n, L.tin, L.tcnt, a->ep, a->gQ);
             std::swap(a->gP, a->gQ);
         }
         hipLaunchKernelGGL(k_rev_predict, g, b, 0, st, k, perm, a->gP, a->gvs, a->gxold, a->gpos, a->gvel);
@@ -1458,7 +1672,7 @@ int forward_impl(fsi_adjoint* a, const void* pos0, const void* vel0, bool f64_io
             hipLaunchKernelGGL(k_f2d, g, b, 0, st, a->n, (const float4*)pos0, t.pos, 1);
             hipLaunchKernelGGL(k_f2d, g, b, 0, st, a->n, (const float4*)vel0, t.vel, 1);
         }
-        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(kAB), 0, st, a->flags, (size_t)4);
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(kAB), 0, st, a->flags, (size_t)kFlagWords);
         for (uint32_t f = 0; f < a->frames; ++f) {
             FSI_HIP_TRY(hipMemcpyAsync(t.ck_pos + f * n, t.pos, sizeof(double4) * n, hipMemcpyDeviceToDevice, st));
             FSI_HIP_TRY(hipMemcpyAsync(t.ck_vel + f * n, t.vel, sizeof(double4) * n, hipMemcpyDeviceToDevice, st));
@@ -1475,9 +1689,12 @@ int forward_impl(fsi_adjoint* a, const void* pos0, const void* vel0, bool f64_io
         }
         FSI_HIP_TRY(hipGetLastError());
         uint32_t flags[4];
-        FSI_HIP_TRY(hipMemcpyAsync(flags, a->flags, sizeof(flags), hipMemcpyDeviceToHost, st));
+        a->host_words.resize(kFlagWords);
+        FSI_HIP_TRY(hipMemcpyAsync(a->host_words.data(), a->flags, sizeof(uint32_t) * kFlagWords,
+                                   hipMemcpyDeviceToHost, st));
         FSI_HIP_TRY(join_out(a, stream));
         FSI_HIP_TRY(hipStreamSynchronize(st));
+        reduce_flags(a->host_words.data(), flags);
         if (flags[0])
             return fail(FSI_ERR_CAPACITY, "more than 96 neighbours (" + std::to_string(flags[2]) +
                                               ") or 128 pairs into one particle (" + std::to_string(flags[3]) +
@@ -1523,8 +1740,9 @@ int backward_impl(fsi_adjoint* a, const void* grad_pos, const void* grad_vel, bo
     const size_t n = a->n;
     const dim3 g = agrid(a->n), b(kAB);
     FSI_HIP_TRY(join_in(a, stream));
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(kAB), 0, st, a->flags, (size_t)4);
-    hipLaunchKernelGGL(k_zero_u32, agrid(2 * a->frames), dim3(kAB), 0, st, a->norms, 2 * (size_t)a->frames);
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(kAB), 0, st, a->flags, (size_t)kFlagWords);
+    hipLaunchKernelGGL(k_zero_u32, agrid(2 * kSpread * a->frames), dim3(kAB), 0, st, a->norms,
+                       2 * kSpread * (size_t)a->frames);
     hipLaunchKernelGGL(k_zero_f64, g, b, 0, st, a->acc_visc, n);
     hipLaunchKernelGGL(k_zero_f64, g, b, 0, st, a->acc_eps, n);
     for (int q = 0; q < 2; ++q) {
@@ -1555,7 +1773,7 @@ int backward_impl(fsi_adjoint* a, const void* grad_pos, const void* grad_vel, bo
             hipLaunchKernelGGL((k_compare<float4>), g, b, 0, st, a->n, d.vel, t.ck_vel + (f + 1) * n, a->flags + 1);
             if ((rc = reverse_frame(a, k, t, (const float4*)d.posS, (const float4*)d.curl))) return rc;
         }
-        hipLaunchKernelGGL(k_absmax, g, b, 0, st, a->n, a->gpos, a->gvel, a->norms + 2 * (size_t)f);
+        hipLaunchKernelGGL(k_absmax, g, b, 0, st, a->n, a->gpos, a->gvel, a->norms + 2 * kSpread * (size_t)f);
     }
     hipLaunchKernelGGL(k_reduce2, dim3(1), dim3(kAB), 0, st, a->n, a->acc_visc, a->acc_eps, a->dred);
     for (int q = 0; q < 2; ++q) {
@@ -1571,12 +1789,19 @@ int backward_impl(fsi_adjoint* a, const void* grad_pos, const void* grad_vel, bo
     double red[2] = {0.0, 0.0};
     uint32_t* flags = a->stats;
     FSI_HIP_TRY(hipMemcpyAsync(red, a->dred, sizeof(red), hipMemcpyDeviceToHost, st));
-    FSI_HIP_TRY(hipMemcpyAsync(flags, a->flags, sizeof(a->stats), hipMemcpyDeviceToHost, st));
-    a->host_norms.resize(2 * (size_t)a->frames);
-    FSI_HIP_TRY(hipMemcpyAsync(a->host_norms.data(), a->norms, sizeof(uint32_t) * a->host_norms.size(),
+    const size_t nnorm = 2 * kSpread * (size_t)a->frames;
+    a->host_words.resize(kFlagWords + nnorm);
+    FSI_HIP_TRY(hipMemcpyAsync(a->host_words.data(), a->flags, sizeof(uint32_t) * kFlagWords, hipMemcpyDeviceToHost,
+                               st));
+    FSI_HIP_TRY(hipMemcpyAsync(a->host_words.data() + kFlagWords, a->norms, sizeof(uint32_t) * nnorm,
                                hipMemcpyDeviceToHost, st));
     FSI_HIP_TRY(join_out(a, stream));
     FSI_HIP_TRY(hipStreamSynchronize(st));
+    reduce_flags(a->host_words.data(), flags);
+    a->host_norms.assign(2 * (size_t)a->frames, 0u);
+    for (size_t q = 0; q < a->host_norms.size(); ++q)
+        for (uint32_t w = 0; w < kSpread; ++w)
+            a->host_norms[q] = std::max(a->host_norms[q], a->host_words[kFlagWords + q * kSpread + w]);
     if (flags[1])
         return fail(FSI_ERR_STATE, "replay differs from the forward (" + std::to_string(flags[1]) + " records)");
     if (flags[0])

@@ -21,6 +21,12 @@ struct BucketWalk {
           hz((uint32_t)(cz - 1) * 83492791u),
           mask(dm.hash_mask),
           base(dm.hash_base) {}
+    __device__ __forceinline__ BucketWalk(int32_t cx, int32_t cy, int32_t cz, uint32_t hash_mask, uint32_t hash_base)
+        : hx((uint32_t)(cx - 1) * 73856093u),
+          hy((uint32_t)(cy - 1) * 19349663u),
+          hz((uint32_t)(cz - 1) * 83492791u),
+          mask(hash_mask),
+          base(hash_base) {}
     // q = 9 dz + 3 dy + dx over offsets 0..2 (dx fastest)
     __device__ __forceinline__ uint32_t bucket(int q) const {
         const uint32_t ix = (uint32_t)(q % 3), iy = (uint32_t)((q / 3) % 3), iz = (uint32_t)(q / 9);

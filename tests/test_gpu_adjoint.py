@@ -178,6 +178,25 @@ def test_differentiable_workload_65k_20_frames_fp64_trajectory(gpu):
     assert max(errs) <= 1e-3, errs
 
 
+@pytest.mark.parametrize("precision", [32, 64])
+def test_backward_is_bit_reproducible(gpu, precision):
+    """The transposed rows fill in atomic order, which differs between runs;
+    the gradient must not."""
+    import fsi_torch
+
+    cfg, parts, v0, frames = scene(dims=(12, 10, 8), frames=3, n_iter=3)
+    dtype = torch.float64 if precision == 64 else torch.float32
+    fh = fsi_torch.Frames(cfg, len(parts), frames, precision=precision)
+    grads = []
+    for _ in range(3):
+        pos, vel, vel0, vt, et = run(fh, parts, v0, 0.01, 600.0, need_grad=True, dtype=dtype)
+        (pos.square().sum() + vel.sin().sum()).backward()
+        grads.append((vel0.grad.cpu().numpy().copy(), float(vt.grad), float(et.grad)))
+    fh.close()
+    for g in grads[1:]:
+        assert np.array_equal(g[0], grads[0][0]) and g[1] == grads[0][1] and g[2] == grads[0][2]
+
+
 def test_backward_needs_its_forward(gpu):
     import fsi_torch
 
