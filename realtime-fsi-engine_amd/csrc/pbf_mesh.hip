@@ -27,6 +27,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <new>
 #include <string>
@@ -181,38 +182,76 @@ struct MeshView {
     int32_t dims[3];
 };
 
+// The dirty-refit statistics: the number of recomputed voxels and their
+// bounding box, each kept in kSpread slots (block b folds into slot
+// b mod kSpread; atomics of many blocks on one address would queue at the
+// cache).  Words: [0, kSpread) count, then per axis k the minima at
+// (1 + k) * kSpread and the maxima at (4 + k) * kSpread.
+constexpr uint32_t kSpread = 64, kDirtyWords = 7 * kSpread;
+
 // SDFBoundary::signedDistanceAt (SDFBoundary.cpp:400-419) of every voxel or,
-// with a mask, of the marked voxels (clearing the marks).  Voxels x-fastest.
+// with a mask, of the marked voxels (clearing the marks and adding them to
+// the dirty statistics).  Voxels x-fastest.
 __global__ __launch_bounds__(kMB) void k_sdf(MeshView m, float* __restrict__ grid, uint32_t* __restrict__ mask,
-                                             uint32_t total) {
+                                             uint32_t total, int32_t* __restrict__ dirty) {
     const uint32_t v = mgid();
-    if (v >= total) return;
-    if (mask) {
-        if (mask[v] == 0u) return;
-        mask[v] = 0u;
+    bool work = v < total;
+    if (work && mask) {
+        work = mask[v] != 0u;
+        if (work) mask[v] = 0u;
     }
     const uint32_t W = (uint32_t)m.dims[0], H = (uint32_t)m.dims[1];
     const int32_t x = (int32_t)(v % W), y = (int32_t)((v / W) % H), z = (int32_t)(v / (W * H));
-    const Q3 p{m.origin[0] + x * m.cell, m.origin[1] + y * m.cell, m.origin[2] + z * m.cell};
-    const DHit hit = bvh_closest(m.nodes, m.tris, p);
-    if (hit.tri < 0) {  // only when every distance is NaN (non-finite vertices)
-        grid[v] = __int_as_float(0x7fc00000);
-        return;
+    if (work) {
+        const Q3 p{m.origin[0] + x * m.cell, m.origin[1] + y * m.cell, m.origin[2] + z * m.cell};
+        const DHit hit = bvh_closest(m.nodes, m.tris, p);
+        if (hit.tri < 0) {  // only when every distance is NaN (non-finite vertices)
+            grid[v] = __int_as_float(0x7fc00000);
+        } else {
+            const DNorm& nt = m.norms[hit.tri];
+            const int32_t* f = m.faces + 3 * (size_t)hit.tri;
+            Q3 pn;
+            switch (hit.feature) {
+                case mesh::F_FACE: pn = q3(nt.fn); break;
+                case mesh::F_EDGE_AB: pn = q3(nt.ab); break;
+                case mesh::F_EDGE_AC: pn = q3(nt.ac); break;
+                case mesh::F_EDGE_BC: pn = q3(nt.bc); break;
+                case mesh::F_VERT_A: pn = q3(m.vn + 3 * (size_t)f[0]); break;
+                case mesh::F_VERT_B: pn = q3(m.vn + 3 * (size_t)f[1]); break;
+                default: pn = q3(m.vn + 3 * (size_t)f[2]); break;
+            }
+            const double sign = qdot(qsub(p, hit.p), pn) >= 0.0 ? 1.0 : -1.0;
+            grid[v] = (float)(sign * sqrt(hit.d2));
+        }
     }
-    const DNorm& nt = m.norms[hit.tri];
-    const int32_t* f = m.faces + 3 * (size_t)hit.tri;
-    Q3 pn;
-    switch (hit.feature) {
-        case mesh::F_FACE: pn = q3(nt.fn); break;
-        case mesh::F_EDGE_AB: pn = q3(nt.ab); break;
-        case mesh::F_EDGE_AC: pn = q3(nt.ac); break;
-        case mesh::F_EDGE_BC: pn = q3(nt.bc); break;
-        case mesh::F_VERT_A: pn = q3(m.vn + 3 * (size_t)f[0]); break;
-        case mesh::F_VERT_B: pn = q3(m.vn + 3 * (size_t)f[1]); break;
-        default: pn = q3(m.vn + 3 * (size_t)f[2]); break;
+    if (!mask) return;  // uniform over the grid
+    // statistics of the block's recomputed voxels
+    __shared__ int32_t part[7][kMB / 64];
+    int32_t st[7] = {work ? 1 : 0,          work ? x : INT32_MAX, work ? y : INT32_MAX, work ? z : INT32_MAX,
+                     work ? x : INT32_MIN, work ? y : INT32_MIN, work ? z : INT32_MIN};
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+        for (int o = 32; o > 0; o >>= 1) {
+            const int32_t u = __shfl_xor(st[k], o);
+            st[k] = k == 0 ? st[k] + u : (k < 4 ? min(st[k], u) : max(st[k], u));
+        }
+        if ((threadIdx.x & 63u) == 0u) part[k][threadIdx.x / 64u] = st[k];
     }
-    const double sign = qdot(qsub(p, hit.p), pn) >= 0.0 ? 1.0 : -1.0;
-    grid[v] = (float)(sign * sqrt(hit.d2));
+    __syncthreads();
+    if (threadIdx.x < 7u) {
+        const int k = (int)threadIdx.x;
+        int32_t r = part[k][0];
+        for (uint32_t w = 1; w < kMB / 64; ++w)
+            r = k == 0 ? r + part[k][w] : (k < 4 ? min(r, part[k][w]) : max(r, part[k][w]));
+        int32_t marked = 0;
+        for (uint32_t w = 0; w < kMB / 64; ++w) marked += part[0][w];
+        if (marked > 0) {
+            int32_t* slot = dirty + (size_t)k * kSpread + blockIdx.x % kSpread;
+            if (k == 0) atomicAdd(slot, r);
+            else if (k < 4) atomicMin(slot, r);
+            else atomicMax(slot, r);
+        }
+    }
 }
 
 // updateGeometryAndRefit, first half (SDFBoundary.cpp:438-452): corners from
@@ -264,50 +303,44 @@ __global__ __launch_bounds__(kRefitBlock) void k_refit(DNode* __restrict__ nodes
 
 // refitDirty steps 1-2 (SDFBoundary.cpp:480-525): every vertex that moved by
 // more than move_eps marks the voxels of a band around its old and its new
-// position.  dirty[0] counts newly marked voxels, dirty[1..3] / [4..6] the box.
-__device__ void mark_band(const double* w, double band, const double* origin, double cell, const int32_t* dims,
-                          uint32_t* __restrict__ mask, int32_t* __restrict__ dirty) {
+// position.  One wave per vertex; its lanes split the band's box.  Marks are
+// plain stores of 1 (no value is read back); k_sdf counts and clears them.
+__device__ void mark_band(const double* w, double band, const MeshView& m, uint32_t lane,
+                          uint32_t* __restrict__ mask) {
     int32_t b0[3], b1[3];
     for (int k = 0; k < 3; ++k) {
-        const double lo = ((w[k] - band) - origin[k]) / cell;
-        const double hi = ((w[k] + band) - origin[k]) / cell;
-        const double fl = fmin(fmax(floor(lo), -1.0), (double)dims[k]);
-        const double cl = fmin(fmax(ceil(hi), -1.0), (double)dims[k]);
-        b0[k] = min(max((int32_t)fl, 0), dims[k] - 1);
-        b1[k] = min(max((int32_t)cl, 0), dims[k] - 1);
+        const double lo = ((w[k] - band) - m.origin[k]) / m.cell;
+        const double hi = ((w[k] + band) - m.origin[k]) / m.cell;
+        const double fl = fmin(fmax(floor(lo), -1.0), (double)m.dims[k]);
+        const double cl = fmin(fmax(ceil(hi), -1.0), (double)m.dims[k]);
+        b0[k] = min(max((int32_t)fl, 0), m.dims[k] - 1);
+        b1[k] = min(max((int32_t)cl, 0), m.dims[k] - 1);
     }
-    const uint32_t W = (uint32_t)dims[0], H = (uint32_t)dims[1];
-    for (int32_t z = b0[2]; z <= b1[2]; ++z)
-        for (int32_t y = b0[1]; y <= b1[1]; ++y)
-            for (int32_t x = b0[0]; x <= b1[0]; ++x) {
-                const size_t v = (size_t)x + W * ((size_t)y + (size_t)H * (size_t)z);
-                if (atomicExch(&mask[v], 1u) == 0u) atomicAdd(&dirty[0], 1);
-            }
-    for (int k = 0; k < 3; ++k) {
-        atomicMin(&dirty[1 + k], b0[k]);
-        atomicMax(&dirty[4 + k], b1[k]);
+    const uint32_t nx = (uint32_t)(b1[0] - b0[0] + 1), ny = (uint32_t)(b1[1] - b0[1] + 1);
+    const uint32_t nz = (uint32_t)(b1[2] - b0[2] + 1);
+    const size_t W = (size_t)m.dims[0], H = (size_t)m.dims[1];
+    for (uint32_t t = lane; t < nx * ny * nz; t += 64u) {
+        const size_t x = (size_t)b0[0] + t % nx, y = (size_t)b0[1] + (t / nx) % ny, z = (size_t)b0[2] + t / (nx * ny);
+        __hip_atomic_store(&mask[x + W * (y + H * z)], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
 __global__ __launch_bounds__(kMB) void k_mark(const double* __restrict__ old_v, const double* __restrict__ new_v,
                                               uint32_t nv, double band, double move_eps2, MeshView m,
-                                              uint32_t* __restrict__ mask, int32_t* __restrict__ dirty) {
-    const uint32_t i = mgid();
-    if (i >= nv) return;
+                                              uint32_t* __restrict__ mask) {
+    const uint32_t i = mgid() / 64u, lane = threadIdx.x & 63u;
+    if (i >= nv) return;  // the wave's vertex
     const Q3 nq = q3(new_v + 3 * (size_t)i), oq = q3(old_v + 3 * (size_t)i);
     const Q3 d = qsub(nq, oq);
     if (qdot(d, d) <= move_eps2) return;
-    mark_band(old_v + 3 * (size_t)i, band, m.origin, m.cell, m.dims, mask, dirty);
-    mark_band(new_v + 3 * (size_t)i, band, m.origin, m.cell, m.dims, mask, dirty);
+    mark_band(old_v + 3 * (size_t)i, band, m, lane, mask);
+    mark_band(new_v + 3 * (size_t)i, band, m, lane, mask);
 }
 
-__global__ void k_dirty_init(int32_t* __restrict__ dirty) {
-    if (threadIdx.x == 0) {
-        dirty[0] = 0;
-        for (int k = 0; k < 3; ++k) {
-            dirty[1 + k] = 0x7fffffff;
-            dirty[4 + k] = (int32_t)0x80000000;
-        }
+__global__ __launch_bounds__(kMB) void k_dirty_init(int32_t* __restrict__ dirty) {
+    for (uint32_t w = threadIdx.x; w < kDirtyWords; w += kMB) {
+        const uint32_t k = w / kSpread;
+        dirty[w] = k == 0 ? 0 : (k < 4 ? INT32_MAX : INT32_MIN);
     }
 }
 
@@ -552,7 +585,7 @@ hipError_t record_ready(fsi_mesh* m) { return hipEventRecord(m->ready, m->stream
 
 hipError_t launch_sdf(fsi_mesh* m, bool masked) {
     hipLaunchKernelGGL(k_sdf, mgrid(m->voxels), dim3(kMB), 0, m->stream, view(m), m->grid,
-                       masked ? m->mask : nullptr, (uint32_t)m->voxels);
+                       masked ? m->mask : nullptr, (uint32_t)m->voxels, m->dirty);
     return hipGetLastError();
 }
 
@@ -786,7 +819,7 @@ int fsi_mesh_create(const double* verts, uint32_t nverts, const int32_t* faces, 
     ALLOC(m->level_off, (size_t)levels + 1);
     ALLOC(m->grid, m->voxels);
     ALLOC(m->mask, m->voxels);
-    ALLOC(m->dirty, 8);
+    ALLOC(m->dirty, kDirtyWords);
     ALLOC(m->seg, (size_t)1 << bits);
     ALLOC(m->forces, nverts);
     ALLOC(m->counters, 2);
@@ -885,9 +918,9 @@ int fsi_mesh_refit_dirty(fsi_mesh* m, const double* verts, double band_radius, d
     if (!(band_radius >= 0.0) || !(move_eps >= 0.0)) return fail(FSI_ERR_INVALID_ARG, "negative band or eps");
     hipStream_t st = m->stream;
     FSI_HIP_TRY(hipMemcpyAsync(m->verts_in, verts, sizeof(double) * 3 * m->nv, hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(k_dirty_init, dim3(1), dim3(64), 0, st, m->dirty);
-    hipLaunchKernelGGL(k_mark, mgrid(m->nv), dim3(kMB), 0, st, m->verts_build, m->verts_in, m->nv, band_radius,
-                       move_eps * move_eps, view(m), m->mask, m->dirty);
+    hipLaunchKernelGGL(k_dirty_init, dim3(1), dim3(kMB), 0, st, m->dirty);
+    hipLaunchKernelGGL(k_mark, mgrid(64 * (size_t)m->nv), dim3(kMB), 0, st, m->verts_build, m->verts_in, m->nv,
+                       band_radius, move_eps * move_eps, view(m), m->mask);
     hipLaunchKernelGGL(k_update_tris, mgrid(m->nt), dim3(kMB), 0, st, m->verts_in, m->tris, m->norms, m->nt);
     hipLaunchKernelGGL(k_refit, dim3(1), dim3(kRefitBlock), 0, st, m->nodes, m->tris, m->level_nodes, m->level_off,
                        m->levels);
@@ -896,9 +929,15 @@ int fsi_mesh_refit_dirty(fsi_mesh* m, const double* verts, double band_radius, d
     FSI_HIP_TRY(hipMemcpyAsync(m->verts, m->verts_in, sizeof(double) * 3 * m->nv, hipMemcpyDeviceToDevice, st));
     FSI_HIP_TRY(hipMemcpyAsync(m->verts_build, m->verts_in, sizeof(double) * 3 * m->nv, hipMemcpyDeviceToDevice, st));
     FSI_HIP_TRY(record_ready(m));
-    int32_t d[8];
-    FSI_HIP_TRY(hipMemcpyAsync(d, m->dirty, sizeof(d), hipMemcpyDeviceToHost, st));
+    std::vector<int32_t> w(kDirtyWords);
+    FSI_HIP_TRY(hipMemcpyAsync(w.data(), m->dirty, sizeof(int32_t) * kDirtyWords, hipMemcpyDeviceToHost, st));
     FSI_HIP_TRY(hipStreamSynchronize(st));
+    int32_t d[7] = {0, INT32_MAX, INT32_MAX, INT32_MAX, INT32_MIN, INT32_MIN, INT32_MIN};
+    for (uint32_t q = 0; q < kSpread; ++q) {
+        d[0] += w[q];
+        for (int k = 1; k < 4; ++k) d[k] = std::min(d[k], w[k * kSpread + q]);
+        for (int k = 4; k < 7; ++k) d[k] = std::max(d[k], w[k * kSpread + q]);
+    }
     if (dirty_voxels) *dirty_voxels = (uint32_t)d[0];
     for (int k = 0; k < 3; ++k) {
         if (dirty_min) dirty_min[k] = d[0] ? d[1 + k] : 0;
