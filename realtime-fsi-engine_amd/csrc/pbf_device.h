@@ -127,11 +127,28 @@ __device__ __forceinline__ float poly6_r2(const StepConst& c, float r2) {
     return c.poly6 * diff * diff * diff;
 }
 
-// Spiky gradient (CS_ComputeLambdas.glsl:83-92).
+// Correctly rounded square root of x >= 2^-96 (finite): the hardware
+// estimate s (within 1 ulp) is replaced by its neighbour s -/+ 1 ulp when the
+// residual x - s' * s, formed exactly by an FMA, shows that the neighbour is
+// nearer.  This is the sequence the compiler emits for an IEEE sqrtf less the
+// scaling of inputs below 2^-96 and the pass-through of 0 and infinity, so it
+// returns the same bits on that range (checked for every float in it by the
+// test library of this package).
+__device__ __forceinline__ float sqrt_rn_normal(float x) {
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float lo = __uint_as_float(__float_as_uint(s) - 1u);
+    const float hi = __uint_as_float(__float_as_uint(s) + 1u);
+    float r = __builtin_fmaf(-lo, s, x) <= 0.0f ? lo : s;
+    r = __builtin_fmaf(-hi, s, x) > 0.0f ? hi : r;
+    return r;
+}
+
+// Spiky gradient (CS_ComputeLambdas.glsl:83-92).  r2 >= 1e-10 here, inside
+// the range of sqrt_rn_normal.
 __device__ __forceinline__ V3 spiky(const StepConst& c, V3 r, float r2) {
     const float h2 = c.h * c.h;
     if (r2 > h2 || r2 < 1e-10f) return mk(0.0f, 0.0f, 0.0f);
-    const float dist = sqrtf(r2);
+    const float dist = sqrt_rn_normal(r2);
     const float diff = c.h - dist;
     const float scalar = -c.spiky * diff * diff / dist;
     return scl(r, scalar);

@@ -52,45 +52,56 @@ constexpr uint32_t kFlushAt = 16;
 // bucket, if any lane holds kFlushAt entries or more, every lane hands its
 // entries to flush(first, count) and empties its column, so the expensive
 // arithmetic of phase 2 runs with the lanes of the wave together; a lane whose
-// column fills inside a bucket flushes alone.  Entries reach flush in visit
-// order, numbered from `first`.  Returns the lane's neighbour count.
+// column has no room for the next group of four candidates flushes alone.
+// Entries reach flush in visit order, numbered from `first`.  Returns the
+// lane's neighbour count.  X holds n positions (n < 2^28).
 template <bool TAPS, class Flush>
 __device__ __forceinline__ uint32_t collect_flush(V3 x, float h, const Domain& dm, const int2* __restrict__ cells,
-                                                  const float4* __restrict__ X, uint32_t self, uint32_t* col,
-                                                  Flush&& flush, uint32_t& ncand) {
+                                                  const float4* __restrict__ X, uint32_t n, uint32_t self,
+                                                  uint32_t* col, Flush&& flush, uint32_t& ncand) {
     const float h2 = h * h;
     int32_t cx, cy, cz;
     cell_of(x, h, cx, cy, cz);
     const BucketWalk bw(cx, cy, cz, dm);
+    // Candidate positions are read through a buffer resource over X: 32-bit
+    // offsets, and the loads of a group of four that run past the end of X
+    // return zeros (the hardware bounds check) where the indexes would
+    // otherwise be clamped.  Those candidates are not valid and not counted.
+    const __amdgpu_buffer_rsrc_t xr =
+        __builtin_amdgcn_make_buffer_rsrc((void*)X, (short)0, (int)(n * (uint32_t)sizeof(float4)), 0x00020000);
+    auto load = [&](uint32_t k) {
+        const auto v = __builtin_amdgcn_raw_buffer_load_b96(xr, (int)(k * (uint32_t)sizeof(float4)), 0, 0);
+        return mk(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]));
+    };
     int2 se = cells[bw.bucket(0)];
     uint32_t cnt = 0, total = 0;
     // branch-free candidate test: the slot is always written to the column
     // and the fill level advances only for a neighbour (a valid slot other
     // than the particle's own, inside the support)
-    auto visit = [&](uint32_t k, float4 xk, bool valid) {
-        const V3 r = sub(x, xyz(xk));
+    auto visit = [&](uint32_t k, V3 xk, bool valid) {
+        const V3 r = sub(x, xk);
         const bool other = valid && k != self;
         if (TAPS) ncand += other ? 1u : 0u;
         col[cnt * kBlock] = k;
         cnt += (other && dot(r, r) <= h2) ? 1u : 0u;
-        if (cnt == kLdsCap) {
-            flush(total, cnt);
-            total += cnt;
-            cnt = 0;
-        }
     };
 #pragma unroll 1
     for (int q = 0; q < 27; ++q) {
         int2 nxt = make_int2(-1, -1);
         if (q < 26) nxt = cells[bw.bucket(q + 1)];
-        const uint32_t end = (uint32_t)se.y, last = end - 1u;
+        // an empty bucket is (-1, -1): no iterations
+        const uint32_t end = (uint32_t)se.y;
         for (uint32_t k = (uint32_t)se.x; k < end; k += 4) {
-            const uint32_t k1 = min(k + 1, last), k2 = min(k + 2, last), k3 = min(k + 3, last);
-            const float4 a0 = X[k], a1 = X[k1], a2 = X[k2], a3 = X[k3];
+            if (cnt > kLdsCap - 4u) {
+                flush(total, cnt);
+                total += cnt;
+                cnt = 0;
+            }
+            const V3 a0 = load(k), a1 = load(k + 1), a2 = load(k + 2), a3 = load(k + 3);
             visit(k, a0, true);
-            visit(k + 1, a1, k + 1 <= last);
-            visit(k + 2, a2, k + 2 <= last);
-            visit(k + 3, a3, k + 3 <= last);
+            visit(k + 1, a1, k + 1 < end);
+            visit(k + 2, a2, k + 2 < end);
+            visit(k + 3, a3, k + 3 < end);
         }
         se = nxt;
         if (__any(cnt >= kFlushAt)) {
@@ -134,8 +145,8 @@ __device__ __forceinline__ void walk_list(uint32_t cnt, const float4* __restrict
 // Pairs with r^2 > h^2 are skipped: in the reference they add exact zeros.
 template <int V, bool TAPS, class Body>
 __device__ __forceinline__ void gather(V3 xi, float h, const Domain& dm, const int2* __restrict__ cells,
-                                       const float4* __restrict__ X, uint32_t self, uint32_t* col, Body&& body,
-                                       uint32_t& ncand, uint32_t& nnbr) {
+                                       const float4* __restrict__ X, uint32_t n, uint32_t self, uint32_t* col,
+                                       Body&& body, uint32_t& ncand, uint32_t& nnbr) {
     if constexpr (V == 0) {
         const float h2 = h * h;
         scan_buckets(xi, h, dm, cells, X, self, [&](uint32_t k, float4 xk, V3 r, float r2) {
@@ -146,7 +157,7 @@ __device__ __forceinline__ void gather(V3 xi, float h, const Domain& dm, const i
             }
         });
     } else {
-        nnbr = collect_flush<TAPS>(xi, h, dm, cells, X, self, col, [&](uint32_t, uint32_t cnt) {
+        nnbr = collect_flush<TAPS>(xi, h, dm, cells, X, n, self, col, [&](uint32_t, uint32_t cnt) {
             walk_list(cnt, X, [&](uint32_t e) { return col[e * kBlock]; }, [&](uint32_t j, float4 xj) {
                 const V3 r = sub(xi, xyz(xj));
                 body(j, xj, r, dot(r, r));
@@ -302,11 +313,11 @@ __global__ __launch_bounds__(kBlock) void k_lambda(StepConst c, const uint32_t* 
         gi = add(gi, gw);
     };
     if constexpr (V == 0) {
-        gather<0, TAPS>(xi, c.h, domain_of<B>(c, k), cells, P, k, nullptr, body, ncand, nnbr);
+        gather<0, TAPS>(xi, c.h, domain_of<B>(c, k), cells, P, c.n, k, nullptr, body, ncand, nnbr);
     } else {
         __shared__ uint32_t lst[kLdsCap * kBlock];
         uint32_t* col = &lst[threadIdx.x];
-        nnbr = collect_flush<TAPS>(xi, c.h, domain_of<B>(c, k), cells, P, k, col, [&](uint32_t first, uint32_t cnt) {
+        nnbr = collect_flush<TAPS>(xi, c.h, domain_of<B>(c, k), cells, P, c.n, k, col, [&](uint32_t first, uint32_t cnt) {
 #pragma unroll 1
             for (uint32_t e = 0; e < cnt; ++e)
                 if (first + e < (uint32_t)kNbrCap) list[(size_t)t * kNbrCap + first + e] = col[e * kBlock];
@@ -445,7 +456,7 @@ __global__ __launch_bounds__(kBlock) void k_integrate(StepConst c, const uint32_
     V3 curl = mk(0.0f, 0.0f, 0.0f);
     uint32_t ncand = 0, nnbr = 0;
     __shared__ uint32_t lst[kLdsCap * kBlock];
-    gather<V, false>(xp, c.h, domain_of<B>(c, k), cells, P, k, &lst[threadIdx.x], [&](uint32_t j, float4, V3 r, float r2) {
+    gather<V, false>(xp, c.h, domain_of<B>(c, k), cells, P, c.n, k, &lst[threadIdx.x], [&](uint32_t j, float4, V3 r, float r2) {
         const V3 vij = sub(xyz(vs[j]), old_vel);
         if (do_visc) {
             const float w = poly6_r2(c, r2);
@@ -532,7 +543,7 @@ __global__ __launch_bounds__(kBlock) void k_vorticity(StepConst c, const uint32_
     V3 eta = mk(0.0f, 0.0f, 0.0f);
     uint32_t ncand = 0, nnbr = 0;
     __shared__ uint32_t lst[kLdsCap * kBlock];
-    gather<V, false>(pi, c.h, domain_of<B>(c, k), cells, posS, k, &lst[threadIdx.x], [&](uint32_t j, float4, V3 r, float r2) {
+    gather<V, false>(pi, c.h, domain_of<B>(c, k), cells, posS, c.n, k, &lst[threadIdx.x], [&](uint32_t j, float4, V3 r, float r2) {
         const V3 gw = spiky(c, r, r2);
         const float om = len(xyz(curl[j]));
         eta = add(eta, scl(gw, om));
