@@ -231,27 +231,25 @@ __device__ __forceinline__ V3 sdf_gradient(const Domain& c, V3 p) {
 }
 
 // SDF projection shared by CS_ApplyDeltaP.glsl:108-121 and CS_Integrate.glsl:216-240.
+// The shaders sample the gradient before testing the distance; its six
+// samples only matter when the distance test passes, so they are taken
+// after it (same results: a failed test returns false whatever the gradient).
 __device__ __forceinline__ bool sdf_project(const Domain& c, V3& p, V3& nrm) {
     const float pad = c.sdf_pad;
     const float d = sdf_query(c, p);
+    const bool container = c.sdf_mode == 1u;
+    if (container ? !(d > -pad) : !(d < pad)) return false;
     const V3 g = sdf_gradient(c, p);
     const float l = len(g);
-    if (l > 1e-6f) {
-        if (c.sdf_mode == 1u) {
-            if (d > -pad) {
-                nrm = dvs(mk(-g.x, -g.y, -g.z), l);
-                p = add(p, scl(nrm, d + pad));
-                return true;
-            }
-        } else {
-            if (d < pad) {
-                nrm = dvs(g, l);
-                p = add(p, scl(nrm, pad - d));
-                return true;
-            }
-        }
+    if (!(l > 1e-6f)) return false;
+    if (container) {
+        nrm = dvs(mk(-g.x, -g.y, -g.z), l);
+        p = add(p, scl(nrm, d + pad));
+    } else {
+        nrm = dvs(g, l);
+        p = add(p, scl(nrm, pad - d));
     }
-    return false;
+    return true;
 }
 
 }  // namespace fsi

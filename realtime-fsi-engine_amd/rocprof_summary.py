@@ -25,6 +25,11 @@ PASS_OF = [
     ("k_clear_cells", "tables"), ("k_build_cells", "tables"), ("k_reorder", "tables"), ("k_order", "tables"),
     ("k_bin", "tables"), ("k_fill", "tables"), ("k_lod", "lod"), ("k_lambda", "lambda"), ("k_deltap", "deltap"),
     ("k_apply_sdf", "deltap"), ("k_integrate", "integrate"), ("k_vorticity", "vorticity"), ("k_contact", "contacts"),
+    ("k_seg_", "contacts"), ("k_vertex_sum", "contacts"), ("k_sdf", "mesh"), ("k_mark", "mesh"), ("k_refit", "mesh"),
+    ("k_update_tris", "mesh"), ("k_dirty_init", "mesh"), ("k_edges", "adjoint lists"), ("k_transpose", "adjoint lists"),
+    ("k_tstats", "adjoint lists"), ("k_zero_tcnt", "adjoint lists"), ("k_rec_", "adjoint replay"),
+    ("k_predict64", "adjoint replay"), ("k_reorder64", "adjoint replay"), ("k_rev_", "adjoint reverse"),
+    ("k_absmax", "adjoint reverse"), ("k_compare", "adjoint replay"), ("k_reduce2", "adjoint reverse"),
 ]
 
 
