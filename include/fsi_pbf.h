@@ -188,6 +188,16 @@ int fsi_pbf_particle_count(fsi_pbf* h, uint32_t* n);
  * current state (all 48 bytes per record) into out[0..n); cap must be >= n. */
 int fsi_pbf_read_particles(fsi_pbf* h, fsi_particle* out, uint32_t cap, uint32_t* n);
 
+/* PBFluids::bindParticlesForRendering  (include/fluid/PBFluids.h:143): the
+ * particle buffer the renderer reads, the std430 `Particle` records
+ * {vec4 pos, vec4 predPos, vec4 vel} at binding 0 of
+ * resources/shaders/graphics/VS_Particle.glsl:3-12.  Writes the current state
+ * as n 48-byte records (pads included, predPos as last set) into device_out
+ * (device memory of n * 48 bytes, 16-byte aligned), on the device, enqueued
+ * after every step already enqueued; if stream is not NULL it waits for the
+ * write. */
+int fsi_pbf_write_render_buffer(fsi_pbf* h, void* device_out, void* stream);
+
 /* PBFluids::setCollisionPadding  (include/fluid/PBFluids.h:108).  Like the
  * reference it takes effect at the next set_bounds / set_params. */
 int fsi_pbf_set_collision_padding(fsi_pbf* h, float padding);
@@ -351,6 +361,30 @@ int fsi_batch_read_timing(fsi_batch* b, fsi_pbf_timing* out);
 int fsi_scene_lattice(uint32_t nx, uint32_t ny, uint32_t nz, const float origin[3],
                       float spacing, float jitter, uint32_t seed, const float v0[3],
                       fsi_particle* out);
+
+/* spawnParticles  (src/main.cpp:160-188): cfg->particle_count particles in
+ * [spawn_min, spawn_max].  Grid mode: x, y, z stepped by max(1e-6, spacing)
+ * from spawn_min in FP32 (z outermost, x fastest) while inside spawn_max and
+ * below the count.  Random mode: std::mt19937 seeded 1337 and
+ * std::uniform_real_distribution<float>(0, 1), three draws (x, y, z) per
+ * particle, inside [spawn_min + spacing, spawn_max - spacing].  predPos = pos,
+ * vel = initial_velocity, pads 0.  *count gets the number spawned (the grid
+ * can hold fewer than particle_count); out may be NULL to ask for it. */
+int fsi_spawn_particles(const fsi_fluid_config* cfg, fsi_particle* out, uint32_t cap, uint32_t* count);
+
+/* loadOFF  (src/main.cpp:364-415): an OFF file, polygons fan-triangulated.
+ * Comment (#) and blank lines are skipped; vertex lines may carry extra
+ * values (normals, colours), which are ignored.  Two calls: with verts and
+ * faces NULL only *nverts and *nfaces (triangles) are returned; then
+ * verts (3 * nverts doubles) and faces (3 * nfaces ints) are filled.
+ * FSI_ERR_INVALID_ARG if the file cannot be opened or is not an OFF file. */
+int fsi_load_off(const char* path, double* verts, uint32_t vcap, int32_t* faces, uint32_t fcap,
+                 uint32_t* nverts, uint32_t* nfaces);
+
+/* fitMeshToDomain  (src/main.cpp:419-432): centre the mesh's bounding box on
+ * `center` and scale its largest extent (at least 1e-9) to target_size, in
+ * place. */
+int fsi_mesh_fit_to_domain(double* verts, uint32_t nverts, const float center[3], double target_size);
 
 /* Icosphere: subdivisions of a unit icosahedron projected on the sphere
  * (centre c, radius r).  Sizes: V = 10*4^s + 2, F = 20*4^s. */

@@ -38,7 +38,8 @@ struct fsi_pbf {
     bool interacting = false;
     float ipos[3] = {0, 0, 0};
     float cam[3] = {0, 0, 0};
-    std::vector<float> host_pred;  // predPos.xyz + pad1 per particle (never touched on device)
+    std::vector<float> host_pred;  // predPos.xyz + pad1 per particle (never touched by the step)
+    float4* pred = nullptr;        // the same on the device, for the render buffer
     // contacts
     float4* contacts = nullptr;
     uint32_t* contact_count = nullptr;
@@ -195,6 +196,7 @@ int fsi_pbf_destroy(fsi_pbf* h) {
     if (h->core.stream) (void)hipStreamSynchronize(h->core.stream);
     dev_free(&h->contacts);
     dev_free(&h->contact_count);
+    dev_free(&h->pred);
     dev_free(&h->sdf_grid);
     core_destroy(h->core);
     delete h;
@@ -241,6 +243,7 @@ int fsi_pbf_set_particles(fsi_pbf* h, const fsi_particle* ps, uint32_t n) {
         FSI_HIP_TRY(core_alloc_particles(c, n));
         FSI_HIP_TRY(dev_alloc(&h->contacts, 2 * (size_t)n));
         FSI_HIP_TRY(dev_alloc(&h->contact_count, 1));
+        FSI_HIP_TRY(dev_alloc(&h->pred, n));
     }
     c.d.n = n;
     h->cfg.particle_count = n;
@@ -264,6 +267,8 @@ int fsi_pbf_set_particles(fsi_pbf* h, const fsi_particle* ps, uint32_t n) {
     }
     FSI_HIP_TRY(hipMemcpyAsync(c.d.pos, p4.data(), p4.size() * sizeof(float), hipMemcpyHostToDevice, c.stream));
     FSI_HIP_TRY(hipMemcpyAsync(c.d.vel, v4.data(), v4.size() * sizeof(float), hipMemcpyHostToDevice, c.stream));
+    FSI_HIP_TRY(hipMemcpyAsync(h->pred, h->host_pred.data(), h->host_pred.size() * sizeof(float),
+                               hipMemcpyHostToDevice, c.stream));
     // SolverBuffer starts zeroed (src/fluid/PBFluids.cpp:123-124)
     FSI_HIP_TRY(hipMemsetAsync(c.d.P, 0, sizeof(float4) * n, c.stream));
     FSI_HIP_TRY(hipMemsetAsync(c.d.PL, 0, sizeof(float4) * n, c.stream));
@@ -303,6 +308,42 @@ int fsi_pbf_read_particles(fsi_pbf* h, fsi_particle* out, uint32_t cap, uint32_t
         q.pad0 = p4[4 * (size_t)i + 3];
         q.pad1 = h->host_pred[4 * (size_t)i + 3];
         q.pad2 = v4[4 * (size_t)i + 3];
+    }
+    return FSI_OK;
+}
+
+namespace {
+// 48-byte Particle records {pos, predPos, vel} (three float4 each, id order).
+__global__ __launch_bounds__(256) void k_pack_render(uint32_t n, const float4* __restrict__ pos,
+                                                     const float4* __restrict__ pred,
+                                                     const float4* __restrict__ vel, float4* __restrict__ out) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= n) return;
+    out[3 * (size_t)i] = pos[i];
+    out[3 * (size_t)i + 1] = pred[i];
+    out[3 * (size_t)i + 2] = vel[i];
+}
+}  // namespace
+
+int fsi_pbf_write_render_buffer(fsi_pbf* h, void* device_out, void* stream) {
+    int rc = bind(h);
+    if (rc) return rc;
+    const uint32_t n = h->core.d.n;
+    if (n == 0) return FSI_OK;
+    if (!device_out) return fail(FSI_ERR_INVALID_ARG, "device_out is NULL");
+    if ((uintptr_t)device_out % alignof(float4) != 0) return fail(FSI_ERR_INVALID_ARG, "device_out is not 16-byte aligned");
+    hipStream_t st = h->core.stream;
+    hipLaunchKernelGGL(k_pack_render, dim3((n + 255u) / 256u), dim3(256), 0, st, n, h->core.d.pos, h->pred,
+                       h->core.d.vel, (float4*)device_out);
+    FSI_HIP_TRY(hipGetLastError());
+    if (stream) {
+        // the caller's stream (e.g. the renderer's) waits for the pack
+        hipEvent_t ev = nullptr;
+        FSI_HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        hipError_t e = hipEventRecord(ev, st);
+        if (e == hipSuccess) e = hipStreamWaitEvent((hipStream_t)stream, ev, 0);
+        (void)hipEventDestroy(ev);
+        FSI_HIP_TRY(e);
     }
     return FSI_OK;
 }

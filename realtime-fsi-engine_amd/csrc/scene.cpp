@@ -1,5 +1,6 @@
-// scene.cpp — host helpers of the C-ABI: lattice spawn, icosphere mesh and the
-// FP64 signed-distance grid of a triangle mesh.
+// scene.cpp — host helpers of the C-ABI: lattice spawn, the reference's
+// spawnParticles, the OFF loader and mesh fitting, icosphere mesh and the FP64
+// signed-distance grid of a triangle mesh.
 //
 // The SDF build follows SDFBoundary::buildFromMesh (src/fluid/SDFBoundary.cpp:278-436):
 //   - grid origin = bbox - 5 cells, dims = ceil(span / cell)             (:296-308)
@@ -16,7 +17,10 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <fstream>
 #include <map>
+#include <random>
+#include <sstream>
 #include <string>
 #include <thread>
 #include <vector>
@@ -25,6 +29,7 @@
 #include "mesh_bvh.h"
 
 namespace fsi {
+int fail(int code, const std::string& msg);  // pbf_core.hip: records fsi_last_error()
 namespace {
 
 // ---------------- MT19937 (Matsumoto & Nishimura, 1998) ----------------
@@ -55,7 +60,89 @@ struct Mt19937 {
 using mesh::D3;
 using mesh::norm;
 
-thread_local std::string g_scene_error;
+// Particle records of spawnParticles (src/main.cpp:160-188), appended to out
+// while out->size() < limit.
+void spawn(const fsi_fluid_config& c, std::vector<fsi_particle>& out) {
+    auto record = [&](float x, float y, float z) {
+        fsi_particle pt;
+        std::memset(&pt, 0, sizeof(pt));
+        pt.pos[0] = x;
+        pt.pos[1] = y;
+        pt.pos[2] = z;
+        for (int a = 0; a < 3; ++a) {
+            pt.pred_pos[a] = pt.pos[a];
+            pt.vel[a] = c.initial_velocity[a];
+        }
+        out.push_back(pt);
+    };
+    std::mt19937 rng(1337);
+    if (c.spawn_random) {
+        // frand01: uniform_real_distribution<float>(0, 1) on the same engine
+        std::uniform_real_distribution<float> d(0.f, 1.f);
+        const float lo[3] = {c.spawn_min[0] + c.spacing, c.spawn_min[1] + c.spacing, c.spawn_min[2] + c.spacing};
+        const float hi[3] = {c.spawn_max[0] - c.spacing, c.spawn_max[1] - c.spacing, c.spawn_max[2] - c.spacing};
+        for (uint32_t i = 0; i < c.particle_count; ++i) {
+            const float x = lo[0] + (hi[0] - lo[0]) * d(rng);
+            const float y = lo[1] + (hi[1] - lo[1]) * d(rng);
+            const float z = lo[2] + (hi[2] - lo[2]) * d(rng);
+            record(x, y, z);
+        }
+    } else {
+        const float dx = std::max(1e-6f, c.spacing);
+        const size_t limit = c.particle_count;
+        for (float z = c.spawn_min[2]; z <= c.spawn_max[2] && out.size() < limit; z += dx)
+            for (float y = c.spawn_min[1]; y <= c.spawn_max[1] && out.size() < limit; y += dx)
+                for (float x = c.spawn_min[0]; x <= c.spawn_max[0] && out.size() < limit; x += dx) record(x, y, z);
+    }
+}
+
+// loadOFF (src/main.cpp:364-415): comment (#) and blank lines skipped before
+// the header, which must contain "OFF", and before the counts line; each
+// vertex line gives its first three numbers (the rest of the line, normals or
+// colours, is ignored); each face "n i0 .. i(n-1)" is fan-triangulated
+// (i0, ij, ij+1).  Lines missing at the end of the file read as empty: their
+// vertices are 0 and their faces add nothing.
+int load_off(const char* path, std::vector<std::array<double, 3>>& verts, std::vector<std::array<int32_t, 3>>& faces) {
+    std::ifstream f(path);
+    if (!f) return fail(FSI_ERR_INVALID_ARG, std::string("cannot open: ") + path);
+    std::string header;
+    while (std::getline(f, header)) {
+        if (header.empty() || header[0] == '#') continue;
+        while (!header.empty() && (header.back() == '\r' || header.back() == ' ')) header.pop_back();
+        break;
+    }
+    if (header.find("OFF") == std::string::npos) return fail(FSI_ERR_INVALID_ARG, std::string("not an OFF file: ") + path);
+    int nv = 0, nf = 0, ne = 0;
+    std::string line;
+    while (std::getline(f, line)) {
+        if (line.empty() || line[0] == '#') continue;
+        std::istringstream ss(line);
+        if (ss >> nv >> nf >> ne) break;
+    }
+    if (nv < 0 || nf < 0) return fail(FSI_ERR_INVALID_ARG, std::string("negative counts in ") + path);
+    auto next_line = [&]() {
+        while (std::getline(f, line))
+            if (!line.empty() && line[0] != '#') return;
+    };
+    verts.assign((size_t)nv, std::array<double, 3>{0.0, 0.0, 0.0});
+    for (int i = 0; i < nv; ++i) {
+        next_line();
+        std::istringstream ss(line);
+        ss >> verts[(size_t)i][0] >> verts[(size_t)i][1] >> verts[(size_t)i][2];
+    }
+    faces.clear();
+    faces.reserve((size_t)nf);
+    for (int i = 0; i < nf; ++i) {
+        next_line();
+        std::istringstream ss(line);
+        int n = 0;
+        ss >> n;
+        std::vector<int> poly((size_t)std::max(n, 0), 0);
+        for (int& v : poly) ss >> v;
+        for (int j = 1; j + 1 < n; ++j) faces.push_back({poly[0], poly[(size_t)j], poly[(size_t)j + 1]});
+    }
+    return FSI_OK;
+}
 
 }  // namespace
 }  // namespace fsi
@@ -63,6 +150,58 @@ thread_local std::string g_scene_error;
 using namespace fsi;
 
 extern "C" {
+
+int fsi_spawn_particles(const fsi_fluid_config* cfg, fsi_particle* out, uint32_t cap, uint32_t* count) {
+    if (!cfg || !count) return fail(FSI_ERR_INVALID_ARG, "NULL argument");
+    std::vector<fsi_particle> ps;
+    ps.reserve(std::min<uint32_t>(cfg->particle_count, 1u << 24));
+    spawn(*cfg, ps);
+    *count = (uint32_t)ps.size();
+    if (!out) return FSI_OK;
+    if (cap < ps.size()) return fail(FSI_ERR_CAPACITY, "cap smaller than the spawned count");
+    std::memcpy(out, ps.data(), ps.size() * sizeof(fsi_particle));
+    return FSI_OK;
+}
+
+int fsi_load_off(const char* path, double* verts, uint32_t vcap, int32_t* faces, uint32_t fcap, uint32_t* nverts,
+                 uint32_t* nfaces) {
+    if (!path || !nverts || !nfaces) return fail(FSI_ERR_INVALID_ARG, "NULL argument");
+    std::vector<std::array<double, 3>> v;
+    std::vector<std::array<int32_t, 3>> fc;
+    const int rc = load_off(path, v, fc);
+    if (rc) return rc;
+    *nverts = (uint32_t)v.size();
+    *nfaces = (uint32_t)fc.size();
+    if (!verts && !faces) return FSI_OK;
+    if (!verts || !faces) return fail(FSI_ERR_INVALID_ARG, "verts and faces must both be given");
+    if (vcap < v.size() || fcap < fc.size()) return fail(FSI_ERR_CAPACITY, "verts or faces too small");
+    for (size_t i = 0; i < v.size(); ++i)
+        for (int k = 0; k < 3; ++k) verts[3 * i + k] = v[i][k];
+    for (size_t i = 0; i < fc.size(); ++i)
+        for (int k = 0; k < 3; ++k) faces[3 * i + k] = fc[i][k];
+    return FSI_OK;
+}
+
+int fsi_mesh_fit_to_domain(double* verts, uint32_t nverts, const float center[3], double target_size) {
+    if (!center || (nverts > 0 && !verts)) return fail(FSI_ERR_INVALID_ARG, "NULL argument");
+    if (nverts == 0) return FSI_OK;
+    double lo[3] = {1e30, 1e30, 1e30}, hi[3] = {-1e30, -1e30, -1e30};
+    for (uint32_t i = 0; i < nverts; ++i)
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = std::min(lo[k], verts[3 * (size_t)i + k]);
+            hi[k] = std::max(hi[k], verts[3 * (size_t)i + k]);
+        }
+    const double cx = 0.5 * (lo[0] + hi[0]), cy = 0.5 * (lo[1] + hi[1]), cz = 0.5 * (lo[2] + hi[2]);
+    const double extent = std::max({hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2], 1e-9});
+    const double s = target_size / extent;
+    for (uint32_t i = 0; i < nverts; ++i) {
+        double* v = verts + 3 * (size_t)i;
+        v[0] = (v[0] - cx) * s + center[0];
+        v[1] = (v[1] - cy) * s + center[1];
+        v[2] = (v[2] - cz) * s + center[2];
+    }
+    return FSI_OK;
+}
 
 int fsi_scene_lattice(uint32_t nx, uint32_t ny, uint32_t nz, const float origin[3], float spacing, float jitter,
                       uint32_t seed, const float v0[3], fsi_particle* out) {

@@ -162,6 +162,10 @@ def load() -> C.CDLL:
         "fsi_pbf_set_particles": ([P, P, C.c_uint32], C.c_int),
         "fsi_pbf_particle_count": ([P, u32p], C.c_int),
         "fsi_pbf_read_particles": ([P, P, C.c_uint32, u32p], C.c_int),
+        "fsi_pbf_write_render_buffer": ([P, P, P], C.c_int),
+        "fsi_spawn_particles": ([C.POINTER(CFluidConfig), P, C.c_uint32, u32p], C.c_int),
+        "fsi_load_off": ([C.c_char_p, f64p, C.c_uint32, i32p, C.c_uint32, u32p, u32p], C.c_int),
+        "fsi_mesh_fit_to_domain": ([f64p, C.c_uint32, f32p, C.c_double], C.c_int),
         "fsi_pbf_set_collision_padding": ([P, C.c_float], C.c_int),
         "fsi_pbf_set_bounds": ([P, f32p, f32p, C.c_float], C.c_int),
         "fsi_pbf_set_sdf": ([P, f32p, i32p, f32p, C.c_float], C.c_int),
@@ -260,6 +264,37 @@ def lattice(nx: int, ny: int, nz: int, origin, spacing: float, jitter: float, se
     _check(load().fsi_scene_lattice(nx, ny, nz, _f3(origin), float(np.float32(spacing)), float(np.float32(jitter)),
                                     seed, _f3(v0), out.ctypes.data_as(C.c_void_p)))
     return out
+
+
+def spawn_particles(cfg: "FluidConfig") -> np.ndarray:
+    """spawnParticles (src/main.cpp:160-188) of cfg: (n, 12) float32 records."""
+    lib = load()
+    cc = cfg.to_c()
+    n = C.c_uint32(0)
+    _check(lib.fsi_spawn_particles(C.byref(cc), None, 0, C.byref(n)))
+    out = np.zeros((n.value, 12), dtype=np.float32)
+    _check(lib.fsi_spawn_particles(C.byref(cc), out.ctypes.data_as(C.c_void_p), n.value, C.byref(n)))
+    return out
+
+
+def load_off(path: str):
+    """loadOFF (src/main.cpp:364-415): (verts (V, 3) float64, faces (F, 3) int32)."""
+    lib = load()
+    nv, nf = C.c_uint32(0), C.c_uint32(0)
+    p = os.fsencode(path)
+    _check(lib.fsi_load_off(p, None, 0, None, 0, C.byref(nv), C.byref(nf)))
+    verts = np.zeros((nv.value, 3), dtype=np.float64)
+    faces = np.zeros((nf.value, 3), dtype=np.int32)
+    _check(lib.fsi_load_off(p, _ptr(verts, C.c_double), nv.value, _ptr(faces, C.c_int32), nf.value,
+                            C.byref(nv), C.byref(nf)))
+    return verts, faces
+
+
+def fit_mesh_to_domain(verts: np.ndarray, center, target_size: float) -> np.ndarray:
+    """fitMeshToDomain (src/main.cpp:419-432) on a copy of verts."""
+    v = np.array(verts, dtype=np.float64, order="C", copy=True)
+    _check(load().fsi_mesh_fit_to_domain(_ptr(v, C.c_double), len(v), _f3(center), float(target_size)))
+    return v
 
 
 def icosphere(subdivisions: int, centre, radius: float):
@@ -413,6 +448,11 @@ class Solver:
         n = C.c_uint32(0)
         _check(self.lib.fsi_pbf_read_particles(self.h, out.ctypes.data_as(C.c_void_p), self.n, C.byref(n)))
         return out
+
+    def write_render_buffer(self, device_ptr: int, stream: int = 0):
+        """bindParticlesForRendering: the 48-byte records into device memory at
+        device_ptr (n * 48 bytes); `stream` (a hipStream_t as int) waits for it."""
+        _check(self.lib.fsi_pbf_write_render_buffer(self.h, C.c_void_p(device_ptr), C.c_void_p(stream or None)))
 
     def set_collision_padding(self, p: float):
         _check(self.lib.fsi_pbf_set_collision_padding(self.h, float(p)))

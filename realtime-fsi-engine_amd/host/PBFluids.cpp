@@ -153,6 +153,13 @@ void PBFluids::bindParticlesForRendering() const {
     if (_h) fsi_pbf_sync(_h);
 }
 
+void PBFluids::bindParticlesForRendering(void* deviceBuffer, void* stream) const {
+    if (!_h) return;
+    const int rc = fsi_pbf_write_render_buffer(_h, deviceBuffer, stream);
+    if (rc)
+        std::cerr << "PBFluids::bindParticlesForRendering: " << fsi_last_error() << " (status " << rc << ")\n";
+}
+
 // src/fluid/PBFluids.cpp:154-191
 void PBFluids::readbackParticles(std::vector<Particle>& out) {
     out.resize(_particles.size());

@@ -120,9 +120,13 @@ public:
 
     void step();
 
-    // The reference binds its particle SSBO for the render shaders; rendering
-    // is not part of this library, so this only finishes pending work.
+    // The reference binds its particle SSBO (binding 0) for the render
+    // shaders.  Without a buffer this finishes pending work; with one (device
+    // memory of particles().size() * 48 bytes, e.g. a GL/Vulkan buffer mapped
+    // for HIP) it writes the 48-byte records there on the device, and the
+    // renderer's stream, if given, waits for them.
     void bindParticlesForRendering() const;
+    void bindParticlesForRendering(void* deviceBuffer, void* stream = nullptr) const;
 
     void readbackParticles(std::vector<Particle>& out);
     void filterContactCandidates(float contactBand);
