@@ -509,6 +509,67 @@ int fsi_adj_stats(fsi_adjoint* a, uint32_t out[4]);
  * (NaN if any was NaN); `frames` pairs. */
 int fsi_adj_frame_norms(fsi_adjoint* a, float* out);
 
+/* ------------------------------------------------------------------------- */
+/* Host mesh deformation of the two-way coupling: LaplacianDeformation        */
+/* (src/deformation/LaplacianDeformation.cpp, include/deformation/            */
+/* LaplacianDeformation.h) and its per-frame driver deformStep                */
+/* (src/main.cpp:862-944).  Host code, FP64, with a project-written sparse    */
+/* LDL^T (reverse Cuthill-McKee order) where the reference uses Eigen's        */
+/* SimplicialLDLT, CHOLMOD and its IncrementalCholesky.                       */
+/* ------------------------------------------------------------------------- */
+
+typedef struct fsi_deform fsi_deform; /* opaque handle */
+
+/* The deformStep tuning (src/main.cpp:251, 300-304, 343). */
+typedef struct fsi_deform_params {
+    float stiffness;             /* force -> displacement gain (1.0)             */
+    float max_displacement;      /* per-frame push clamp (0.02)                  */
+    float max_total_disp;        /* clamp of |target - rest| (0.40)              */
+    uint32_t restore_enabled;    /* elastic pull back toward rest (1)            */
+    float restore_strength;      /* fraction of the offset removed per frame (0.05) */
+    float control_threshold;     /* |force| to enter the handle set (0.50)       */
+    float control_threshold_low; /* |force| to stay in it (0.30)                 */
+} fsi_deform_params;
+int fsi_deform_params_default(fsi_deform_params* p);
+
+/* LaplacianDeformation(mesh) + initialize() (LaplacianDeformation.cpp:7-23,
+ * 77-195): cotangent weights max(0.5 * sum cot, 1e-4), L_ii = 1,
+ * L_ij = -w_ij / sum w, delta = L V.  The given vertices are the rest pose. */
+int fsi_deform_create(const double* verts, uint32_t nverts, const int32_t* faces, uint32_t nfaces,
+                      fsi_deform** out);
+int fsi_deform_destroy(fsi_deform* d);
+/* setUseLocal / setLocalKRing / setUseIncremental (LaplacianDeformation.h:
+ * 40-66).  Defaults: local patch on, k = 3, incremental off. */
+int fsi_deform_set_mode(fsi_deform* d, int use_local, int kring, int use_incremental);
+/* addAnchor (LaplacianDeformation.cpp:407-417): pinned at its current
+ * position; duplicates ignored. */
+int fsi_deform_add_anchor(fsi_deform* d, int32_t vertex);
+/* setControlPoints (LaplacianDeformation.cpp:424-428): count handles and
+ * their targets (3 doubles each); replaces the previous set. */
+int fsi_deform_set_control_points(fsi_deform* d, const int32_t* idx, const double* targets, uint32_t count);
+/* precomputeSystem / solve (LaplacianDeformation.cpp:434-623): factor for
+ * the current constrained set, then the positions for the current targets. */
+int fsi_deform_precompute(fsi_deform* d);
+int fsi_deform_solve(fsi_deform* d);
+/* The current vertices (3 doubles each). */
+int fsi_deform_read_vertices(fsi_deform* d, double* verts);
+/* deformStep (src/main.cpp:862-944) given the per-vertex contact forces
+ * (3 floats each, as fsi_pbf_contact_forces returns them) and the frame dt:
+ * handles = non-anchor vertices pushed (|F| >= threshold, or >= the low
+ * threshold if a handle last frame) or relaxing (a handle last frame, more
+ * than 1e-3 from rest, restoring on); target = current + clamped push, eased
+ * toward rest, clamped to max_total_disp from rest.  Refactors when the handle
+ * set changed, solves when there are handles or the set changed (*solved). */
+int fsi_deform_step(fsi_deform* d, const float* forces, float dt, const fsi_deform_params* p,
+                    uint32_t* handle_count, int* solved);
+/* out[0] local patch size, out[1] factor steps of the last (re)factorization,
+ * out[2] handles of the last step, out[3] factorizations so far. */
+int fsi_deform_info(fsi_deform* d, uint32_t out[4]);
+/* verifyIncremental (LaplacianDeformation.cpp:671-716): max difference of a
+ * solve through the incrementally updated factor and through a fresh one;
+ * -1 when the incremental path is not active. */
+int fsi_deform_verify_incremental(fsi_deform* d, double* max_diff);
+
 #ifdef __cplusplus
 }
 #endif

@@ -72,6 +72,12 @@ class CSceneDesc(C.Structure):
     ]
 
 
+class CDeformParams(C.Structure):
+    _fields_ = [("stiffness", C.c_float), ("max_displacement", C.c_float), ("max_total_disp", C.c_float),
+                ("restore_enabled", C.c_uint32), ("restore_strength", C.c_float), ("control_threshold", C.c_float),
+                ("control_threshold_low", C.c_float)]
+
+
 class CTiming(C.Structure):
     _fields_ = [("ms", C.c_double * 8), ("launches", C.c_uint64 * 8), ("substeps", C.c_uint64),
                 ("frames", C.c_uint64)]
@@ -227,6 +233,18 @@ def load() -> C.CDLL:
         "fsi_mesh_read_forces": ([P, f32p], C.c_int),
         "fsi_pbf_set_sdf_from_mesh": ([P, P], C.c_int),
         "fsi_pbf_contact_forces": ([P, P, C.c_float, C.c_int, C.c_float, f32p, u32p], C.c_int),
+        "fsi_deform_params_default": ([C.POINTER(CDeformParams)], C.c_int),
+        "fsi_deform_create": ([f64p, C.c_uint32, i32p, C.c_uint32, C.POINTER(P)], C.c_int),
+        "fsi_deform_destroy": ([P], C.c_int),
+        "fsi_deform_set_mode": ([P, C.c_int, C.c_int, C.c_int], C.c_int),
+        "fsi_deform_add_anchor": ([P, C.c_int32], C.c_int),
+        "fsi_deform_set_control_points": ([P, i32p, f64p, C.c_uint32], C.c_int),
+        "fsi_deform_precompute": ([P], C.c_int),
+        "fsi_deform_solve": ([P], C.c_int),
+        "fsi_deform_read_vertices": ([P, f64p], C.c_int),
+        "fsi_deform_step": ([P, f32p, C.c_float, C.POINTER(CDeformParams), u32p, C.POINTER(C.c_int)], C.c_int),
+        "fsi_deform_info": ([P, u32p], C.c_int),
+        "fsi_deform_verify_incremental": ([P, f64p], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -327,6 +345,89 @@ def sdf_build(verts: np.ndarray, faces: np.ndarray, cell: float) -> SdfGrid:
     grid = np.zeros((int(dims[2]), int(dims[1]), int(dims[0])), dtype=np.float32)
     _check(lib.fsi_sdf_build(*args, _ptr(grid, C.c_float)))
     return SdfGrid(grid=grid, origin=origin, cell=float(cell), dims=(int(dims[0]), int(dims[1]), int(dims[2])))
+
+
+def deform_params(**kw) -> CDeformParams:
+    """deformStep's tuning (src/main.cpp:251, 300-304, 343) with overrides."""
+    p = CDeformParams()
+    _check(load().fsi_deform_params_default(C.byref(p)))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+class Deform:
+    """LaplacianDeformation on the host (fsi_deform_*): cotangent Laplacian,
+    local patch / global soft / hard-constraint (incremental) solves, and the
+    per-frame deformStep driven by contact forces."""
+
+    def __init__(self, verts: np.ndarray, faces: np.ndarray, use_local: bool = True, kring: int = 3,
+                 incremental: bool = False):
+        self.lib = load()
+        v = np.ascontiguousarray(verts, dtype=np.float64)
+        f = np.ascontiguousarray(faces, dtype=np.int32)
+        self.nv = len(v)
+        self.h = C.c_void_p()
+        _check(self.lib.fsi_deform_create(_ptr(v, C.c_double), len(v), _ptr(f, C.c_int32), len(f), C.byref(self.h)))
+        _check(self.lib.fsi_deform_set_mode(self.h, int(use_local), int(kring), int(incremental)))
+
+    def close(self):
+        if self.h:
+            self.lib.fsi_deform_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_mode(self, use_local: bool, kring: int = 3, incremental: bool = False):
+        _check(self.lib.fsi_deform_set_mode(self.h, int(use_local), int(kring), int(incremental)))
+
+    def add_anchor(self, v: int):
+        _check(self.lib.fsi_deform_add_anchor(self.h, int(v)))
+
+    def set_control_points(self, idx, targets):
+        i = np.ascontiguousarray(idx, dtype=np.int32).reshape(-1)
+        t = np.ascontiguousarray(targets, dtype=np.float64).reshape(-1, 3)
+        _check(self.lib.fsi_deform_set_control_points(self.h, _ptr(i, C.c_int32), _ptr(t, C.c_double), len(i)))
+
+    def precompute(self):
+        _check(self.lib.fsi_deform_precompute(self.h))
+
+    def solve(self):
+        _check(self.lib.fsi_deform_solve(self.h))
+
+    def vertices(self) -> np.ndarray:
+        out = np.zeros((self.nv, 3), dtype=np.float64)
+        _check(self.lib.fsi_deform_read_vertices(self.h, _ptr(out, C.c_double)))
+        return out
+
+    def step(self, forces: np.ndarray, dt: float, params: Optional[CDeformParams] = None):
+        """deformStep: returns (handle count, solved)."""
+        f = np.ascontiguousarray(forces, dtype=np.float32).reshape(self.nv, 3)
+        p = params if params is not None else deform_params()
+        nh, solved = C.c_uint32(0), C.c_int(0)
+        _check(self.lib.fsi_deform_step(self.h, _ptr(f, C.c_float), float(np.float32(dt)), C.byref(p), C.byref(nh),
+                                        C.byref(solved)))
+        return nh.value, bool(solved.value)
+
+    def info(self) -> dict:
+        out = np.zeros(4, dtype=np.uint32)
+        _check(self.lib.fsi_deform_info(self.h, _ptr(out, C.c_uint32)))
+        return {"patch": int(out[0]), "factor_steps": int(out[1]), "handles": int(out[2]), "factorizations": int(out[3])}
+
+    def verify_incremental(self) -> float:
+        d = C.c_double(0.0)
+        _check(self.lib.fsi_deform_verify_incremental(self.h, C.byref(d)))
+        return d.value
 
 
 class Mesh:

@@ -144,9 +144,26 @@ def test_sdf_build_matches_analytic_sphere():
 
 
 def test_cpp_mirror_class_compiles():
-    """The C++ PBFluids / SDFBoundary mirror builds against the header."""
-    src = os.path.join(PKG, "host", "PBFluids.cpp")
-    subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I", os.path.join(ROOT, "include"), src], check=True)
+    """The C++ PBFluids / SDFBoundary / LaplacianDeformation mirrors build against the header."""
+    for name in ("PBFluids.cpp", "SDFBoundary.cpp", "LaplacianDeformation.cpp"):
+        src = os.path.join(PKG, "host", name)
+        subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I", os.path.join(ROOT, "include"), src], check=True)
+
+
+def test_deform_params_layout_matches_header(tmp_path):
+    """The ctypes mirror of fsi_deform_params has the C compiler's offsets and size."""
+    import fsi_pbf
+
+    names = [f[0] for f in fsi_pbf.CDeformParams._fields_]
+    body = "\n".join(f'    printf("%zu\\n", offsetof(fsi_deform_params, {n}));' for n in names)
+    src = tmp_path / "layout.c"
+    src.write_text("#include <stddef.h>\n#include <stdio.h>\n#include \"fsi_pbf.h\"\nint main(void) {\n" + body +
+                   '\n    printf("%zu\\n", sizeof(fsi_deform_params));\n    return 0;\n}\n')
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    want = [getattr(fsi_pbf.CDeformParams, n).offset for n in names] + [C.sizeof(fsi_pbf.CDeformParams)]
+    assert got == want
 
 
 def test_scene_desc_layout_matches_header(tmp_path):
