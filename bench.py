@@ -294,72 +294,95 @@ def plate_mesh(nx=128, nz=32, size=(4.0, 1.0), origin=(1.2, 1.0, 2.7)):
 
 
 def fsi_leg(fsi, frames):
-    """The FSI coupling loop of src/main.cpp:1674-1707 on the device:
-    contact forces from the last frame's candidates, a prescribed bending of
-    the plate (the host Laplacian deformation is not part of this build:
-    the tip moves by at most 0.02 per frame, the reference's displacement
-    clamp), dirty SDF refit, the fluid step, the contact filter."""
+    """The two-way coupling loop of src/main.cpp:1674-1707: contact forces on
+    the device from the last frame's candidates, the host Laplacian
+    deformation of the plate (deformStep, src/main.cpp:862-944: the plate's
+    first column anchored, a cantilever), the SDF refresh of syncDeformedMesh
+    (src/main.cpp:803-860: every 3rd deformation, a dirty refit on the device;
+    every 100th refresh a full one), the fluid step and the contact filter."""
     verts, faces = plate_mesh()
-    cell, band, thresh, max_disp = 0.05, 0.12, 0.5, 0.02
-    dirty_band = max(0.025, 2.0 * cell) + max_disp + cell
+    cell, band, thresh, padding = 0.05, 0.12, 0.5, 0.025
+    params = fsi.deform_params()  # stiffness 1, 0.02 per frame, 0.4 total, restore 0.05, thresholds 0.5 / 0.3
+    every, full_every = 3, 100  # g_sdfUpdateEvery, g_sdfFullRebuildEvery
+    dirty_band = max(padding, 2.0 * cell) + every * float(params.max_displacement) + cell
     cfg = fsi.FluidConfig(dt=1.0 / 120.0, gravity=(0.0, -9.81, 0.0), h=0.1, rho0=6378.0, solver_iterations=6,
                           substep_iterations=2, eps=600.0, bounds_min=(0.0, 0.0, 0.0), bounds_max=(6.4, 6.4, 6.4),
                           hash_size=1 << 19)
     parts = fsi.lattice(64, 64, 64, (1.6, 2.0, 1.6), 0.05, 0.001, 1337)
     n = len(parts)
-    x0, length = verts[:, 0].min(), np.ptp(verts[:, 0])
-    shape = ((verts[:, 0] - x0) / length) ** 2  # cantilever: the first column stays put
-
-    def bent(f):
-        return verts + np.outer(0.3 * np.sin(0.02 * f) * shape, (0.0, -1.0, 0.0))
+    anchors = np.nonzero(verts[:, 0] == verts[:, 0].min())[0]
 
     mesh = fsi.Mesh(verts, faces, cell)
+    deform = fsi.Deform(verts, faces)
+    for v in anchors:
+        deform.add_anchor(int(v))
     s = fsi.Solver(cfg)
     s.set_particles(parts)
-    s.set_sdf_from_mesh(mesh, padding=0.025)
-    t_contact = t_refit = t_step = 0.0
-    contacts = dirty = 0
+    s.set_sdf_from_mesh(mesh, padding=padding)
+    t_contact = t_deform = t_refit = t_step = 0.0
+    contacts = dirty = handles = solves = 0
+    syncs = sdf_syncs = 0
     s.filter_contacts(band)
     s.sync()
     t0 = time.perf_counter()
     for f in range(frames):
         a = time.perf_counter()
-        _, (nc, _) = s.contact_forces(mesh, band, True, thresh)
+        forces, (nc, _) = s.contact_forces(mesh, band, True, thresh)
         b = time.perf_counter()
-        nd, _, _ = mesh.refit_dirty(bent(f + 1), dirty_band, 1e-4)
-        s.set_sdf_from_mesh(mesh, padding=0.025)
+        nh, solved = deform.step(forces, cfg.dt, params)
         c = time.perf_counter()
+        if solved:
+            solves += 1
+            if syncs % every == 0:
+                cur = deform.vertices()
+                if sdf_syncs % full_every == 0:
+                    mesh.refit(cur)
+                else:
+                    nd, _, _ = mesh.refit_dirty(cur, dirty_band, 1e-4)
+                    dirty += nd
+                sdf_syncs += 1
+                s.set_sdf_from_mesh(mesh, padding=padding)
+            syncs += 1
+        d = time.perf_counter()
         s.step()
         s.filter_contacts(band)
         s.sync()
-        d = time.perf_counter()
+        e = time.perf_counter()
         t_contact += b - a
-        t_refit += c - b
-        t_step += d - c
+        t_deform += c - b
+        t_refit += d - c
+        t_step += e - d
         contacts += nc
-        dirty += nd
+        handles += nh
     wall = time.perf_counter() - t0
+    final = deform.vertices()
+    info = deform.info()
     # a full refit (SDFBoundary::refitFromMesh) on the device vs the host build
     reps = 5
-    mesh.refit(bent(frames))
     a = time.perf_counter()
     for _ in range(reps):
-        mesh.refit(bent(frames))
+        mesh.refit(final)
     full_ms = (time.perf_counter() - a) / reps * 1e3
     a = time.perf_counter()
-    fsi.sdf_build(bent(frames), faces, cell)
+    fsi.sdf_build(final, faces, cell)
     host_ms = (time.perf_counter() - a) * 1e3
     got = s.read_particles()
     s.close()
+    deform.close()
     mesh.close()
     return {"particles": n, "frames": frames, "iterations": 6, "substeps": 2,
-            "plate": {"vertices": len(verts), "triangles": len(faces), "sdf_cell": cell, "sdf_dims": list(mesh.dims)},
+            "plate": {"vertices": len(verts), "triangles": len(faces), "anchors": int(len(anchors)), "sdf_cell": cell,
+                      "sdf_dims": list(mesh.dims)},
             "ms_per_frame": round(wall / frames * 1e3, 3),
             "value": n * 12 * frames / wall, "unit": "particle-iterations/s",
             "contact_ms_per_frame": round(t_contact / frames * 1e3, 3),
+            "deform_ms_per_frame": round(t_deform / frames * 1e3, 3),
             "sdf_refit_ms_per_frame": round(t_refit / frames * 1e3, 3),
             "step_ms_per_frame": round(t_step / frames * 1e3, 3),
-            "mean_contacts": round(contacts / frames, 1), "mean_dirty_voxels": round(dirty / frames, 1),
+            "mean_contacts": round(contacts / frames, 1), "mean_handles": round(handles / frames, 1),
+            "deform_solves": solves, "factorizations": info["factorizations"], "sdf_refreshes": sdf_syncs,
+            "mean_dirty_voxels_per_refresh": round(dirty / max(1, sdf_syncs - 1), 1),
+            "max_displacement_from_rest": round(float(np.abs(final - verts).max()), 4),
             "full_refit_ms": round(full_ms, 3), "host_sdf_build_ms": round(host_ms, 3),
             "host_sdf_build_threads": min(16, os.cpu_count() or 1),
             "finite": bool(np.all(np.isfinite(got[:, 0:3])))}

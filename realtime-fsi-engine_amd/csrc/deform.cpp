@@ -449,62 +449,69 @@ bool Laplacian::precompute_local() {
     L.ready = false;
     if (ctrl_.empty()) return true;  // nothing to solve: the mesh stays as it is
     const std::vector<int> ring = kring(ctrl_, kring_);
-    L.interior.clear();
-    L.boundary.clear();
-    std::vector<int> g2l((size_t)n(), -1), g2b((size_t)n(), -1);
-    for (int v : ring) {
-        if (is_anchor(v)) {
-            g2b[(size_t)v] = (int)L.boundary.size();
-            L.boundary.push_back(v);
-        } else {
-            g2l[(size_t)v] = (int)L.interior.size();
-            L.interior.push_back(v);
-        }
-    }
-    const int ni = (int)L.interior.size();
-    if (ni == 0) return true;
-    std::vector<Triplet> tii, tib;
-    for (int li = 0; li < ni; ++li) {
-        const int gi = L.interior[(size_t)li];
-        double wsum = 0.0;
-        for (double w : wnbr_[(size_t)gi]) wsum += w;
-        if (wsum < 1e-12) wsum = 1e-12;
-        tii.push_back({li, li, 1.0});
-        for (size_t k = 0; k < nbr_[(size_t)gi].size(); ++k) {
-            const int gj = nbr_[(size_t)gi][k];
-            const double w = wnbr_[(size_t)gi][k] / wsum;
-            if (g2l[(size_t)gj] >= 0) {
-                tii.push_back({li, g2l[(size_t)gj], -w});
+    if (!L.analyzed || ring != L.ring) {
+        // a new patch: its sub-Laplacians, Lii^T Lii and the factor's pattern
+        // (kept while the handles move inside the same k-ring)
+        L.analyzed = false;
+        L.interior.clear();
+        L.boundary.clear();
+        L.g2l.assign((size_t)n(), -1);
+        std::vector<int> g2b((size_t)n(), -1);
+        for (int v : ring) {
+            if (is_anchor(v)) {
+                g2b[(size_t)v] = (int)L.boundary.size();
+                L.boundary.push_back(v);
             } else {
-                if (g2b[(size_t)gj] < 0) {
-                    g2b[(size_t)gj] = (int)L.boundary.size();
-                    L.boundary.push_back(gj);
-                }
-                tib.push_back({li, g2b[(size_t)gj], -w});
+                L.g2l[(size_t)v] = (int)L.interior.size();
+                L.interior.push_back(v);
             }
         }
+        const int ni = (int)L.interior.size();
+        if (ni == 0) return true;
+        std::vector<Triplet> tii, tib;
+        for (int li = 0; li < ni; ++li) {
+            const int gi = L.interior[(size_t)li];
+            double wsum = 0.0;
+            for (double w : wnbr_[(size_t)gi]) wsum += w;
+            if (wsum < 1e-12) wsum = 1e-12;
+            tii.push_back({li, li, 1.0});
+            for (size_t k = 0; k < nbr_[(size_t)gi].size(); ++k) {
+                const int gj = nbr_[(size_t)gi][k];
+                const double w = wnbr_[(size_t)gi][k] / wsum;
+                if (L.g2l[(size_t)gj] >= 0) {
+                    tii.push_back({li, L.g2l[(size_t)gj], -w});
+                } else {
+                    if (g2b[(size_t)gj] < 0) {
+                        g2b[(size_t)gj] = (int)L.boundary.size();
+                        L.boundary.push_back(gj);
+                    }
+                    tib.push_back({li, g2b[(size_t)gj], -w});
+                }
+            }
+        }
+        L.Lii = from_triplets(ni, ni, std::move(tii));
+        L.Lib = from_triplets(ni, (int)L.boundary.size(), std::move(tib));
+        L.LiiT = transpose(L.Lii);
+        L.LtL = multiply(L.LiiT, L.Lii);  // unit diagonal of Lii: every diagonal entry is present
+        L.diag.assign((size_t)ni, -1);
+        for (int c = 0; c < ni; ++c)
+            for (int q = L.LtL.p[(size_t)c]; q < L.LtL.p[(size_t)c + 1]; ++q)
+                if (L.LtL.i[(size_t)q] == c) L.diag[(size_t)c] = q;
+        L.ldlt.analyze(L.LtL, true);
+        L.ring = ring;
+        L.analyzed = true;
     }
-    L.Lii = from_triplets(ni, ni, std::move(tii));
-    L.Lib = from_triplets(ni, (int)L.boundary.size(), std::move(tib));
-    L.LiiT = transpose(L.Lii);
     L.handle_local.clear();
     L.handle_global.clear();
     for (int h : ctrl_)
-        if (g2l[(size_t)h] >= 0) {
-            L.handle_local.push_back(g2l[(size_t)h]);
+        if (L.g2l[(size_t)h] >= 0) {
+            L.handle_local.push_back(L.g2l[(size_t)h]);
             L.handle_global.push_back(h);
         }
     if (L.handle_local.empty()) return true;
-    std::vector<Triplet> th;
-    for (int c : L.handle_local) th.push_back({c, c, 1.0});
-    Csc AtA = multiply(L.LiiT, L.Lii);
-    // + I_handles
-    std::vector<Triplet> ta;
-    for (int c = 0; c < AtA.cols; ++c)
-        for (int q = AtA.p[(size_t)c]; q < AtA.p[(size_t)c + 1]; ++q) ta.push_back({AtA.i[(size_t)q], c, AtA.x[(size_t)q]});
-    ta.insert(ta.end(), th.begin(), th.end());
-    L.AtA = from_triplets(ni, ni, std::move(ta));
-    L.ldlt.analyze(L.AtA, true);
+    // Lii^T Lii + I_handles (a handle listed twice counts twice)
+    L.AtA = L.LtL;
+    for (int c : L.handle_local) L.AtA.x[(size_t)L.diag[(size_t)c]] += 1.0;
     if (!L.ldlt.factorize(L.AtA)) return fail(FSI_ERR_STATE, "local LDL^T factorization failed") == FSI_OK;
     last_steps_ = L.ldlt.last_steps();
     L.ready = true;

@@ -8,6 +8,7 @@
 // deform.cpp.
 #pragma once
 
+#include <cstddef>
 #include <cstdint>
 #include <vector>
 
@@ -70,6 +71,8 @@ public:
     bool ok() const { return ok_; }
     // Steps recomputed by the last factorize / update.
     int last_steps() const { return last_steps_; }
+    // Entries of L below the diagonal.
+    size_t nnz() const { return li_.size(); }
 
 private:
     int n_ = 0;
@@ -132,12 +135,14 @@ private:
     int last_steps_ = 0;
 
     struct Local {
-        std::vector<int> interior, boundary;
-        Csc Lii, Lib, LiiT;
+        std::vector<int> ring;               // the k-ring the structures below belong to
+        std::vector<int> interior, boundary, g2l;
+        Csc Lii, Lib, LiiT, LtL;             // LtL = Lii^T Lii
+        std::vector<int> diag;               // position of each diagonal entry in LtL.x
         std::vector<int> handle_local, handle_global;
-        Csc AtA;
+        Csc AtA;                             // LtL + I_handles
         Ldlt ldlt;
-        bool ready = false;
+        bool analyzed = false, ready = false;
     } local_;
 
     // global soft path

@@ -215,8 +215,21 @@ def test_incremental_path_hard_constraints_and_updates():
             assert np.abs(got - want).max() < 1e-8
             assert np.abs(got[handles] - targets).max() < 1e-12  # hard: the targets are hit
             assert d.verify_incremental() < 1e-9
-        # after the first full factorization, set changes recompute only tree paths
-        assert steps[0] == len(verts) and min(steps[1:]) < len(verts)
+        assert steps[0] == len(verts)
+    # a local change of the constrained set recomputes only the tree paths
+    # from the changed columns (fewer steps than a full factorization)
+    verts, faces = plate(40, 10, (4.0, 1.0))
+    with fsi_pbf.Deform(verts, faces, use_local=False, incremental=True) as d:
+        for a in range(0, 41 * 11, 41):
+            d.add_anchor(a)
+        d.set_control_points([200], verts[[200]] + [0.0, 0.0, 0.05])
+        d.precompute()
+        assert d.info()["factor_steps"] == len(verts)
+        d.set_control_points([200, 439], verts[[200, 439]] + [0.0, 0.0, 0.05])
+        d.precompute()
+        d.solve()
+        assert 0 < d.info()["factor_steps"] < len(verts)
+        assert d.verify_incremental() < 1e-9
 
 
 def step_restated(cur, rest, forces, dt, last, anchors, p):

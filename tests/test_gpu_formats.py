@@ -6,12 +6,12 @@ import pytest
 
 from conftest import dam_break_config
 
+torch = pytest.importorskip("torch")  # before the library is loaded: one HIP runtime
+
 pytestmark = pytest.mark.gpu
 
 
 def test_render_buffer_is_the_particle_records(gpu):
-    import torch
-
     import fsi_pbf
 
     cfg = dam_break_config(n_iter=2, substeps=1)

@@ -154,3 +154,45 @@ def test_contact_forces_need_candidates(gpu):
         s.set_particles(parts)
         with pytest.raises(fsi_pbf.FsiError):
             s.contact_forces(m, 0.05)
+
+
+def test_two_way_coupling_frames(gpu):
+    """Contact forces from the device drive the host deformation
+    (deformStep), whose mesh refits the device SDF: anchors stay, the plate
+    is pushed along the fluid's impact, and the grid follows the mesh."""
+    import fsi_pbf
+    from conftest import dam_break_config
+
+    verts, faces = plate(32, 8, (3.2, 0.8), (0.8, 0.8, 0.8))
+    verts = verts[:, [0, 2, 1]].copy()  # the plate in y = 0.8 (x 0.8..4.0, z 0.8..1.6), normals +y
+    faces = faces[:, [0, 2, 1]].copy()
+    anchors = np.nonzero(verts[:, 0] == verts[:, 0].min())[0]
+    cfg = dam_break_config(n_iter=3, substeps=2, box=(4.8, 3.2, 3.2), hash_size=1 << 16)
+    parts = fsi_pbf.lattice(32, 16, 12, (1.2, 1.0, 1.0), 0.05, 0.001, 3)
+    parts[:, 8:11] = (0.0, -4.0, 0.0)  # falling onto the plate
+    params = fsi_pbf.deform_params()
+    with fsi_pbf.Mesh(verts, faces, 0.05) as m, fsi_pbf.Solver(cfg) as s, fsi_pbf.Deform(verts, faces) as d:
+        for a in anchors:
+            d.add_anchor(int(a))
+        s.set_particles(parts)
+        s.set_sdf_from_mesh(m, padding=0.025)
+        s.filter_contacts(0.12)
+        moved = 0
+        for _ in range(12):
+            forces, (nc, _) = s.contact_forces(m, 0.12, True, params.control_threshold)
+            nh, solved = d.step(forces, cfg.dt, params)
+            if solved:
+                m.refit(d.vertices())
+                s.set_sdf_from_mesh(m, padding=0.025)
+                moved += 1
+            s.step()
+            s.filter_contacts(0.12)
+        cur = d.vertices()
+        grid = m.read_sdf()
+    assert moved > 0
+    assert np.array_equal(cur[anchors], verts[anchors])
+    disp = cur - verts
+    assert disp[:, 1].min() < -1e-3  # pushed down where the fluid lands
+    assert np.abs(disp).max() <= 0.4 + 1e-9  # the total-displacement clamp
+    ref = pyoracle.mesh_sdf(cur, faces, grid.origin, grid.cell, grid.dims, verts_build=verts)
+    assert np.abs(grid.grid - ref).max() <= 1e-6
