@@ -467,6 +467,12 @@ def main():
     s.set_timing(False)
     got = s.read_particles()
     assert np.all(np.isfinite(got[:, 0:3])), "non-finite positions"
+    # density-constraint error of the last lambda pass (CS_ComputeLambdas.glsl:
+    # C = max(rho / rho0 - 1, -cohesion)), from the solver's own rho
+    _, drho = s.read_solver()
+    c_err = np.maximum(drho[:, 3].astype(np.float64) / float(cfg.rho0) - 1.0, -float(cfg.cohesion_strength))
+    density_error = {"mean_abs_C": float(np.abs(c_err).mean()), "max_C": float(c_err.max()),
+                     "of": f"the last lambda pass of frame {args.warmup + args.steps + min(args.steps, 5)}"}
     s.close()
 
     value = n * SUBSTEPS * ITERS * args.steps / wall
@@ -507,6 +513,7 @@ def main():
         "config": {"workload": "plain PBF, 1,048,576 particles, icosphere SDF obstacle, I=6, S=2, H=2^20",
                    "particles": n, "iterations": ITERS, "substeps": SUBSTEPS, "hash_size": HASH},
         "roofline": roofline,
+        "density_error": density_error,
         "passes": passes,
         "passes_note": "separate run of min(steps, 5) frames with events around every pass",
     }
