@@ -289,9 +289,10 @@ int fsi_pbf_set_tuning(fsi_pbf* h, int knob, int value);
 int fsi_pbf_get_tuning(fsi_pbf* h, int knob, int* value);
 
 /* Autotuner: times `frames` frames of the current state under every
- * combination of gather form 0 or 1 and digit width 6, 8 or 10
- * (restoring the particle state after each trial), keeps the fastest and
- * returns it.  No counterpart in the reference. */
+ * combination of gather form 0, 1 or 2 (2 when the processing order's bins
+ * are 4x4x4-cell tiles) and digit width 6, 8 or 10 (restoring the particle
+ * state after each trial), keeps the fastest and returns it.  No
+ * counterpart in the reference. */
 int fsi_pbf_autotune(fsi_pbf* h, uint32_t frames, int* gather, int* digit_bits);
 
 /* Replay each frame from a HIP graph captured on first use (re-captured when

@@ -436,7 +436,10 @@ int core_autotune(Core& c, const StepConst& k, const FramePlan& plan, uint32_t f
     int best_g = c.d.variant, best_b = c.sort_digit_bits;
     int rc = FSI_OK;
     const int bits[] = {6, 8, 10};
-    for (int g = 0; g <= 1 && rc == FSI_OK; ++g)
+    // the staged-tile form (2) is a candidate when the processing order's
+    // bins are tiles; otherwise it would run as form 1
+    const int gmax = c.d.tiled ? 2 : 1;
+    for (int g = 0; g <= gmax && rc == FSI_OK; ++g)
         for (int b : bits) {
             c.d.variant = g;
             c.sort_digit_bits = b;
