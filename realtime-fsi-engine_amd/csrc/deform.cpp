@@ -252,6 +252,7 @@ void Ldlt::set_constrained(const std::vector<char>& constrained) {
     for (int k = 0; k < n_; ++k) con_[(size_t)inv_[(size_t)k]] = (k < (int)constrained.size()) ? constrained[(size_t)k] : 0;
 }
 
+template <bool SEQ>
 bool Ldlt::step(const Csc& a, int k) {
     // y = column k of A^B (rows <= k); pattern = reach of its rows in the tree
     int top = n_;
@@ -275,10 +276,17 @@ bool Ldlt::step(const Csc& a, int k) {
         const int i = pattern_[(size_t)top];
         const double yi = y_[(size_t)i];
         y_[(size_t)i] = 0.0;
-        // slot of row k in column i; the rows above it are L(:, i) of earlier steps
-        const int* b = li_.data() + lp_[(size_t)i];
-        const int* e = li_.data() + lp_[(size_t)i + 1];
-        const int s = (int)(std::lower_bound(b, e, k) - li_.data());
+        // slot of row k in column i; the rows above it are L(:, i) of earlier
+        // steps.  In a full pass (SEQ) the rows of a column are reached in
+        // increasing k, so the slot is the column's next one.
+        int s;
+        if (SEQ) {
+            s = next_[(size_t)i]++;
+        } else {
+            const int* b = li_.data() + lp_[(size_t)i];
+            const int* e = li_.data() + lp_[(size_t)i + 1];
+            s = (int)(std::lower_bound(b, e, k) - li_.data());
+        }
         for (int p = lp_[(size_t)i]; p < s; ++p) y_[(size_t)li_[(size_t)p]] -= lx_[(size_t)p] * yi;
         const double lki = yi / d_[(size_t)i];
         dk -= lki * yi;
@@ -291,8 +299,9 @@ bool Ldlt::step(const Csc& a, int k) {
 bool Ldlt::factorize(const Csc& a) {
     ok_ = false;
     std::fill(flag_.begin(), flag_.end(), -1);
+    next_.assign(lp_.begin(), lp_.end() - 1);
     for (int k = 0; k < n_; ++k)
-        if (!step(a, k)) return false;
+        if (!step<true>(a, k)) return false;
     last_steps_ = n_;
     ok_ = true;
     return true;
@@ -315,7 +324,7 @@ bool Ldlt::update(const Csc& a, const std::vector<int>& changed) {
     // recomputed now does not stop at a mark left by the previous step k
     std::fill(flag_.begin(), flag_.end(), -1);
     for (int k : steps)
-        if (!step(a, k)) return false;
+        if (!step<false>(a, k)) return false;
     last_steps_ = (int)steps.size();
     ok_ = true;
     return true;

@@ -86,6 +86,8 @@ private:
     mutable std::vector<double> y_;
     mutable std::vector<int> pattern_, flag_;
     std::vector<int> ap_src_;      // for each entry of ap_, its position in a.x
+    std::vector<int> next_;        // next slot of each column during a full pass
+    template <bool SEQ>
     bool step(const Csc& a, int k);  // row k of L and D[k]
 };
 
