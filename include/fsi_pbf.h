@@ -303,6 +303,15 @@ int fsi_pbf_set_graph(fsi_pbf* h, int enable);
 /* Device pointer of the HIP stream the handle enqueues on (hipStream_t). */
 int fsi_pbf_get_stream(fsi_pbf* h, void** stream);
 
+/* Enqueue every later launch and copy of the handle on the caller's HIP
+ * stream (hipStream_t of the handle's device; NULL: back to the handle's own
+ * stream).  The handle's current stream is synchronised first.  The caller
+ * keeps the stream alive while the handle uses it.  Several handles on one
+ * stream run their frames in submission order, as the reference's dispatches
+ * do on its one GL context (src/fluid/PBFluids.cpp:323-517); that is the only
+ * counterpart. */
+int fsi_pbf_set_stream(fsi_pbf* h, void* stream);
+
 /* ------------------------------------------------------------------------- */
 /* Multi-scene solver: many independent scenes, one launch per pass            */
 /* (no counterpart in the reference, which runs one scene per PBFluids; every */

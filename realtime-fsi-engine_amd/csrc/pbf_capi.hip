@@ -650,4 +650,10 @@ int fsi_pbf_get_stream(fsi_pbf* h, void** stream) {
     return FSI_OK;
 }
 
+int fsi_pbf_set_stream(fsi_pbf* h, void* stream) {
+    int rc = bind(h);
+    if (rc) return rc;
+    return core_set_stream(h->core, (hipStream_t)stream);
+}
+
 }  // extern "C"

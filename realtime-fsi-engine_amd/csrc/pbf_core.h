@@ -62,7 +62,8 @@ struct FramePlan {
 
 struct Core {
     int device = 0;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;      // where every launch and copy of the handle goes
+    hipStream_t own_stream = nullptr;  // the stream the handle created (stream unless the caller set one)
     DeviceState d{};
     uint32_t capacity = 0;        // particle buffers allocated for
     uint32_t cells_capacity = 0;  // cell-table entries allocated
@@ -85,6 +86,9 @@ struct Core {
 
 // Binds the handle's device and creates its stream.
 int core_init(Core& c);
+// Enqueues on s from now on (nullptr: the handle's own stream) after the
+// current stream has drained; s must belong to the handle's device.
+int core_set_stream(Core& c, hipStream_t s);
 void core_destroy(Core& c);
 // (Re)allocates the per-particle buffers for n particles.
 hipError_t core_alloc_particles(Core& c, uint32_t n);

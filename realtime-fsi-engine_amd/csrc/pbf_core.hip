@@ -170,7 +170,21 @@ int core_init(Core& c) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, c.device) == hipSuccess && std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
         return fail(FSI_ERR_NO_DEVICE, std::string("device is ") + prop.gcnArchName + ", this library is built for gfx950");
-    FSI_HIP_TRY(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+    FSI_HIP_TRY(hipStreamCreateWithFlags(&c.own_stream, hipStreamNonBlocking));
+    c.stream = c.own_stream;
+    return FSI_OK;
+}
+
+int core_set_stream(Core& c, hipStream_t s) {
+    if (!s) s = c.own_stream;
+    if (s == c.stream) return FSI_OK;
+    hipDevice_t dev = -1;
+    FSI_HIP_TRY(hipStreamGetDevice(s, &dev));
+    if (dev != c.device) return fail(FSI_ERR_INVALID_ARG, "the stream belongs to another device");
+    int rc = core_drain_timing(c);
+    if (rc) return rc;
+    FSI_HIP_TRY(hipStreamSynchronize(c.stream));
+    c.stream = s;
     return FSI_OK;
 }
 
@@ -231,7 +245,8 @@ void core_destroy(Core& c) {
     c.pending.clear();
     for (hipEvent_t e : c.event_pool) (void)hipEventDestroy(e);
     c.event_pool.clear();
-    if (c.stream) (void)hipStreamDestroy(c.stream);
+    if (c.own_stream) (void)hipStreamDestroy(c.own_stream);
+    c.own_stream = nullptr;
     c.stream = nullptr;
 }
 

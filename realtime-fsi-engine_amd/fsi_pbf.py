@@ -193,6 +193,7 @@ def load() -> C.CDLL:
         "fsi_pbf_timing_reset": ([P], C.c_int),
         "fsi_pbf_read_timing": ([P, C.POINTER(CTiming)], C.c_int),
         "fsi_pbf_get_stream": ([P, C.POINTER(P)], C.c_int),
+        "fsi_pbf_set_stream": ([P, P], C.c_int),
         "fsi_pbf_set_tuning": ([P, C.c_int, C.c_int], C.c_int),
         "fsi_pbf_get_tuning": ([P, C.c_int, C.POINTER(C.c_int)], C.c_int),
         "fsi_pbf_autotune": ([P, C.c_uint32, C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
@@ -645,6 +646,16 @@ class Solver:
 
     def set_graph(self, on: bool):
         _check(self.lib.fsi_pbf_set_graph(self.h, 1 if on else 0))
+
+    def get_stream(self) -> int:
+        """hipStream_t the handle enqueues on, as an integer."""
+        s = C.c_void_p()
+        _check(self.lib.fsi_pbf_get_stream(self.h, C.byref(s)))
+        return s.value or 0
+
+    def set_stream(self, stream: int = 0):
+        """Enqueue on the given hipStream_t of this device (0: the handle's own)."""
+        _check(self.lib.fsi_pbf_set_stream(self.h, C.c_void_p(stream or None)))
 
     def autotune(self, frames: int = 3):
         """Time every (gather form, sort digit bits) pair; keeps the fastest. Particle state is restored."""

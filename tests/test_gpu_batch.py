@@ -151,3 +151,38 @@ def test_batch_error_paths(gpu):
     assert np.array_equal(k0, k1)
     assert b.lib.fsi_batch_read_grid(b.h, 2, None, None, None) != 0  # scene out of range
     b.close()
+
+
+def test_single_scene_handles_on_one_stream(gpu):
+    """fsi_pbf_set_stream: handles that share one stream (the reference's
+    single-queue order) step exactly as on their own streams; NULL restores
+    the handle's own stream."""
+    fsi = _fsi()
+    base = dam_break_config()
+    scenes = make_scenes(seed=11, count=3)
+    expect = [run_single(base, sc, 2)[0] for sc in scenes]
+    solvers = []
+    for sc in scenes:
+        s = fsi.Solver(scene_cfg(base, sc))
+        s.set_particles(sc.particles)
+        if sc.sdf is not None:
+            s.set_sdf(sc.sdf, sc.sdf_padding, sc.sdf_containment)
+        solvers.append(s)
+    own = [s.get_stream() for s in solvers]
+    assert len(set(own)) == len(own) and all(own)
+    shared = own[0]
+    for s in solvers[1:]:
+        s.set_stream(shared)
+        assert s.get_stream() == shared
+    for _ in range(2):
+        for s in solvers:
+            s.step()
+    for s, e in zip(solvers, expect):
+        assert np.array_equal(s.read_particles(), e)
+    for s, o in zip(solvers[1:], own[1:]):
+        s.set_stream(0)
+        assert s.get_stream() == o
+    solvers[1].step()  # on its own stream again, while solver 0 still owns the shared one
+    solvers[1].sync()
+    for s in solvers:
+        s.close()
