@@ -42,7 +42,7 @@ a prescribed bending of the plate, dirty SDF refit, step, contact filter.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch-steps K2]
                        [--fsi-steps K3] [--no-batched] [--no-adjoint] [--no-fsi]
-                       [--no-cpu-baseline]
+                       [--no-cpu-baseline] [--gather G] [--order O]
 """
 import argparse
 import json
@@ -455,6 +455,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--fsi-steps", type=int, default=300)
     ap.add_argument("--no-fsi", action="store_true")
+    ap.add_argument("--gather", type=int, default=None,
+                    help="pin the gather form of the 1M solver (0, 1, 2; default: the library's, 1)")
+    ap.add_argument("--order", type=int, default=None,
+                    help="pin the processing order of the 1M solver (0 sorted slots, 1 Morton; default 1)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain device copy of 256 MiB (3 launches), the reference point that scales "
                          "the FETCH_SIZE / WRITE_SIZE counters to bytes")
@@ -486,6 +490,12 @@ def main():
     s = fsi.Solver(cfg)
     s.set_particles(parts)
     s.set_sdf(sdf, 0.025, False)
+    if args.gather is not None:
+        s.set_tuning(fsi.TUNE_GATHER, args.gather)
+    if args.order is not None:
+        s.set_tuning(fsi.TUNE_ORDER, args.order)
+    knobs = {"gather": s.get_tuning(fsi.TUNE_GATHER), "sort_digit_bits": s.get_tuning(fsi.TUNE_SORT_DIGIT_BITS),
+             "order": s.get_tuning(fsi.TUNE_ORDER)}
     s.step(args.warmup)
     s.sync()
     # timed region: events bracket the launches of the dominant pass (lambda)
@@ -550,7 +560,8 @@ def main():
         "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic (seeded lattice + procedural icosphere SDF)",
         "config": {"workload": "plain PBF, 1,048,576 particles, icosphere SDF obstacle, I=6, S=2, H=2^20",
-                   "particles": n, "iterations": ITERS, "substeps": SUBSTEPS, "hash_size": HASH},
+                   "particles": n, "iterations": ITERS, "substeps": SUBSTEPS, "hash_size": HASH,
+                   "knobs": knobs},
         "roofline": roofline,
         "density_error": density_error,
         "passes": passes,

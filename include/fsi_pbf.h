@@ -283,7 +283,11 @@ enum fsi_pbf_knob {
     FSI_TUNE_GATHER = 0,          /* neighbour-gather form: 0 single loop, 1 two-phase list, */
                                   /* 2 LDS-staged 4x4x4-cell tiles                           */
     FSI_TUNE_SORT_DIGIT_BITS = 1, /* widest radix digit, 1..10                                */
-    FSI_TUNE_COUNT = 2
+    FSI_TUNE_ORDER = 2,           /* thread -> particle order of the gather passes: 0 sorted  */
+                                  /* slots as the sort leaves them, 1 Morton order of the      */
+                                  /* predicted cells in per-XCD ranges (default; form 2 needs  */
+                                  /* it and runs as form 1 under 0)                            */
+    FSI_TUNE_COUNT = 3
 };
 int fsi_pbf_set_tuning(fsi_pbf* h, int knob, int value);
 int fsi_pbf_get_tuning(fsi_pbf* h, int knob, int* value);

@@ -289,7 +289,7 @@ hipError_t core_alloc_particles(Core& c, uint32_t n) {
     ALLOC(d.bin_rank, n);
     ALLOC(d.order, n);
     ALLOC(d.list_count, n);
-    ALLOC(d.list, (size_t)kNbrCap * n);
+    ALLOC(d.list, (size_t)kNbrCap * (((size_t)n + 63) & ~(size_t)63));  // whole 64-thread blocks
     ALLOC(d.tflag, n);
     ALLOC(d.tidx, n);
     ALLOC(d.tiles, (size_t)n + 1);
@@ -398,10 +398,11 @@ int core_frame(Core& c, const StepConst& k, const FramePlan& plan) {
         rc = enqueue_frame(c, k, plan, c.timing);
     } else {
         // key: everything the launches of a frame depend on
-        std::vector<unsigned char> key(sizeof(StepConst) + sizeof(FramePlan) + 4 * sizeof(int));
+        std::vector<unsigned char> key(sizeof(StepConst) + sizeof(FramePlan) + 5 * sizeof(int));
         std::memcpy(key.data(), &k, sizeof(StepConst));
         std::memcpy(key.data() + sizeof(StepConst), &plan, sizeof(FramePlan));
-        const int knobs[4] = {c.d.variant, c.sort_digit_bits, c.taps ? 1 : 0, c.d.any_sdf ? 1 : 0};
+        const int knobs[5] = {c.d.variant, c.sort_digit_bits, c.taps ? 1 : 0, c.d.any_sdf ? 1 : 0,
+                              c.d.morton ? 1 : 0};
         std::memcpy(key.data() + sizeof(StepConst) + sizeof(FramePlan), knobs, sizeof(knobs));
         if (!c.graph_valid || key != c.graph_key) {
             core_invalidate_graph(c);
@@ -520,6 +521,10 @@ int core_set_tuning(Core& c, int knob, int value) {
             if (value < 1 || value > kSortMaxDigitBits) return fail(FSI_ERR_INVALID_ARG, "digit bits must be 1..10");
             c.sort_digit_bits = value;
             return FSI_OK;
+        case FSI_TUNE_ORDER:
+            if (value < 0 || value > 1) return fail(FSI_ERR_INVALID_ARG, "processing order must be 0 or 1");
+            c.d.morton = value == 1;
+            return FSI_OK;
         default:
             return fail(FSI_ERR_INVALID_ARG, "unknown knob");
     }
@@ -529,6 +534,7 @@ int core_get_tuning(const Core& c, int knob, int* value) {
     switch (knob) {
         case FSI_TUNE_GATHER: *value = c.d.variant; return FSI_OK;
         case FSI_TUNE_SORT_DIGIT_BITS: *value = c.sort_digit_bits; return FSI_OK;
+        case FSI_TUNE_ORDER: *value = c.d.morton ? 1 : 0; return FSI_OK;
         default: return fail(FSI_ERR_INVALID_ARG, "unknown knob");
     }
 }

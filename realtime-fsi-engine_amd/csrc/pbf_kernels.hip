@@ -257,6 +257,12 @@ __global__ __launch_bounds__(kBlock) void k_fill_u32(uint32_t* __restrict__ p, u
     if (i < n) p[i] = v;
 }
 
+// Sorted-slot processing order: thread t takes slot t.
+__global__ __launch_bounds__(kBlock) void k_order_slots(uint32_t n, uint32_t* __restrict__ order) {
+    const uint32_t k = gid();
+    if (k < n) order[k] = k;
+}
+
 __global__ __launch_bounds__(kBlock) void k_order(uint32_t n, const uint32_t* __restrict__ base,
                                                   const uint32_t* __restrict__ bin_of,
                                                   const uint32_t* __restrict__ bin_rank, uint32_t* __restrict__ order) {
@@ -280,9 +286,20 @@ __global__ __launch_bounds__(kBlock) void k_lod(StepConst c, const float4* __res
     lod[k] = l;
 }
 
+// Neighbour-list slot of entry r (< kNbrCap) of thread t.  The 64 threads of
+// a wave own one block of 64 x kNbrCap slots, in which entries r..r+3 (r a
+// multiple of four) of all 64 threads form one contiguous 1 KiB run, four
+// slots per thread.  The delta-p pass reads four entries per thread with one
+// 16-byte load, so each of its wave loads covers whole cache lines once.  (One
+// row per thread instead made every wave load touch 64 lines, re-fetched four
+// times each as the row was walked, and the lines of the waves in flight did
+// not fit in the L2.)
+__device__ __forceinline__ size_t list_slot(uint32_t t, uint32_t r) {
+    return (size_t)(t >> 6) * (64u * (uint32_t)kNbrCap) + (r >> 2) * 256u + (t & 63u) * 4u + (r & 3u);
+}
+
 // CS_ComputeLambdas.glsl:108-162.  In the two-phase form the neighbour list
-// is also written to HBM, one row of kNbrCap slots per thread (entry r of
-// thread t at list[t * kNbrCap + r]), for the delta-p pass of the same
+// is also written to HBM (list_slot), for the delta-p pass of the same
 // iteration, which sees the same positions.
 template <int V, bool TAPS, bool B>
 __global__ __launch_bounds__(kBlock) void k_lambda(StepConst c, const uint32_t* __restrict__ order,
@@ -320,7 +337,7 @@ __global__ __launch_bounds__(kBlock) void k_lambda(StepConst c, const uint32_t* 
         nnbr = collect_flush<TAPS>(xi, c.h, domain_of<B>(c, k), cells, P, c.n, k, col, [&](uint32_t first, uint32_t cnt) {
 #pragma unroll 1
             for (uint32_t e = 0; e < cnt; ++e)
-                if (first + e < (uint32_t)kNbrCap) list[(size_t)t * kNbrCap + first + e] = col[e * kBlock];
+                if (first + e < (uint32_t)kNbrCap) list[list_slot(t, first + e)] = col[e * kBlock];
             walk_list(cnt, P, [&](uint32_t e) { return col[e * kBlock]; }, [&](uint32_t j, float4 xj) {
                 const V3 d = sub(xi, xyz(xj));
                 body(j, xj, d, dot(d, d));
@@ -377,12 +394,11 @@ __global__ __launch_bounds__(kBlock) void k_deltap(StepConst c, const uint32_t* 
     } else {
         const uint32_t cnt = list_count[t];
         if (cnt <= (uint32_t)kNbrCap) {
-            // this thread's row of the list: 16-byte aligned, four slots per load
-            const uint4* row = reinterpret_cast<const uint4*>(list + (size_t)t * kNbrCap);
+            // four slots per 16-byte load (list_slot keeps them together)
             uint32_t r = 0;
 #pragma unroll 1
             for (; r + 4 <= cnt; r += 4) {
-                const uint4 j = row[r >> 2];
+                const uint4 j = *reinterpret_cast<const uint4*>(list + list_slot(t, r));
                 const float4 a0 = PL[j.x], a1 = PL[j.y], a2 = PL[j.z], a3 = PL[j.w];
                 const V3 d0 = sub(xi, xyz(a0));
                 body(a0.w, d0, dot(d0, d0));
@@ -395,7 +411,7 @@ __global__ __launch_bounds__(kBlock) void k_deltap(StepConst c, const uint32_t* 
             }
 #pragma unroll 1
             for (; r < cnt; ++r) {
-                const float4 a = PL[list[(size_t)t * kNbrCap + r]];
+                const float4 a = PL[list[list_slot(t, r)]];
                 const V3 d = sub(xi, xyz(a));
                 body(a.w, d, dot(d, d));
             }
@@ -981,6 +997,9 @@ hipError_t launch_cells(const StepConst& c, DeviceState& d, hipStream_t st) {
     return hipGetLastError();
 }
 
+// The staged-tile gathers take Morton tiles of the processing order.
+static inline bool tiled_form(const DeviceState& d) { return d.variant == 2 && d.tiled && d.morton; }
+
 hipError_t launch_reorder(const StepConst& c, DeviceState& d, hipStream_t st) {
     hipLaunchKernelGGL(k_fill_u32, grid_for(kOrderBins), dim3(kBlock), 0, st, d.bins, kOrderBins, 0u);
 #define L(B)                                                                                                    \
@@ -990,8 +1009,11 @@ hipError_t launch_reorder(const StepConst& c, DeviceState& d, hipStream_t st) {
 #undef L
     hipError_t e = scan_exclusive_u32(d.bins, d.bins, kOrderBins, d.sort.partial, st);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_order, grid_for(c.n), dim3(kBlock), 0, st, c.n, d.bins, d.bin_of, d.bin_rank, d.order);
-    if (d.variant == 2 && d.tiled) {
+    if (d.morton)
+        hipLaunchKernelGGL(k_order, grid_for(c.n), dim3(kBlock), 0, st, c.n, d.bins, d.bin_of, d.bin_rank, d.order);
+    else
+        hipLaunchKernelGGL(k_order_slots, grid_for(c.n), dim3(kBlock), 0, st, c.n, d.order);
+    if (tiled_form(d)) {
         hipLaunchKernelGGL(k_tile_flags, grid_for(c.n), dim3(kBlock), 0, st, c.n, d.order, d.bin_of, d.tflag);
         e = scan_exclusive_u32(d.tflag, d.tidx, c.n, d.sort.partial, st);
         if (e != hipSuccess) return e;
@@ -1014,7 +1036,7 @@ hipError_t launch_lod(const StepConst& c, DeviceState& d, hipStream_t st) {
     }
 
 hipError_t launch_lambda(const StepConst& c, DeviceState& d, uint32_t iter, bool taps, hipStream_t st) {
-    if (d.variant == 2 && d.tiled) {
+    if (tiled_form(d)) {
         hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(kBlock), 0, st, d.ndefer, 1u, 0u);
 #define L(T, B)                                                                                                  \
     hipLaunchKernelGGL((k_lambda_tile<T, B>), dim3(kTileGrid), dim3(kBlock), 0, st, c, d.tiles, d.ntiles, d.order, \
@@ -1042,7 +1064,7 @@ hipError_t launch_lambda(const StepConst& c, DeviceState& d, uint32_t iter, bool
 }
 
 hipError_t launch_deltap(const StepConst& c, DeviceState& d, uint32_t iter, bool taps, hipStream_t st) {
-    if (d.variant == 2 && d.tiled) {
+    if (tiled_form(d)) {
         hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(kBlock), 0, st, d.ndefer, 1u, 0u);
 #define L(T, B)                                                                                                   \
     hipLaunchKernelGGL((k_deltap_tile<T, B>), dim3(kTileGrid), dim3(kBlock), 0, st, c, d.tiles, d.ntiles, d.order,   \

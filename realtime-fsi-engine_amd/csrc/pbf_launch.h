@@ -67,7 +67,7 @@ struct DeviceState {
     uint32_t* bin_rank = nullptr;  // sorted slot -> rank inside its bin
     uint32_t* order = nullptr; // processing order: thread t handles sorted slot order[t]
     uint32_t* list_count = nullptr;  // by thread: neighbours found by the last lambda pass
-    uint32_t* list = nullptr;  // n x kNbrCap: neighbour slots, one row per thread
+    uint32_t* list = nullptr;  // kNbrCap slots per thread, n rounded up to 64 (layout: list_slot in pbf_kernels.hip)
     uint32_t* tflag = nullptr; // n: 1 where a tile starts in the processing order
     uint32_t* tidx = nullptr;  // n: exclusive scan of tflag
     uint32_t* tiles = nullptr; // n + 1: processing-order position where tile i starts (then the end)
@@ -76,6 +76,7 @@ struct DeviceState {
     uint32_t* ndefer = nullptr;  // 1
     SortScratch sort{};
     int variant = 1;           // gather form (tuning knob, see gather() in pbf_kernels.hip)
+    bool morton = true;        // processing order: Morton order of the cells (else sorted slots)
     bool tiled = false;        // every domain bins by 2^kTileShift cells (tiled gathers possible)
     bool batched = false;      // several scenes (StepConst::doms / scene_of)
     bool any_sdf = false;      // some domain has an SDF boundary

@@ -24,7 +24,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libfsi_pbf.so")
 
 FSI_OK = 0
-TUNE_GATHER, TUNE_SORT_DIGIT_BITS = 0, 1
+TUNE_GATHER, TUNE_SORT_DIGIT_BITS, TUNE_ORDER = 0, 1, 2
 PASS_NAMES = ["predict", "sort", "tables", "lambda", "deltap", "integrate", "vorticity", "lod"]
 
 

@@ -4,6 +4,11 @@
       per-kernel calls / average / share from <trace_dir>/run_kernel_stats.csv
   python rocprof_summary.py counters <pmc_dir> --cmd "<command>"
       per-kernel average of every counter of one --pmc run (+ VALU per wave)
+  python rocprof_summary.py caches --run LABEL=<cache_dir>,<fetch_dir> [--run ...] --fetch-factor F
+                            --cmd "<command>" [--cmd ...]
+      per-kernel L1 and L2 hit rates (TCP_TOTAL_CACHE_ACCESSES_sum,
+      TCP_TCC_READ_REQ_sum, TCC_HIT_sum, TCC_MISS_sum) and HBM read bytes
+      (FETCH_SIZE x F) of several labelled runs side by side
   python rocprof_summary.py traffic <fetch_dir> <write_dir> --calib-fetch <dir> --calib-write <dir>
                             --calib-bytes B --cmd "<command>" [--json out.json]
       HBM bytes per dispatch of each kernel: FETCH_SIZE and WRITE_SIZE (KiB per
@@ -123,6 +128,36 @@ def counters(args):
         print(line + " |")
 
 
+def caches(args):
+    """L1 / L2 hit rates and HBM reads of each kernel across labelled runs."""
+    runs = []
+    for spec in args.run:
+        label, dirs = spec.split("=", 1)
+        cdir, fdir = dirs.split(",")
+        hit, calls = counter_means(cdir, "TCC_HIT_sum")
+        miss, _ = counter_means(cdir, "TCC_MISS_sum")
+        l1, _ = counter_means(cdir, "TCP_TOTAL_CACHE_ACCESSES_sum")
+        l1m, _ = counter_means(cdir, "TCP_TCC_READ_REQ_sum")
+        fetch, _ = counter_means(fdir, "FETCH_SIZE")
+        runs.append((label, hit, miss, l1, l1m, fetch, calls))
+    print("# L1 / L2 hit rates and HBM reads per dispatch\n\nCommands:\n")
+    for c in args.cmd:
+        print(f"    {c}")
+    print(f"\nHBM read bytes = FETCH_SIZE (KiB) x {args.fetch_factor:.1f}, the factor of the copy-kernel calibration.\n")
+    print("| kernel | pass | run | dispatches | L1 hit % | L2 hit % | L2 requests | HBM read MB |")
+    print("|---|---|---|---|---|---|---|---|")
+    names = sorted({k for r in runs for k in r[1]}, key=lambda k: (-max(r[5].get(k, 0.0) for r in runs), k))
+    for k in names:
+        for label, hit, miss, l1, l1m, fetch, calls in runs:
+            if k not in hit:
+                continue
+            req = hit[k] + miss.get(k, 0.0)
+            l2 = 100.0 * hit[k] / req if req else 0.0
+            l1r = 100.0 * (1.0 - l1m.get(k, 0.0) / l1[k]) if l1.get(k) else 0.0
+            mb = fetch.get(k, 0.0) * args.fetch_factor / 1e6
+            print(f"| `{k}` | {pass_of(k)} | {label} | {calls[k]} | {l1r:.1f} | {l2:.1f} | {req:.4g} | {mb:.2f} |")
+
+
 def main():
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="what", required=True)
@@ -141,8 +176,12 @@ def main():
     c = sub.add_parser("counters")
     c.add_argument("dir")
     c.add_argument("--cmd", required=True)
+    h = sub.add_parser("caches")
+    h.add_argument("--run", action="append", required=True)
+    h.add_argument("--fetch-factor", type=float, required=True)
+    h.add_argument("--cmd", action="append", required=True)
     a = ap.parse_args()
-    {"kernels": kernels, "traffic": traffic, "counters": counters}[a.what](a)
+    {"kernels": kernels, "traffic": traffic, "counters": counters, "caches": caches}[a.what](a)
 
 
 if __name__ == "__main__":

@@ -270,8 +270,9 @@ def test_full_size_properties(gpu):
 
 @pytest.mark.parametrize("graph", [False, True])
 def test_knobs_and_graph_do_not_change_results(gpu, graph):
-    """Every gather form and radix digit width, direct or replayed from a HIP
-    graph, gives the same bits (the oracle fixes the summation order)."""
+    """Every gather form, radix digit width and processing order, direct or
+    replayed from a HIP graph, gives the same bits (the oracle fixes the
+    summation order)."""
     fsi = _fsi()
     cfg = dam_break_config(n_iter=3, substeps=2, hash_size=1 << 14)
     parts = fsi.lattice(24, 20, 16, (0.0, 0.0, 0.0), 0.05, 0.001, 99)
@@ -280,20 +281,23 @@ def test_knobs_and_graph_do_not_change_results(gpu, graph):
     ref = _oracle().OracleSim(cfg, parts, taps=False, sdf=sdf, sdf_padding=0.025)
     ref.step(2)
     exp = ref.particles()
-    for gather in (0, 1, 2):
-        for bits in (6, 8, 10):
-            with fsi.Solver(cfg) as s:
-                s.set_particles(parts)
-                s.set_sdf(sdf, 0.025, False)
-                s.set_tuning(fsi.TUNE_GATHER, gather)
-                s.set_tuning(fsi.TUNE_SORT_DIGIT_BITS, bits)
-                s.set_graph(graph)
-                s.step(2)
-                got = s.read_particles()
-                keys, ids, cells = s.read_grid()
-            assert np.array_equal(keys, ref.keys) and np.array_equal(ids, ref.ids), (gather, bits)
-            assert np.array_equal(cells, ref.cells), (gather, bits)
-            assert np.abs(got[:, 0:3] - exp[:, 0:3]).max() <= 1e-5 * cfg.h, (gather, bits)
+    knobs = [(g, b, 1) for g in (0, 1, 2) for b in (6, 8, 10)] + [(g, 8, 0) for g in (0, 1, 2)]
+    for gather, bits, order in knobs:
+        with fsi.Solver(cfg) as s:
+            s.set_particles(parts)
+            s.set_sdf(sdf, 0.025, False)
+            s.set_tuning(fsi.TUNE_GATHER, gather)
+            s.set_tuning(fsi.TUNE_SORT_DIGIT_BITS, bits)
+            s.set_tuning(fsi.TUNE_ORDER, order)
+            assert s.get_tuning(fsi.TUNE_ORDER) == order
+            s.set_graph(graph)
+            s.step(2)
+            got = s.read_particles()
+            keys, ids, cells = s.read_grid()
+        case = (gather, bits, order)
+        assert np.array_equal(keys, ref.keys) and np.array_equal(ids, ref.ids), case
+        assert np.array_equal(cells, ref.cells), case
+        assert np.abs(got[:, 0:3] - exp[:, 0:3]).max() <= 1e-5 * cfg.h, case
 
 
 def test_density_error_report_against_fp64(gpu):
