@@ -4,6 +4,9 @@
       per-kernel calls / average / share from <trace_dir>/run_kernel_stats.csv
   python rocprof_summary.py counters <pmc_dir> --cmd "<command>"
       per-kernel average of every counter of one --pmc run (+ VALU per wave)
+  python rocprof_summary.py derived <pmcA_dir> <pmcB_dir>
+      VALU instructions, active lanes, VALU-busy share, loads and LDS
+      instructions per wave of the gather kernels from the two SQ passes
   python rocprof_summary.py caches --run LABEL=<cache_dir>,<fetch_dir> [--run ...] --fetch-factor F
                             --cmd "<command>" [--cmd ...]
       per-kernel L1 and L2 hit rates (TCP_TOTAL_CACHE_ACCESSES_sum,
@@ -128,6 +131,31 @@ def counters(args):
         print(line + " |")
 
 
+def derived(args):
+    """Per-wave rates of the gather kernels from two --pmc runs (A: SQ_WAVES,
+    SQ_INSTS_VALU, SQ_ACTIVE_INST_VALU, SQ_BUSY_CYCLES, SQ_INSTS_VMEM_RD,
+    SQ_INSTS_LDS; B: SQ_THREAD_CYCLES_VALU), on a device of `--simds` SIMDs
+    and `--engines` shader engines."""
+    a = collections.defaultdict(lambda: collections.defaultdict(list))
+    for d in (args.dir_a, args.dir_b):
+        for r in csv.DictReader(open(os.path.join(d, "run_counter_collection.csv"))):
+            a[short_name(r["Kernel_Name"])][r["Counter_Name"]].append(float(r["Counter_Value"]))
+    print("| kernel | VALU inst / wave | thread-cycles / VALU inst | VALU-active wave cycles / SIMD cycles | "
+          "vector loads / wave | LDS inst / wave |")
+    print("|---|---|---|---|---|---|")
+    m = {k: {c: sum(v) / len(v) for c, v in cs.items()} for k, cs in a.items()}
+    for k in sorted(m, key=lambda k: -m[k].get("SQ_INSTS_VALU", 0.0)):
+        c = m[k]
+        w, v = c.get("SQ_WAVES", 0.0), c.get("SQ_INSTS_VALU", 0.0)
+        if not w or not v or not re.match(args.match, k):
+            continue
+        lanes = c.get("SQ_THREAD_CYCLES_VALU", 0.0) / v
+        cycles = c.get("SQ_BUSY_CYCLES", 0.0) / args.engines
+        busy = 4.0 * c.get("SQ_ACTIVE_INST_VALU", 0.0) / (cycles * args.simds) if cycles else 0.0
+        print(f"| `{k}` | {v / w:.0f} | {lanes:.1f} of 64 ({lanes / 64:.0%}) | {busy:.0%} | "
+              f"{c.get('SQ_INSTS_VMEM_RD', 0.0) / w:.0f} | {c.get('SQ_INSTS_LDS', 0.0) / w:.0f} |")
+
+
 def caches(args):
     """L1 / L2 hit rates and HBM reads of each kernel across labelled runs."""
     runs = []
@@ -176,12 +204,19 @@ def main():
     c = sub.add_parser("counters")
     c.add_argument("dir")
     c.add_argument("--cmd", required=True)
+    dv = sub.add_parser("derived")
+    dv.add_argument("dir_a")
+    dv.add_argument("dir_b")
+    dv.add_argument("--simds", type=int, default=1024)
+    dv.add_argument("--engines", type=int, default=32)
+    dv.add_argument("--match", default="k_(lambda|deltap|integrate|vorticity|apply_sdf)")
     h = sub.add_parser("caches")
     h.add_argument("--run", action="append", required=True)
     h.add_argument("--fetch-factor", type=float, required=True)
     h.add_argument("--cmd", action="append", required=True)
     a = ap.parse_args()
-    {"kernels": kernels, "traffic": traffic, "counters": counters, "caches": caches}[a.what](a)
+    {"kernels": kernels, "traffic": traffic, "counters": counters, "caches": caches,
+     "derived": derived}[a.what](a)
 
 
 if __name__ == "__main__":
