@@ -496,15 +496,18 @@ __global__ __launch_bounds__(kBlock) void k_integrate(StepConst c, const uint32_
 // The rest of CS_Integrate.glsl:206-280 per particle: AABB clamp, SDF
 // projection, velocity from the displacement plus the XSPH change, speed
 // clamp, wall damping, removal of the normal velocity at an SDF contact; the
-// committed state goes to pos / vel (id order) and posS (sorted).  A kernel
-// of its own so that the trilinear SDF samples do not raise the register
-// count of the gather.
+// committed state goes to pos / vel (id order) and posS (sorted).  posS.w
+// carries |curl| (the |omega_j| of CS_VorticityConfinement.glsl:156, the same
+// FP32 length), so that the vorticity gather reads one record per neighbour.
+// A kernel of its own so that the trilinear SDF samples do not raise the
+// register count of the gather.
 template <bool B>
 __global__ __launch_bounds__(kBlock) void k_integrate_finish(StepConst c, const uint32_t* __restrict__ order,
                                                              const float4* __restrict__ P,
                                                              const float4* __restrict__ vs,
                                                              const float4* __restrict__ vdelta,
                                                              const uint32_t* __restrict__ perm,
+                                                             const float4* __restrict__ curl,
                                                              float4* __restrict__ pos, float4* __restrict__ vel,
                                                              float4* __restrict__ posS) {
     const uint32_t t = xcd_gid();
@@ -542,7 +545,8 @@ __global__ __launch_bounds__(kBlock) void k_integrate_finish(StepConst c, const 
     }
     pos[id] = make_float4(xp.x, xp.y, xp.z, op4.w);
     vel[id] = make_float4(v.x, v.y, v.z, vw);
-    posS[k] = make_float4(xp.x, xp.y, xp.z, 0.0f);
+    const float om = c.enable_vort == 1u ? len(xyz(curl[k])) : 0.0f;
+    posS[k] = make_float4(xp.x, xp.y, xp.z, om);
 }
 
 // CS_VorticityConfinement.glsl:133-167 (pass 1)
@@ -559,10 +563,9 @@ __global__ __launch_bounds__(kBlock) void k_vorticity(StepConst c, const uint32_
     V3 eta = mk(0.0f, 0.0f, 0.0f);
     uint32_t ncand = 0, nnbr = 0;
     __shared__ uint32_t lst[kLdsCap * kBlock];
-    gather<V, false>(pi, c.h, domain_of<B>(c, k), cells, posS, c.n, k, &lst[threadIdx.x], [&](uint32_t j, float4, V3 r, float r2) {
+    gather<V, false>(pi, c.h, domain_of<B>(c, k), cells, posS, c.n, k, &lst[threadIdx.x], [&](uint32_t, float4 xj, V3 r, float r2) {
         const V3 gw = spiky(c, r, r2);
-        const float om = len(xyz(curl[j]));
-        eta = add(eta, scl(gw, om));
+        eta = add(eta, scl(gw, xj.w));  // xj.w = |omega_j|
     }, ncand, nnbr);
     eta = scl(eta, c.inv_rho0);
     const float el = len(eta);
@@ -1106,7 +1109,7 @@ hipError_t launch_integrate(const StepConst& c, DeviceState& d, hipStream_t st) 
     }
 #define L(B)                                                                                                   \
     hipLaunchKernelGGL((k_integrate_finish<B>), grid_for(c.n), dim3(kBlock), 0, st, c, d.order, d.P, d.vs, d.vdelta, \
-                       d.perm, d.pos, d.vel, d.posS);
+                       d.perm, d.curl, d.pos, d.vel, d.posS);
     FSI_B_SWITCH(d.batched, L)
 #undef L
     return hipGetLastError();
