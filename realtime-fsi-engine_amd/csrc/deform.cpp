@@ -512,10 +512,15 @@ bool Laplacian::precompute_local() {
     }
     L.handle_local.clear();
     L.handle_global.clear();
+    L.handle_ctrl.clear();
+    std::vector<int> first((size_t)n(), -1);
+    for (size_t t = 0; t < ctrl_.size(); ++t)
+        if (first[(size_t)ctrl_[t]] < 0) first[(size_t)ctrl_[t]] = (int)t;
     for (int h : ctrl_)
         if (L.g2l[(size_t)h] >= 0) {
             L.handle_local.push_back(L.g2l[(size_t)h]);
             L.handle_global.push_back(h);
+            L.handle_ctrl.push_back(first[(size_t)h]);
         }
     if (L.handle_local.empty()) return true;
     // Lii^T Lii + I_handles (a handle listed twice counts twice)
@@ -541,9 +546,15 @@ bool Laplacian::solve_local() {
         gemv_t(L.Lii, b.data(), rhs[a].data());
     }
     for (size_t c = 0; c < L.handle_local.size(); ++c) {
+        // the target of the handle's first occurrence in the control list
+        // (found again if the list was reordered since the precompute)
         const int gv = L.handle_global[c];
-        size_t t = 0;
-        while (t < ctrl_.size() && ctrl_[t] != gv) ++t;
+        size_t t = (size_t)L.handle_ctrl[c];
+        if (t >= ctrl_.size() || ctrl_[t] != gv) {
+            t = 0;
+            while (t < ctrl_.size() && ctrl_[t] != gv) ++t;
+        }
+        if (t >= ctrl_.size()) continue;  // no longer a control point
         for (int a = 0; a < 3; ++a) rhs[a][(size_t)L.handle_local[c]] += ctrl_t_[3 * t + (size_t)a];
     }
     for (int a = 0; a < 3; ++a) {

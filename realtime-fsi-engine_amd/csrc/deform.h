@@ -142,6 +142,7 @@ private:
         Csc Lii, Lib, LiiT, LtL;             // LtL = Lii^T Lii
         std::vector<int> diag;               // position of each diagonal entry in LtL.x
         std::vector<int> handle_local, handle_global;
+        std::vector<int> handle_ctrl;        // index in ctrl_ of each handle's first occurrence
         Csc AtA;                             // LtL + I_handles
         Ldlt ldlt;
         bool analyzed = false, ready = false;
