@@ -20,7 +20,10 @@ from typing import Optional
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+# FSI_ORACLE_LIB names another build of these same sources: the mutation test
+# (tests/test_oracle_mutants.py) runs the pins against deliberately broken
+# builds to show that each pin fails on the mistake it guards against.
+LIB_PATH = os.environ.get("FSI_ORACLE_LIB") or os.path.join(HERE, "_build", "liboracle.so")
 sys.path.insert(0, os.path.join(os.path.dirname(HERE), "realtime-fsi-engine_amd"))
 from fsi_pbf import CFluidConfig, FluidConfig, SdfGrid  # noqa: E402  (data formats only)
 
@@ -226,18 +229,10 @@ class OracleSim:
                 u.interaction_pos[i] = f32(interaction[i])
         for i in range(3):
             u.camera_pos[i] = f32(camera[i])
+        self.uni = u
         self._grid = None
         if sdf is not None:
-            self._grid = np.ascontiguousarray(sdf.grid, dtype=np.float32)
-            u.enable_sdf = 1
-            u.sdf_mode = 1 if containment else 0
-            for i in range(3):
-                u.sdf_dims[i] = sdf.dims[i]
-                u.sdf_origin[i] = f32(sdf.origin[i])
-            u.sdf_cell = f32(sdf.cell)
-            u.sdf_padding = f32(sdf_padding)
-            u.sdf_grid = self._grid.ctypes.data_as(C.POINTER(C.c_float))
-        self.uni = u
+            self.set_sdf(sdf, sdf_padding, containment)
         st = OracleState()
         st.n = n
         st.hash_size = self.H
@@ -253,6 +248,24 @@ class OracleSim:
             st.cand_count = self.cand.ctypes.data_as(C.POINTER(C.c_uint32))
             st.nbr_count = self.nbr.ctypes.data_as(C.POINTER(C.c_uint32))
         self.st = st
+
+    def set_sdf(self, sdf: Optional[SdfGrid], padding: float = 0.0, containment: bool = False):
+        """PBFluids::setSDFBoundary / setSDFPadding / setSDFContainment; None disables."""
+        u = self.uni
+        if sdf is None:
+            self._grid = None
+            u.enable_sdf = 0
+            u.sdf_grid = None
+            return
+        self._grid = np.ascontiguousarray(sdf.grid, dtype=np.float32)
+        u.enable_sdf = 1
+        u.sdf_mode = 1 if containment else 0
+        for i in range(3):
+            u.sdf_dims[i] = sdf.dims[i]
+            u.sdf_origin[i] = f32(sdf.origin[i])
+        u.sdf_cell = f32(sdf.cell)
+        u.sdf_padding = f32(padding)
+        u.sdf_grid = self._grid.ctypes.data_as(C.POINTER(C.c_float))
 
     def _call(self, name, *args):
         return getattr(self.lib, f"oracle_{name}_{self.p}")(*args)

@@ -495,3 +495,147 @@ def test_interaction_force_closed_form():
             v = v + to / dist * 25.0 * (1.0 - dist / 1.5) * dt
         np.testing.assert_allclose(sim.vel[i, 0:3], v, rtol=0, atol=1e-12)
     assert np.linalg.norm(parts[2, 0:3] - ip) > 1.5  # the third is outside the radius: gravity only
+
+
+# ---------------------------------------------------------------- value pins of lambda, delta-p, the two clamps
+#
+# Closed forms from the published method (Mueller et al. 2003 for the kernels,
+# Macklin & Mueller 2013 for the constraint), evaluated here in FP64 from the
+# FP32 inputs: W(r) = 315/(64 pi h^9) (h^2 - r^2)^3, grad W(d) = -45/(pi h^6)
+# (h - r)^2 d/r, lambda_i = -C_i / (sum_k |grad_k C_i|^2 + eps) with
+# grad_i C_i = (1/rho0) sum_j grad W_ij and grad_j C_i = -(1/rho0) grad W_ij,
+# s_corr = -k (W(r)/W(dq h))^n and
+# dp_i = (1/rho0) sum_j (lambda_i + lambda_j + s_corr) grad W_ij.
+# Each case is chosen so that a plausible slip (a dropped 1/rho0, a missing
+# |grad_i C_i|^2 term, a flipped s_corr sign, exponent n - 1, a missing clamp)
+# moves the pinned value by far more than the tolerance;
+# tests/test_oracle_mutants.py plants those slips in the oracle and checks
+# that these tests then fail.
+
+PRECISIONS = [("f64", 1e-12), ("f32", 3e-5)]
+
+
+def _f(x):
+    return float(np.float32(x))
+
+
+def _w_poly6(r, h):
+    return 315.0 / (64.0 * math.pi * h ** 9) * (h * h - r * r) ** 3 if r <= h else 0.0
+
+
+def _grad_spiky(d, h):
+    r = float(np.linalg.norm(d))
+    return -45.0 / (math.pi * h ** 6) * (h - r) ** 2 * d / r
+
+
+def _pair_case(prec, d, **kw):
+    """Two particles at rest, no gravity, x1 = x0 - d; returns the oracle after
+    predict, sort and cell tables (x* = x) and the FP64 inputs."""
+    kw.setdefault("gravity", (0.0, 0.0, 0.0))
+    cfg = dam_break_config(substeps=1, n_iter=1, box=(4.0, 4.0, 4.0), hash_size=1 << 20, **kw)
+    parts = np.zeros((2, 12), dtype=np.float32)
+    parts[0, 0:3] = [1.5, 1.5, 1.5]
+    parts[1, 0:3] = parts[0, 0:3] - np.asarray(d, dtype=np.float32)
+    parts[:, 4:7] = parts[:, 0:3]
+    sim = O().OracleSim(cfg, parts, precision=prec)
+    sim.predict()
+    sim.sort()
+    sim.build_cells()
+    x = parts[:, 0:3].astype(np.float64)
+    return cfg, sim, x
+
+
+def _pair_lambda(cfg, x):
+    h, rho0, eps = _f(cfg.h), _f(cfg.rho0), _f(cfg.eps)
+    d = x[0] - x[1]
+    r = float(np.linalg.norm(d))
+    rho = _w_poly6(0.0, h) + _w_poly6(r, h)
+    g = _grad_spiky(d, h)
+    C_ = max(rho / rho0 - 1.0, -_f(cfg.cohesion_strength))
+    sum_j = float(g @ g) / rho0 ** 2          # sum_j |grad_j C_i|^2 (one neighbour)
+    grad_i = g / rho0                         # grad_i C_i
+    lam = -C_ / (sum_j + float(grad_i @ grad_i) + eps)
+    return rho, lam, g, d, r
+
+
+def _rel(got, exp):
+    got, exp = np.asarray(got, dtype=np.float64), np.asarray(exp, dtype=np.float64)
+    return float(np.linalg.norm(got - exp) / np.linalg.norm(exp))
+
+
+@pytest.mark.parametrize("prec,tol", PRECISIONS)
+@pytest.mark.parametrize("rho0,eps,coh", [(1000.0, 100.0, 0.001), (6378.0, 1.0, 0.001), (6378.0, 600.0, 0.2)])
+def test_lambda_of_a_pair_closed_form(prec, tol, rho0, eps, coh):
+    """CS_ComputeLambdas.glsl:136-158: rho = W(0) + W(r) and
+    lambda = -max(rho/rho0 - 1, -cohesion) / (2 |grad W|^2 / rho0^2 + eps).
+    The first case is compressed (C > 0), the others sit on the cohesion floor."""
+    cfg, sim, x = _pair_case(prec, (0.031, -0.017, 0.009), rho0=rho0, eps=eps, cohesion_strength=coh)
+    sim.lambda_pass(1)
+    rho, lam, g, _, _ = _pair_lambda(cfg, x)
+    # the case exercises both denominator terms against eps
+    assert float(g @ g) / rho0 ** 2 > 0.5 * eps or rho0 == 6378.0 and eps == 600.0
+    for i in (0, 1):
+        assert float(sim.solver[i, 7]) == pytest.approx(rho, rel=tol)
+        assert float(sim.solver[i, 3]) == pytest.approx(lam, rel=tol)
+
+
+@pytest.mark.parametrize("prec,tol", PRECISIONS)
+@pytest.mark.parametrize("k,n,dq,scorr", [(1e-4, 4.0, 0.3, True), (2e-4, 2.5, 0.5, True), (1e-4, 4.0, 0.3, False)])
+def test_position_correction_of_a_pair_closed_form(prec, tol, k, n, dq, scorr):
+    """CS_ComputeDeltaP.glsl:94-104,150-165: dp_0 = (1/rho0) (2 lambda + s) grad W
+    with s = -k (W(r)/W(dq h))^n (integer and fractional n), and dp_1 = -dp_0."""
+    cfg, sim, x = _pair_case(prec, (0.012, -0.009, 0.011), rho0=1000.0, eps=100.0, k_corr=k, n_corr=n,
+                             delta_q=dq, enable_scorr=scorr)
+    sim.lambda_pass(1)
+    sim.deltap_pass(1)
+    h, rho0 = _f(cfg.h), _f(cfg.rho0)
+    _, lam, g, _, r = _pair_lambda(cfg, x)
+    dqh = _f(dq) * h  # FP64 product of the FP32 inputs; its FP32 rounding is below the f32 tolerance
+    s = -_f(k) * (_w_poly6(r, h) / _w_poly6(dqh, h)) ** _f(n) if scorr else 0.0
+    if scorr:
+        assert abs(s) > 0.5 * abs(2 * lam) and _w_poly6(r, h) > 1.1 * _w_poly6(dqh, h)
+    dp0 = (2.0 * lam + s) * g / rho0
+    assert _rel(sim.solver[0, 4:7], dp0) <= tol
+    assert _rel(sim.solver[1, 4:7], -dp0) <= tol
+
+
+@pytest.mark.parametrize("prec,tol", PRECISIONS)
+@pytest.mark.parametrize("eps", [1e-3, 1e5])
+def test_position_correction_is_clamped_to_half_h(prec, tol, eps):
+    """CS_ApplyDeltaP.glsl:92-104: |dp| > h/2 is scaled back to h/2 along dp;
+    below it x + dp is applied as is.  eps = 1e-3 gives |dp| ~ 0.4 = 8 h/2,
+    eps = 1e5 gives |dp| ~ 0.01 < h/2."""
+    cfg, sim, x = _pair_case(prec, (0.06, 0.05, 0.03), rho0=100.0, eps=eps, enable_scorr=False)
+    sim.lambda_pass(1)
+    sim.deltap_pass(1)
+    h, rho0 = _f(cfg.h), _f(cfg.rho0)
+    _, lam, g, _, _ = _pair_lambda(cfg, x)
+    dp0 = 2.0 * lam * g / rho0
+    assert _rel(sim.solver[0, 4:7], dp0) <= tol
+    big = np.linalg.norm(dp0) > 0.5 * h
+    assert big == (eps < 1.0)
+    step = dp0 * (0.5 * h / np.linalg.norm(dp0)) if big else dp0
+    sim.apply_pass(1)
+    for i, sgn in ((0, 1.0), (1, -1.0)):
+        moved = sim.solver[i, 0:3].astype(np.float64) - x[i]
+        assert _rel(moved, sgn * step) <= max(tol, 2e-5 if prec == "f32" else tol)
+
+
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-9), ("f32", 1e-4)])
+@pytest.mark.parametrize("v0", [(60.0, 30.0, -20.0), (40.0, 20.0, -10.0), (0.0, -51.0, 0.0)])
+def test_speed_is_clamped_to_50(prec, tol, v0):
+    """CS_Integrate.glsl:244-256: v = (x* - x)/dt, and a speed above 50 is
+    scaled back to 50 along v (72.8 and 51 m/s); 45.8 m/s passes unchanged."""
+    cfg = dam_break_config(substeps=1, n_iter=1, box=(100.0, 100.0, 100.0), gravity=(0.0, 0.0, 0.0),
+                           hash_size=1 << 20)
+    parts = np.zeros((1, 12), dtype=np.float32)
+    parts[0, 0:3] = [1.0, 2.0, 1.0]
+    parts[0, 4:7] = parts[0, 0:3]
+    parts[0, 8:11] = v0
+    sim = O().OracleSim(cfg, parts, precision=prec)
+    sim.step(1)
+    v = np.asarray(v0, dtype=np.float64)
+    speed = float(np.linalg.norm(v))
+    expect = v * (50.0 / speed) if speed > 50.0 else v
+    assert _rel(sim.vel[0, 0:3], expect) <= tol
+    assert _rel(sim.pos[0, 0:3], parts[0, 0:3] + v * _f(cfg.dt)) <= tol  # x* = x + v dt, dt the FP32 1/120
