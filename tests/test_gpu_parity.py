@@ -329,3 +329,63 @@ def test_density_error_report_against_fp64(gpu):
     for f, mg, mt, xg, xt in rows:
         assert np.isfinite(mg) and np.isfinite(xg)
         assert mg == pytest.approx(mt, rel=1e-4 if f <= 5 else 0.05), f"frame {f}"
+
+
+@pytest.mark.parametrize("case", ["dense_lattice", "aliased_hash"])
+def test_more_neighbours_than_the_list_holds(gpu, case):
+    """Particles with more than 64 neighbours: the lambda pass keeps the first
+    64 in its list and the delta-p pass walks the 27 buckets again for them.
+    A lattice at spacing 0.3 h (~150 neighbours), and a hash of 8 buckets
+    (every bucket visited several times: duplicates count)."""
+    fsi = _fsi()
+    if case == "dense_lattice":
+        cfg = dam_break_config(n_iter=3, substeps=1, hash_size=1 << 14, box=(1.6, 1.6, 1.6))
+        parts = fsi.lattice(16, 16, 16, (0.2, 0.2, 0.2), 0.03, 0.001, 21)
+    else:
+        cfg = dam_break_config(n_iter=2, substeps=1, hash_size=1 << 3, box=(1.6, 1.6, 1.6))
+        parts = fsi.lattice(12, 12, 12, (0.2, 0.2, 0.2), 0.05, 0.001, 22)
+    s, ref = run_pair(cfg, parts)
+    assert_grid_equal(s, ref)
+    cand, nbr = s.read_counts()
+    assert np.array_equal(cand, ref.cand) and np.array_equal(nbr, ref.nbr)
+    over = int((nbr > 64).sum())
+    print(f"{case}: {over} of {len(nbr)} particles have more than 64 neighbours (max {nbr.max()})")
+    assert over > 100 and (nbr <= 64).any()
+    assert_state_close(s, ref, cfg, label=case)
+    s.close()
+
+
+def _closed_form_cases():
+    # (name, config overrides, particle x list, velocity list): the value pins
+    # of the oracle (tests/test_oracle_pins.py), run through the full substep
+    x0 = np.array([1.5, 1.5, 1.5], dtype=np.float32)
+    pair = lambda d: [x0, x0 - np.asarray(d, dtype=np.float32)]  # noqa: E731
+    yield "lambda_compressed", dict(rho0=1000.0, eps=100.0), pair((0.031, -0.017, 0.009)), None
+    yield "lambda_cohesion_floor", dict(rho0=6378.0, eps=1.0), pair((0.031, -0.017, 0.009)), None
+    yield "scorr_n4", dict(rho0=1000.0, eps=100.0, k_corr=1e-4, n_corr=4.0), pair((0.012, -0.009, 0.011)), None
+    yield "scorr_n2.5", dict(rho0=1000.0, eps=100.0, k_corr=2e-4, n_corr=2.5, delta_q=0.5), \
+        pair((0.012, -0.009, 0.011)), None
+    yield "dp_clamp", dict(rho0=100.0, eps=1e-3, enable_scorr=False), pair((0.06, 0.05, 0.03)), None
+    yield "speed_clamp", dict(box=(100.0, 100.0, 100.0)), [np.array([1.0, 2.0, 1.0], np.float32)], \
+        [np.array([60.0, 30.0, -20.0], np.float32)]
+
+
+@pytest.mark.parametrize("name,kw,xs,vs", list(_closed_form_cases()), ids=[c[0] for c in _closed_form_cases()])
+def test_closed_form_cases(gpu, name, kw, xs, vs):
+    """The isolated pair and single-particle cases whose oracle values are
+    pinned by closed forms (lambda's denominator, s_corr, the |dp| <= h/2
+    clamp, the speed clamp), stepped on the GPU: the same bits as the oracle."""
+    args = dict(substeps=1, n_iter=1, box=(4.0, 4.0, 4.0), hash_size=1 << 20, gravity=(0.0, 0.0, 0.0))
+    args.update(kw)
+    cfg = dam_break_config(**args)
+    parts = np.zeros((len(xs), 12), dtype=np.float32)
+    parts[:, 0:3] = np.stack(xs)
+    parts[:, 4:7] = parts[:, 0:3]
+    if vs is not None:
+        parts[:, 8:11] = np.stack(vs)
+    s, ref = run_pair(cfg, parts, taps=False)
+    got, exp = s.read_particles(), ref.particles()
+    print(f"{name}: positions equal {np.array_equal(got[:, 0:3], exp[:, 0:3])}, "
+          f"velocities equal {np.array_equal(got[:, 8:11], exp[:, 8:11])}")
+    assert np.array_equal(got, exp)
+    s.close()
