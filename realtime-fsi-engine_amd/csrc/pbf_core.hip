@@ -162,6 +162,7 @@ StepConst step_const_common(const fsi_fluid_config& cfg, const fsi_pbf_constants
     c.enable_vort = cfg.enable_vorticity ? 1u : 0u;
     c.interacting = interacting ? 1u : 0u;
     c.vort_eps = cfg.vorticity_epsilon;
+    c.qscale = kQUnitsPerH / u.h;
     return c;
 }
 
@@ -208,6 +209,7 @@ void core_destroy(Core& c) {
     dev_free(&d.keys);
     dev_free(&d.cells);
     dev_free(&d.P);
+    dev_free(&d.Q);
     dev_free(&d.PL);
     dev_free(&d.vs);
     dev_free(&d.curl);
@@ -272,6 +274,7 @@ hipError_t core_alloc_particles(Core& c, uint32_t n) {
     ALLOC(d.xstar, n);
     ALLOC(d.keys, n);
     ALLOC(d.P, n);
+    ALLOC(d.Q, (size_t)n + 4);  // a group of four candidate loads may reach 3 entries past the last
     ALLOC(d.PL, n);
     ALLOC(d.vs, n);
     ALLOC(d.curl, n);

@@ -42,6 +42,7 @@ struct StepConst {
     uint32_t min_lod, max_lod, enable_apbf, enable_vort;
     uint32_t interacting;
     float ipos[3], cam[3], vort_eps;
+    float qscale;               // kQUnitsPerH / h: quantisation of the candidate pre-test
     Domain dom;                 // the single scene of the plain solver
     const Domain* doms;         // batched: one per scene
     const uint16_t* scene_of;   // batched: scene of particle id i (= of sorted slot i)
@@ -141,6 +142,54 @@ __device__ __forceinline__ float sqrt_rn_normal(float x) {
     float r = __builtin_fmaf(-lo, s, x) <= 0.0f ? lo : s;
     r = __builtin_fmaf(-hi, s, x) > 0.0f ? hi : r;
     return r;
+}
+
+// ---- conservative candidate pre-test of the gathers ----
+// Every gather tests the candidates of 27 buckets against the support
+// r^2 <= h^2 (CS_ComputeLambdas.glsl:76,86); about 85% fail.  The scan first
+// tests them on quantised positions, which costs fewer instructions and half
+// the bytes, and hands only the candidates that pass to the exact FP32 test
+// (in the reference's visit order, so the sums are unchanged).
+//
+// Quantisation: t = fl(x * s) with s = fl(1024 / h), q = floor(t) kept modulo
+// 2^16 per axis (x | y << 16 in one word, z in the low half of another), so
+// the offset of two particles is a 16-bit wrapping difference and its square
+// sum a packed dot product (saturating).  While |t| < 2^22 (ulp(t) <= 1/2), q
+// is within 1.25 units of t, and within 1.5 units for a neighbour j of such a
+// particle (|t_j| < 2^23).  A pair that passes the exact FP32 test has
+// r <= h (1 + 1e-6), i.e. |dq| <= 1024.01 + 2.75 sqrt(3) < 1029 units, so the
+// pre-test |dq|^2 <= kQPass keeps it.  A particle at |t| >= 2^22 on some axis
+// (or not finite) keeps every candidate: the exact test decides alone.
+// Offsets that wrap (candidates 64 h away in aliased buckets) can only pass
+// extra candidates to the exact test.
+constexpr float kQUnitsPerH = 1024.0f;
+constexpr int32_t kQPass = 1030 * 1030;
+
+__device__ __forceinline__ uint32_t qaxis(float v, float s) {
+    float t = v * s;
+    t = fminf(fmaxf(t, -1073741824.0f), 1073741824.0f);  // +-2^30; NaN -> -2^30
+    return (uint32_t)(int32_t)floorf(t);
+}
+
+__device__ __forceinline__ uint2 quantize(V3 x, float s) {
+    return make_uint2((qaxis(x.x, s) & 0xffffu) | (qaxis(x.y, s) << 16), qaxis(x.z, s) & 0xffffu);
+}
+
+// The pre-test bound of a particle at x (see above).
+__device__ __forceinline__ int32_t qpass_of(V3 x, float s) {
+    const float lim = 4194304.0f;  // 2^22
+    const bool in = fabsf(x.x * s) < lim && fabsf(x.y * s) < lim && fabsf(x.z * s) < lim;
+    return in ? kQPass : 0x7fffffff;
+}
+
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+
+// |q_i - q_j|^2 in units^2, saturating at 2^31 - 1.
+__device__ __forceinline__ int32_t qdist2(uint2 a, uint2 b) {
+    const s16x2 dxy = __builtin_bit_cast(s16x2, a.x) - __builtin_bit_cast(s16x2, b.x);
+    const s16x2 dz = __builtin_bit_cast(s16x2, a.y) - __builtin_bit_cast(s16x2, b.y);
+    const int32_t s = __builtin_amdgcn_sdot2(dxy, dxy, 0, true);
+    return __builtin_amdgcn_sdot2(dz, dz, s, true);
 }
 
 // Spiky gradient (CS_ComputeLambdas.glsl:83-92).  r2 >= 1e-10 here, inside

@@ -46,44 +46,45 @@ constexpr uint32_t kLdsCap = 24;
 constexpr uint32_t kFlushAt = 16;
 
 // Phase 1 of the two-phase gather, with the phase-2 work interleaved in
-// wave-uniform flushes.  The 27 buckets are scanned as in scan_buckets; the
-// slots of candidates inside the support (r^2 <= h^2) are appended, in visit
-// order, to this lane's column of an LDS list (col[e * kBlock]).  After every
-// bucket, if any lane holds kFlushAt entries or more, every lane hands its
-// entries to flush(first, count) and empties its column, so the expensive
-// arithmetic of phase 2 runs with the lanes of the wave together; a lane whose
-// column has no room for the next group of four candidates flushes alone.
-// Entries reach flush in visit order, numbered from `first`.  Returns the
-// lane's neighbour count.  X holds n positions (n < 2^28).
+// wave-uniform flushes.  The 27 buckets are scanned as in scan_buckets, and
+// every candidate is given the quantised pre-test (qdist2 <= qpass, see
+// pbf_device.h) against this particle's quantised position qi; the slots that
+// pass are appended, in visit order, to this lane's column of an LDS list
+// (col[e * kBlock]).  The particle's own slot passes (r = 0) and is left to
+// the flush, which gives each entry the exact test.  After every bucket, if
+// any lane holds kFlushAt entries or more, every lane hands its entries to
+// flush(count) and empties its column, so the expensive arithmetic runs with
+// the lanes of the wave together; a lane whose column has no room for the
+// next group of four candidates flushes alone.  Q holds the quantised
+// positions of n slots (n < 2^28).  With TAPS, ncand counts the candidates
+// other than the particle itself, as the reference visits them.
 template <bool TAPS, class Flush>
-__device__ __forceinline__ uint32_t collect_flush(V3 x, float h, const Domain& dm, const int2* __restrict__ cells,
-                                                  const float4* __restrict__ X, uint32_t n, uint32_t self,
-                                                  uint32_t* col, Flush&& flush, uint32_t& ncand) {
-    const float h2 = h * h;
+__device__ __forceinline__ void collect_flush(V3 x, float h, const Domain& dm, const int2* __restrict__ cells,
+                                              const uint2* __restrict__ Q, uint32_t n, uint32_t self, uint2 qi,
+                                              int32_t qpass, uint32_t* col, Flush&& flush, uint32_t& ncand) {
     int32_t cx, cy, cz;
     cell_of(x, h, cx, cy, cz);
     const BucketWalk bw(cx, cy, cz, dm);
-    // Candidate positions are read through a buffer resource over X: 32-bit
-    // offsets, and the loads of a group of four that run past the end of X
-    // return zeros (the hardware bounds check) where the indexes would
-    // otherwise be clamped.  Those candidates are not valid and not counted.
-    const __amdgpu_buffer_rsrc_t xr =
-        __builtin_amdgcn_make_buffer_rsrc((void*)X, (short)0, (int)(n * (uint32_t)sizeof(float4)), 0x00020000);
-    auto load = [&](uint32_t k) {
-        const auto v = __builtin_amdgcn_raw_buffer_load_b96(xr, (int)(k * (uint32_t)sizeof(float4)), 0, 0);
-        return mk(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]));
+    // Candidates are read through a buffer resource over Q: 32-bit offsets,
+    // and the loads of a group of four that run past the end of Q return
+    // zeros (the hardware bounds check) where the indexes would otherwise be
+    // clamped.  Those candidates are not valid and not kept.
+    const __amdgpu_buffer_rsrc_t qr =
+        __builtin_amdgcn_make_buffer_rsrc((void*)Q, (short)0, (int)(n * (uint32_t)sizeof(uint2)), 0x00020000);
+    auto load = [&](int off) {
+        const auto v = __builtin_amdgcn_raw_buffer_load_b64(qr, off, 0, 0);
+        return make_uint2(v[0], v[1]);
     };
     int2 se = cells[bw.bucket(0)];
-    uint32_t cnt = 0, total = 0;
-    // branch-free candidate test: the slot is always written to the column
-    // and the fill level advances only for a neighbour (a valid slot other
-    // than the particle's own, inside the support)
-    auto visit = [&](uint32_t k, V3 xk, bool valid) {
-        const V3 r = sub(x, xk);
-        const bool other = valid && k != self;
-        if (TAPS) ncand += other ? 1u : 0u;
+    uint32_t cnt = 0;
+    // branch-free: the slot is always written to the column and the fill
+    // level advances only for a valid slot that passes the pre-test
+    auto visit = [&](uint32_t k, uint2 qk, bool valid) {
+        if (TAPS) ncand += (valid && k != self) ? 1u : 0u;
         col[cnt * kBlock] = k;
-        cnt += (other && dot(r, r) <= h2) ? 1u : 0u;
+        // bitwise, so that the test is not made a branch (which would sink
+        // the group's loads into it)
+        cnt += (uint32_t)(qdist2(qi, qk) <= qpass) & (uint32_t)valid;
     };
 #pragma unroll 1
     for (int q = 0; q < 27; ++q) {
@@ -93,11 +94,12 @@ __device__ __forceinline__ uint32_t collect_flush(V3 x, float h, const Domain& d
         const uint32_t end = (uint32_t)se.y;
         for (uint32_t k = (uint32_t)se.x; k < end; k += 4) {
             if (cnt > kLdsCap - 4u) {
-                flush(total, cnt);
-                total += cnt;
+                flush(cnt);
                 cnt = 0;
             }
-            const V3 a0 = load(k), a1 = load(k + 1), a2 = load(k + 2), a3 = load(k + 3);
+            // one byte offset, the group's other three in the instructions' offset field
+            const int o = (int)(k * (uint32_t)sizeof(uint2));
+            const uint2 a0 = load(o), a1 = load(o + 8), a2 = load(o + 16), a3 = load(o + 24);
             visit(k, a0, true);
             visit(k + 1, a1, k + 1 < end);
             visit(k + 2, a2, k + 2 < end);
@@ -105,13 +107,11 @@ __device__ __forceinline__ uint32_t collect_flush(V3 x, float h, const Domain& d
         }
         se = nxt;
         if (__any(cnt >= kFlushAt)) {
-            flush(total, cnt);
-            total += cnt;
+            flush(cnt);
             cnt = 0;
         }
     }
-    flush(total, cnt);
-    return total + cnt;
+    flush(cnt);
 }
 
 // Run body over the slots idx(0..cnt) in order, fetching four neighbours
@@ -138,15 +138,17 @@ __device__ __forceinline__ void walk_list(uint32_t cnt, const float4* __restrict
 
 // Neighbour gather, bit-identical to the reference's loop in both forms:
 //   V = 0  single loop: body(j, xj, r, r2) for each candidate in the support.
-//   V = 1  two phases (collect_flush): the lanes of a wave execute the
-//          expensive arithmetic (square root, divisions) together on real
-//          neighbours instead of under divergence on the ~85% of candidates
-//          outside the support.
+//   V = 1  two phases (collect_flush): a quantised pre-test drops most of
+//          the ~85% of candidates outside the support, the flush gives the
+//          rest the exact test, and the lanes of a wave execute the expensive
+//          arithmetic (square root, divisions) together on real neighbours
+//          instead of under divergence.
 // Pairs with r^2 > h^2 are skipped: in the reference they add exact zeros.
 template <int V, bool TAPS, class Body>
 __device__ __forceinline__ void gather(V3 xi, float h, const Domain& dm, const int2* __restrict__ cells,
-                                       const float4* __restrict__ X, uint32_t n, uint32_t self, uint32_t* col,
-                                       Body&& body, uint32_t& ncand, uint32_t& nnbr) {
+                                       const float4* __restrict__ X, const uint2* __restrict__ Q, float qscale,
+                                       uint32_t n, uint32_t self, uint32_t* col, Body&& body, uint32_t& ncand,
+                                       uint32_t& nnbr) {
     if constexpr (V == 0) {
         const float h2 = h * h;
         scan_buckets(xi, h, dm, cells, X, self, [&](uint32_t k, float4 xk, V3 r, float r2) {
@@ -157,10 +159,16 @@ __device__ __forceinline__ void gather(V3 xi, float h, const Domain& dm, const i
             }
         });
     } else {
-        nnbr = collect_flush<TAPS>(xi, h, dm, cells, X, n, self, col, [&](uint32_t, uint32_t cnt) {
+        const float h2 = h * h;
+        collect_flush<TAPS>(xi, h, dm, cells, Q, n, self, quantize(xi, qscale), qpass_of(xi, qscale), col,
+                            [&](uint32_t cnt) {
             walk_list(cnt, X, [&](uint32_t e) { return col[e * kBlock]; }, [&](uint32_t j, float4 xj) {
                 const V3 r = sub(xi, xyz(xj));
-                body(j, xj, r, dot(r, r));
+                const float r2 = dot(r, r);
+                if (j != self && r2 <= h2) {
+                    if (TAPS) ++nnbr;
+                    body(j, xj, r, r2);
+                }
             });
         }, ncand);
     }
@@ -220,14 +228,15 @@ template <bool B>
 __global__ __launch_bounds__(kBlock) void k_reorder(StepConst c, const uint32_t* __restrict__ perm,
                                                     const float4* __restrict__ xstar,
                                                     const float4* __restrict__ vel, float4* __restrict__ P,
-                                                    float4* __restrict__ PL, float4* __restrict__ vs,
-                                                    uint32_t* __restrict__ bins, uint32_t* __restrict__ bin_of,
-                                                    uint32_t* __restrict__ bin_rank) {
+                                                    uint2* __restrict__ Q, float4* __restrict__ PL,
+                                                    float4* __restrict__ vs, uint32_t* __restrict__ bins,
+                                                    uint32_t* __restrict__ bin_of, uint32_t* __restrict__ bin_rank) {
     const uint32_t k = gid();
     if (k >= c.n) return;
     const uint32_t id = perm[k];
     const float4 x = xstar[id];
     P[k] = x;
+    Q[k] = quantize(xyz(x), c.qscale);
     PL[k] = x;
     vs[k] = vel[id];
     int32_t cx, cy, cz;
@@ -303,7 +312,8 @@ __device__ __forceinline__ size_t list_slot(uint32_t t, uint32_t r) {
 // iteration, which sees the same positions.
 template <int V, bool TAPS, bool B>
 __global__ __launch_bounds__(kBlock) void k_lambda(StepConst c, const uint32_t* __restrict__ order,
-                                                   const float4* __restrict__ P, float4* __restrict__ PL,
+                                                   const float4* __restrict__ P, const uint2* __restrict__ Q,
+                                                   float4* __restrict__ PL,
                                                    float* __restrict__ rho_out, const uint32_t* __restrict__ lod,
                                                    const int2* __restrict__ cells, uint32_t iter,
                                                    uint32_t* __restrict__ list_count, uint32_t* __restrict__ list,
@@ -330,17 +340,21 @@ __global__ __launch_bounds__(kBlock) void k_lambda(StepConst c, const uint32_t* 
         gi = add(gi, gw);
     };
     if constexpr (V == 0) {
-        gather<0, TAPS>(xi, c.h, domain_of<B>(c, k), cells, P, c.n, k, nullptr, body, ncand, nnbr);
+        gather<0, TAPS>(xi, c.h, domain_of<B>(c, k), cells, P, Q, c.qscale, c.n, k, nullptr, body, ncand, nnbr);
     } else {
         __shared__ uint32_t lst[kLdsCap * kBlock];
         uint32_t* col = &lst[threadIdx.x];
-        nnbr = collect_flush<TAPS>(xi, c.h, domain_of<B>(c, k), cells, P, c.n, k, col, [&](uint32_t first, uint32_t cnt) {
-#pragma unroll 1
-            for (uint32_t e = 0; e < cnt; ++e)
-                if (first + e < (uint32_t)kNbrCap) list[list_slot(t, first + e)] = col[e * kBlock];
+        const float h2 = c.h * c.h;
+        collect_flush<TAPS>(xi, c.h, domain_of<B>(c, k), cells, Q, c.n, k, quantize(xi, c.qscale),
+                            qpass_of(xi, c.qscale), col, [&](uint32_t cnt) {
             walk_list(cnt, P, [&](uint32_t e) { return col[e * kBlock]; }, [&](uint32_t j, float4 xj) {
                 const V3 d = sub(xi, xyz(xj));
-                body(j, xj, d, dot(d, d));
+                const float r2 = dot(d, d);
+                if (j != k && r2 <= h2) {
+                    if (nnbr < (uint32_t)kNbrCap) list[list_slot(t, nnbr)] = j;
+                    ++nnbr;
+                    body(j, xj, d, r2);
+                }
             });
         }, ncand);
         list_count[t] = nnbr;
@@ -367,8 +381,9 @@ __global__ __launch_bounds__(kBlock) void k_lambda(StepConst c, const uint32_t* 
 template <int V, bool TAPS, bool B>
 __global__ __launch_bounds__(kBlock) void k_deltap(StepConst c, const uint32_t* __restrict__ order,
                                                    const float4* __restrict__ PL, float4* __restrict__ P,
-                                                   const uint32_t* __restrict__ lod, const int2* __restrict__ cells,
-                                                   uint32_t iter, const uint32_t* __restrict__ list_count,
+                                                   uint2* __restrict__ Q, const uint32_t* __restrict__ lod,
+                                                   const int2* __restrict__ cells, uint32_t iter,
+                                                   const uint32_t* __restrict__ list_count,
                                                    const uint32_t* __restrict__ list, float4* __restrict__ dp_tap) {
     const uint32_t t = xcd_gid();
     if (t >= c.n) return;
@@ -431,6 +446,7 @@ __global__ __launch_bounds__(kBlock) void k_deltap(StepConst c, const uint32_t* 
     }
     const V3 p = add(xi, dp);
     P[k] = make_float4(p.x, p.y, p.z, 0.0f);
+    Q[k] = quantize(p, c.qscale);
 }
 
 // The SDF projection of CS_ApplyDeltaP.glsl:108-121 on the corrected
@@ -438,8 +454,8 @@ __global__ __launch_bounds__(kBlock) void k_deltap(StepConst c, const uint32_t* 
 // samples do not raise the register count of the delta-p gather).
 template <bool B>
 __global__ __launch_bounds__(kBlock) void k_apply_sdf(StepConst c, const uint32_t* __restrict__ order,
-                                                      float4* __restrict__ P, const uint32_t* __restrict__ lod,
-                                                      uint32_t iter) {
+                                                      float4* __restrict__ P, uint2* __restrict__ Q,
+                                                      const uint32_t* __restrict__ lod, uint32_t iter) {
     const uint32_t t = xcd_gid();
     if (t >= c.n) return;
     const uint32_t k = order[t];
@@ -448,7 +464,10 @@ __global__ __launch_bounds__(kBlock) void k_apply_sdf(StepConst c, const uint32_
     if (dom.enable_sdf != 1u) return;
     V3 p = xyz(P[k]);
     V3 nrm;
-    if (sdf_project(dom, p, nrm)) P[k] = make_float4(p.x, p.y, p.z, 0.0f);
+    if (sdf_project(dom, p, nrm)) {
+        P[k] = make_float4(p.x, p.y, p.z, 0.0f);
+        Q[k] = quantize(p, c.qscale);
+    }
 }
 
 // CS_Integrate.glsl:135-280 in two kernels.  The gather: XSPH sum and curl
@@ -457,7 +476,8 @@ __global__ __launch_bounds__(kBlock) void k_apply_sdf(StepConst c, const uint32_
 // at :171).  It writes the XSPH velocity change and the curl.
 template <int V, bool B>
 __global__ __launch_bounds__(kBlock) void k_integrate(StepConst c, const uint32_t* __restrict__ order,
-                                                      const float4* __restrict__ P, const float4* __restrict__ vs,
+                                                      const float4* __restrict__ P, const uint2* __restrict__ Q,
+                                                      const float4* __restrict__ vs,
                                                       const int2* __restrict__ cells, float4* __restrict__ vdelta,
                                                       float4* __restrict__ curl_out) {
     const uint32_t t = xcd_gid();
@@ -472,7 +492,8 @@ __global__ __launch_bounds__(kBlock) void k_integrate(StepConst c, const uint32_
     V3 curl = mk(0.0f, 0.0f, 0.0f);
     uint32_t ncand = 0, nnbr = 0;
     __shared__ uint32_t lst[kLdsCap * kBlock];
-    gather<V, false>(xp, c.h, domain_of<B>(c, k), cells, P, c.n, k, &lst[threadIdx.x], [&](uint32_t j, float4, V3 r, float r2) {
+    gather<V, false>(xp, c.h, domain_of<B>(c, k), cells, P, Q, c.qscale, c.n, k, &lst[threadIdx.x],
+                     [&](uint32_t j, float4, V3 r, float r2) {
         const V3 vij = sub(xyz(vs[j]), old_vel);
         if (do_visc) {
             const float w = poly6_r2(c, r2);
@@ -509,7 +530,7 @@ __global__ __launch_bounds__(kBlock) void k_integrate_finish(StepConst c, const 
                                                              const uint32_t* __restrict__ perm,
                                                              const float4* __restrict__ curl,
                                                              float4* __restrict__ pos, float4* __restrict__ vel,
-                                                             float4* __restrict__ posS) {
+                                                             float4* __restrict__ posS, uint2* __restrict__ Q) {
     const uint32_t t = xcd_gid();
     if (t >= c.n) return;
     const uint32_t k = order[t];
@@ -547,11 +568,13 @@ __global__ __launch_bounds__(kBlock) void k_integrate_finish(StepConst c, const 
     vel[id] = make_float4(v.x, v.y, v.z, vw);
     const float om = c.enable_vort == 1u ? len(xyz(curl[k])) : 0.0f;
     posS[k] = make_float4(xp.x, xp.y, xp.z, om);
+    Q[k] = quantize(xp, c.qscale);  // the vorticity gather's candidates are the committed positions
 }
 
 // CS_VorticityConfinement.glsl:133-167 (pass 1)
 template <int V, bool B>
-__global__ __launch_bounds__(kBlock) void k_vorticity(StepConst c, const uint32_t* __restrict__ order, const float4* __restrict__ posS,
+__global__ __launch_bounds__(kBlock) void k_vorticity(StepConst c, const uint32_t* __restrict__ order,
+                                                      const float4* __restrict__ posS, const uint2* __restrict__ Q,
                                                       const float4* __restrict__ curl,
                                                       const uint32_t* __restrict__ perm,
                                                       float4* __restrict__ vel, const int2* __restrict__ cells) {
@@ -563,7 +586,8 @@ __global__ __launch_bounds__(kBlock) void k_vorticity(StepConst c, const uint32_
     V3 eta = mk(0.0f, 0.0f, 0.0f);
     uint32_t ncand = 0, nnbr = 0;
     __shared__ uint32_t lst[kLdsCap * kBlock];
-    gather<V, false>(pi, c.h, domain_of<B>(c, k), cells, posS, c.n, k, &lst[threadIdx.x], [&](uint32_t, float4 xj, V3 r, float r2) {
+    gather<V, false>(pi, c.h, domain_of<B>(c, k), cells, posS, Q, c.qscale, c.n, k, &lst[threadIdx.x],
+                     [&](uint32_t, float4 xj, V3 r, float r2) {
         const V3 gw = spiky(c, r, r2);
         eta = add(eta, scl(gw, xj.w));  // xj.w = |omega_j|
     }, ncand, nnbr);
@@ -863,7 +887,7 @@ __global__ __launch_bounds__(kBlock) void k_lambda_deferred(StepConst c, const u
 // The clamp and add of CS_ApplyDeltaP.glsl:80-106 for one particle.
 template <bool TAPS>
 __device__ __forceinline__ void deltap_store(const StepConst& c, uint32_t k, V3 xi, V3 dp, float4* __restrict__ P,
-                                             float4* __restrict__ dp_tap) {
+                                             uint2* __restrict__ Q, float4* __restrict__ dp_tap) {
     dp = scl(dp, c.inv_rho0);
     if (TAPS) dp_tap[k] = make_float4(dp.x, dp.y, dp.z, 0.0f);
     const float maxdp = 0.5f * c.h;
@@ -875,6 +899,7 @@ __device__ __forceinline__ void deltap_store(const StepConst& c, uint32_t k, V3 
     }
     const V3 p = add(xi, dp);
     P[k] = make_float4(p.x, p.y, p.z, 0.0f);
+    Q[k] = quantize(p, c.qscale);
 }
 
 // CS_ComputeDeltaP.glsl:120-166 + the clamp and add of CS_ApplyDeltaP.glsl:80-106,
@@ -885,7 +910,7 @@ __global__ __launch_bounds__(kBlock) void k_deltap_tile(StepConst c, const uint3
                                                         const uint32_t* __restrict__ order,
                                                         const uint32_t* __restrict__ bin_of,
                                                         const float4* __restrict__ PL, float4* __restrict__ P,
-                                                        const uint32_t* __restrict__ lod,
+                                                        uint2* __restrict__ Q, const uint32_t* __restrict__ lod,
                                                         const int2* __restrict__ cells, uint32_t iter,
                                                         float4* __restrict__ dp_tap, uint32_t* __restrict__ defer,
                                                         uint32_t* __restrict__ ndefer) {
@@ -923,7 +948,7 @@ __global__ __launch_bounds__(kBlock) void k_deltap_tile(StepConst c, const uint3
                     dp = add(dp, scl(gw, li + a.w + sc));
                 }
             }, ncand);
-            deltap_store<TAPS>(c, k, xi, dp, P, dp_tap);
+            deltap_store<TAPS>(c, k, xi, dp, P, Q, dp_tap);
         }
         __syncthreads();
     }
@@ -933,7 +958,7 @@ template <bool TAPS, bool B>
 __global__ __launch_bounds__(kBlock) void k_deltap_deferred(StepConst c, const uint32_t* __restrict__ defer,
                                                             const uint32_t* __restrict__ ndefer,
                                                             const float4* __restrict__ PL, float4* __restrict__ P,
-                                                            const int2* __restrict__ cells,
+                                                            uint2* __restrict__ Q, const int2* __restrict__ cells,
                                                             float4* __restrict__ dp_tap) {
     const uint32_t nd = *ndefer;
     const float h2 = c.h * c.h;
@@ -950,7 +975,7 @@ __global__ __launch_bounds__(kBlock) void k_deltap_deferred(StepConst c, const u
                 dp = add(dp, scl(gw, li + xj.w + sc));
             }
         });
-        deltap_store<TAPS>(c, k, xi, dp, P, dp_tap);
+        deltap_store<TAPS>(c, k, xi, dp, P, Q, dp_tap);
     }
 }
 
@@ -1006,8 +1031,8 @@ static inline bool tiled_form(const DeviceState& d) { return d.variant == 2 && d
 hipError_t launch_reorder(const StepConst& c, DeviceState& d, hipStream_t st) {
     hipLaunchKernelGGL(k_fill_u32, grid_for(kOrderBins), dim3(kBlock), 0, st, d.bins, kOrderBins, 0u);
 #define L(B)                                                                                                    \
-    hipLaunchKernelGGL((k_reorder<B>), grid_for(c.n), dim3(kBlock), 0, st, c, d.perm, d.xstar, d.vel, d.P, d.PL, \
-                       d.vs, d.bins, d.bin_of, d.bin_rank);
+    hipLaunchKernelGGL((k_reorder<B>), grid_for(c.n), dim3(kBlock), 0, st, c, d.perm, d.xstar, d.vel, d.P, d.Q, \
+                       d.PL, d.vs, d.bins, d.bin_of, d.bin_rank);
     FSI_B_SWITCH(d.batched, L)
 #undef L
     hipError_t e = scan_exclusive_u32(d.bins, d.bins, kOrderBins, d.sort.partial, st);
@@ -1056,10 +1081,10 @@ hipError_t launch_lambda(const StepConst& c, DeviceState& d, uint32_t iter, bool
     }
 #define L(V, B)                                                                                                 \
     if (taps)                                                                                                   \
-        hipLaunchKernelGGL((k_lambda<V, true, B>), grid_for(c.n), dim3(kBlock), 0, st, c, d.order, d.P, d.PL,   \
+        hipLaunchKernelGGL((k_lambda<V, true, B>), grid_for(c.n), dim3(kBlock), 0, st, c, d.order, d.P, d.Q, d.PL, \
                            d.rho, d.lod, d.cells, iter, d.list_count, d.list, d.cand, d.nbr);                   \
     else                                                                                                        \
-        hipLaunchKernelGGL((k_lambda<V, false, B>), grid_for(c.n), dim3(kBlock), 0, st, c, d.order, d.P, d.PL,  \
+        hipLaunchKernelGGL((k_lambda<V, false, B>), grid_for(c.n), dim3(kBlock), 0, st, c, d.order, d.P, d.Q, d.PL, \
                            d.rho, d.lod, d.cells, iter, d.list_count, d.list, d.cand, d.nbr);
     FSI_GATHER_SWITCH(L)
 #undef L
@@ -1071,9 +1096,9 @@ hipError_t launch_deltap(const StepConst& c, DeviceState& d, uint32_t iter, bool
         hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(kBlock), 0, st, d.ndefer, 1u, 0u);
 #define L(T, B)                                                                                                   \
     hipLaunchKernelGGL((k_deltap_tile<T, B>), dim3(kTileGrid), dim3(kBlock), 0, st, c, d.tiles, d.ntiles, d.order,   \
-                       d.bin_of, d.PL, d.P, d.lod, d.cells, iter, d.dp_tap, d.defer, d.ndefer);                  \
+                       d.bin_of, d.PL, d.P, d.Q, d.lod, d.cells, iter, d.dp_tap, d.defer, d.ndefer);             \
     hipLaunchKernelGGL((k_deltap_deferred<T, B>), dim3(kDeferGrid), dim3(kBlock), 0, st, c, d.defer, d.ndefer, d.PL, \
-                       d.P, d.cells, d.dp_tap);
+                       d.P, d.Q, d.cells, d.dp_tap);
         if (taps) {
             FSI_B_SWITCH(d.batched, L_T)
         } else {
@@ -1084,15 +1109,15 @@ hipError_t launch_deltap(const StepConst& c, DeviceState& d, uint32_t iter, bool
 #define L(V, B)                                                                                                 \
     if (taps)                                                                                                   \
         hipLaunchKernelGGL((k_deltap<V, true, B>), grid_for(c.n), dim3(kBlock), 0, st, c, d.order, d.PL, d.P,   \
-                           d.lod, d.cells, iter, d.list_count, d.list, d.dp_tap);                               \
+                           d.Q, d.lod, d.cells, iter, d.list_count, d.list, d.dp_tap);                          \
     else                                                                                                        \
-        hipLaunchKernelGGL((k_deltap<V, false, B>), grid_for(c.n), dim3(kBlock), 0, st, c, d.order, d.PL, d.P,   \
-                           d.lod, d.cells, iter, d.list_count, d.list, d.dp_tap);
+        hipLaunchKernelGGL((k_deltap<V, false, B>), grid_for(c.n), dim3(kBlock), 0, st, c, d.order, d.PL, d.P,  \
+                           d.Q, d.lod, d.cells, iter, d.list_count, d.list, d.dp_tap);
         FSI_GATHER_SWITCH(L)
 #undef L
     }
     if (d.any_sdf) {
-#define L(B) hipLaunchKernelGGL((k_apply_sdf<B>), grid_for(c.n), dim3(kBlock), 0, st, c, d.order, d.P, d.lod, iter);
+#define L(B) hipLaunchKernelGGL((k_apply_sdf<B>), grid_for(c.n), dim3(kBlock), 0, st, c, d.order, d.P, d.Q, d.lod, iter);
         FSI_B_SWITCH(d.batched, L)
 #undef L
     }
@@ -1102,14 +1127,14 @@ hipError_t launch_deltap(const StepConst& c, DeviceState& d, uint32_t iter, bool
 hipError_t launch_integrate(const StepConst& c, DeviceState& d, hipStream_t st) {
     if (c.enable_visc == 1u || c.enable_vort == 1u) {
 #define L(V, B)                                                                                                     \
-    hipLaunchKernelGGL((k_integrate<V, B>), grid_for(c.n), dim3(kBlock), 0, st, c, d.order, d.P, d.vs, d.cells, \
+    hipLaunchKernelGGL((k_integrate<V, B>), grid_for(c.n), dim3(kBlock), 0, st, c, d.order, d.P, d.Q, d.vs, d.cells, \
                        d.vdelta, d.curl);
         FSI_GATHER_SWITCH(L)
 #undef L
     }
 #define L(B)                                                                                                   \
     hipLaunchKernelGGL((k_integrate_finish<B>), grid_for(c.n), dim3(kBlock), 0, st, c, d.order, d.P, d.vs, d.vdelta, \
-                       d.perm, d.curl, d.pos, d.vel, d.posS);
+                       d.perm, d.curl, d.pos, d.vel, d.posS, d.Q);
     FSI_B_SWITCH(d.batched, L)
 #undef L
     return hipGetLastError();
@@ -1117,7 +1142,7 @@ hipError_t launch_integrate(const StepConst& c, DeviceState& d, hipStream_t st) 
 
 hipError_t launch_vorticity(const StepConst& c, DeviceState& d, hipStream_t st) {
 #define L(V, B)                                                                                                   \
-    hipLaunchKernelGGL((k_vorticity<V, B>), grid_for(c.n), dim3(kBlock), 0, st, c, d.order, d.posS, d.curl, d.perm, \
+    hipLaunchKernelGGL((k_vorticity<V, B>), grid_for(c.n), dim3(kBlock), 0, st, c, d.order, d.posS, d.Q, d.curl, d.perm, \
                        d.vel, d.cells);
     FSI_GATHER_SWITCH(L)
 #undef L

@@ -50,6 +50,7 @@ struct DeviceState {
     uint32_t* perm = nullptr;  // sorted slot -> particle id (points into sort scratch)
     int2* cells = nullptr;     // hash_size (start, end)
     float4* P = nullptr;       // sorted: x* being corrected
+    uint2* Q = nullptr;        // sorted: quantised positions the next gather's candidate pre-test reads
     float4* PL = nullptr;      // sorted: x* and lambda
     float4* vs = nullptr;      // sorted: post-predict velocity (v*, pad2)
     float4* curl = nullptr;    // sorted: curl / rho0 of the last Integrate
