@@ -45,109 +45,211 @@ inline dim3 grid_for(uint32_t n) { return dim3((n + kBlock - 1) / kBlock); }
 constexpr uint32_t kLdsCap = 24;
 constexpr uint32_t kFlushAt = 16;
 
-// Phase 1 of the two-phase gather, with the phase-2 work interleaved in
-// wave-uniform flushes.  The 27 buckets are scanned as in scan_buckets, and
-// every candidate is given the quantised pre-test (qdist2 <= qpass, see
-// pbf_device.h) against this particle's quantised position qi; the slots that
-// pass are appended, in visit order, to this lane's column of an LDS list
-// (col[e * kBlock]).  The particle's own slot passes (r = 0) and is left to
-// the flush, which gives each entry the exact test.  After every bucket, if
-// any lane holds kFlushAt entries or more, every lane hands its entries to
-// flush(count) and empties its column, so the expensive arithmetic runs with
-// the lanes of the wave together; a lane whose column has no room for the
-// next group of four candidates flushes alone.  Q holds the quantised
-// positions of n slots (n < 2^28).  With TAPS, ncand counts the candidates
-// other than the particle itself, as the reference visits them.
-template <bool TAPS, class Flush>
-__device__ __forceinline__ void collect_flush(V3 x, float h, const Domain& dm, const int2* __restrict__ cells,
-                                              const uint2* __restrict__ Q, uint32_t n, uint32_t self, uint2 qi,
-                                              int32_t qpass, uint32_t* col, Flush&& flush, uint32_t& ncand) {
-    int32_t cx, cy, cz;
-    cell_of(x, h, cx, cy, cz);
-    const BucketWalk bw(cx, cy, cz, dm);
-    // Candidates are read through a buffer resource over Q: 32-bit offsets,
-    // and the loads of a group of four that run past the end of Q return
-    // zeros (the hardware bounds check) where the indexes would otherwise be
-    // clamped.  Those candidates are not valid and not kept.
-    const __amdgpu_buffer_rsrc_t qr =
-        __builtin_amdgcn_make_buffer_rsrc((void*)Q, (short)0, (int)(n * (uint32_t)sizeof(uint2)), 0x00020000);
-    auto load = [&](int off) {
-        const auto v = __builtin_amdgcn_raw_buffer_load_b64(qr, off, 0, 0);
-        return make_uint2(v[0], v[1]);
-    };
-    int2 se = cells[bw.bucket(0)];
-    uint32_t cnt = 0;
-    // branch-free: the slot is always written to the column and the fill
-    // level advances only for a valid slot that passes the pre-test
-    auto visit = [&](uint32_t k, uint2 qk, bool valid) {
-        if (TAPS) ncand += (valid && k != self) ? 1u : 0u;
-        col[cnt * kBlock] = k;
-        // bitwise, so that the test is not made a branch (which would sink
-        // the group's loads into it)
-        cnt += (uint32_t)(qdist2(qi, qk) <= qpass) & (uint32_t)valid;
-    };
-#pragma unroll 1
-    for (int q = 0; q < 27; ++q) {
-        int2 nxt = make_int2(-1, -1);
-        if (q < 26) nxt = cells[bw.bucket(q + 1)];
-        // an empty bucket is (-1, -1): no iterations
-        const uint32_t end = (uint32_t)se.y;
-        for (uint32_t k = (uint32_t)se.x; k < end; k += 4) {
-            if (cnt > kLdsCap - 4u) {
-                flush(cnt);
-                cnt = 0;
-            }
-            // one byte offset, the group's other three in the instructions' offset field
-            const int o = (int)(k * (uint32_t)sizeof(uint2));
-            const uint2 a0 = load(o), a1 = load(o + 8), a2 = load(o + 16), a3 = load(o + 24);
-            visit(k, a0, true);
-            visit(k + 1, a1, k + 1 < end);
-            visit(k + 2, a2, k + 2 < end);
-            visit(k + 3, a3, k + 3 < end);
-        }
-        se = nxt;
-        if (__any(cnt >= kFlushAt)) {
-            flush(cnt);
-            cnt = 0;
-        }
-    }
-    flush(cnt);
+// The neighbour list: kNbrCap slots per thread.  The 64 threads of a wave
+// own one block of 64 x kNbrCap slots, in which entries r..r+3 (r a multiple
+// of four) of all 64 threads form one contiguous 1 KiB run, four slots per
+// thread: entry r of thread t is slot
+//   (t >> 6) * 64 * kNbrCap + (r >> 2) * 256 + (t & 63) * 4 + (r & 3).
+// The walks read four entries per thread with one 16-byte load and the list
+// builder writes them with one 16-byte store, so each wave access covers
+// whole cache lines once.  (One row per thread instead made every wave load
+// touch 64 lines, re-fetched four times each as the row was walked, and the
+// lines of the waves in flight did not fit in the L2.)
+// Thread t's entries r..r+3 (r a multiple of four) as one uint4: wave_run(...)[(r >> 2) * 64].
+// The wave's block base is wave-uniform (read from the first active lane).
+__device__ __forceinline__ uint4* wave_run(uint32_t* list, uint32_t t) {
+    return reinterpret_cast<uint4*>(list + (size_t)__builtin_amdgcn_readfirstlane(t >> 6) *
+                                               (64u * (uint32_t)kNbrCap)) + (t & 63u);
+}
+__device__ __forceinline__ const uint4* wave_run(const uint32_t* list, uint32_t t) {
+    return reinterpret_cast<const uint4*>(list + (size_t)__builtin_amdgcn_readfirstlane(t >> 6) *
+                                                     (64u * (uint32_t)kNbrCap)) + (t & 63u);
 }
 
-// Run body over the slots idx(0..cnt) in order, fetching four neighbours
-// ahead: the index loads and then the position loads of a group of four are
-// issued together before the four bodies run.
-template <class Idx, class Body>
-__device__ __forceinline__ void walk_list(uint32_t cnt, const float4* __restrict__ X, Idx&& idx, Body&& body) {
-    uint32_t r = 0;
+// Per-lane staging ring of the list builder (entries; a power of two and a
+// multiple of four).
+constexpr uint32_t kLRing = 16;
+
+// The neighbour-list builder of the gathers (the first of their two
+// kernels).  Thread t takes the particle in slot order[t], whose position is
+// X: it scans the 27 buckets around it as scan_buckets does and gives every
+// candidate the quantised pre-test (qdist2 <= qpass, see pbf_device.h); the
+// slots that pass are its list, in visit order (the particle's own slot
+// passes, r = 0, and so may a candidate just outside the support: the walks
+// give every entry the exact test).  Slots are staged in a per-lane LDS ring
+// and written to HBM four at a time (the layout above), so every store covers whole
+// lines.  list_count[t] is the number of entries, also when it exceeds
+// kNbrCap (then only the first kNbrCap are kept and the walks scan the
+// buckets instead).  With lod, an APBF-inactive particle builds no list.
+// With TAPS, cand_out[k] counts the candidates other than the particle
+// itself, as the reference visits them.  X and Q hold n slots (n < 2^27).
+template <bool TAPS, bool B>
+__global__ __launch_bounds__(kBlock) void k_build_list(StepConst c, const uint32_t* __restrict__ order,
+                                                       const float4* __restrict__ X, const uint2* __restrict__ Q,
+                                                       const uint32_t* __restrict__ lod, uint32_t iter,
+                                                       const int2* __restrict__ cells,
+                                                       uint32_t* __restrict__ list_count, uint32_t* __restrict__ list,
+                                                       uint32_t* __restrict__ cand_out) {
+    const uint32_t t = xcd_gid();
+    if (t >= c.n) return;
+    const uint32_t k = order[t];
+    if (lod != nullptr && c.enable_apbf != 0u && lod[k] < iter) return;
+    const V3 x = xyz(X[k]);
+    const uint2 qi = quantize(x, c.qscale);
+    const int32_t qpass = qpass_of(x, c.qscale);
+    __shared__ uint32_t stage[kLRing * kBlock];
+    uint32_t* ring = &stage[threadIdx.x];
+    uint4* run = wave_run(list, t);
+    int32_t cx, cy, cz;
+    cell_of(x, c.h, cx, cy, cz);
+    const BucketWalk bw(cx, cy, cz, domain_of<B>(c, k));
+    // Candidates are read through a buffer resource over Q: 32-bit offsets,
+    // and the loads of a group of four that run past the end of Q return
+    // zeros (the hardware bounds check); such candidates are not valid.
+    const __amdgpu_buffer_rsrc_t qr =
+        __builtin_amdgcn_make_buffer_rsrc((void*)Q, (short)0, (int)(c.n * (uint32_t)sizeof(uint2)), 0x00020000);
+    // the records of slots s..s+3 (two 16-byte loads); an offset past the end
+    // of Q (slot -1 of an empty bucket, or the no-group offset) reads zeros
+    struct G4 {
+        uint4 a, b;
+    };
+    auto loadg = [&](uint32_t off) {
+        const auto u = __builtin_amdgcn_raw_buffer_load_b128(qr, (int)off, 0, 0);
+        const auto v = __builtin_amdgcn_raw_buffer_load_b128(qr, (int)off + 16, 0, 0);
+        return G4{make_uint4(u[0], u[1], u[2], u[3]), make_uint4(v[0], v[1], v[2], v[3])};
+    };
+    constexpr uint32_t kNoGroup = 0xffffffe0u;
+    uint32_t tail = 0, done = 0, ncand = 0;  // entries appended / written out (a multiple of four)
+    // writes entries done..done+3 (the last group may hold stale slots past
+    // the count: they are never read as entries)
+    auto copy4 = [&]() {
+        const uint32_t* r = ring + (done & (kLRing - 1u)) * kBlock;
+        const uint4 v = make_uint4(r[0], r[kBlock], r[2 * kBlock], r[3 * kBlock]);
+        if (done < (uint32_t)kNbrCap) run[(done >> 2) * 64u] = v;
+        done += 4;
+    };
+    // branch-free append: the slot is always written and the tail advances
+    // only for a valid slot that passes the pre-test
+    auto visit = [&](uint32_t s, uint2 qs, bool valid) {
+        if (TAPS) ncand += (valid && s != k) ? 1u : 0u;
+        ring[(tail & (kLRing - 1u)) * kBlock] = s;
+        // bitwise, so that the test is not made a branch (which would sink
+        // the group's loads into it)
+        tail += (uint32_t)(qdist2(qi, qs) <= qpass) & (uint32_t)valid;
+    };
+    // slots s..s+3 of a bucket that has rem (may be <= 0) slots from s on
+    auto visit4 = [&](uint32_t s, int32_t rem, const G4& g) {
+        if (tail - done > kLRing - 4u) copy4();  // room for four more
+        visit(s, make_uint2(g.a.x, g.a.y), 0 < rem);
+        visit(s + 1, make_uint2(g.a.z, g.a.w), 1 < rem);
+        visit(s + 2, make_uint2(g.b.x, g.b.y), 2 < rem);
+        visit(s + 3, make_uint2(g.b.z, g.b.w), 3 < rem);
+    };
+    // The loads of a bucket's first three groups of four (12 slots, most
+    // buckets) are issued together, so a bucket costs one load latency, not
+    // one per group; a group no lane of the wave needs is not tested.  The
+    // cell-table entry of a bucket is read two buckets ahead.  An empty bucket
+    // is (-1, -1): length 0.
+    int2 se = cells[bw.bucket(0)];
+    int2 nx = cells[bw.bucket(1)];
 #pragma unroll 1
-    for (; r + 4 <= cnt; r += 4) {
-        const uint32_t j0 = idx(r), j1 = idx(r + 1), j2 = idx(r + 2), j3 = idx(r + 3);
-        const float4 a0 = X[j0], a1 = X[j1], a2 = X[j2], a3 = X[j3];
-        body(j0, a0);
-        body(j1, a1);
-        body(j2, a2);
-        body(j3, a3);
+    for (int q = 0; q < 27; ++q) {
+        int2 nn = make_int2(-1, -1);
+        if (q < 25) nn = cells[bw.bucket(q + 2)];
+        const uint32_t s0 = (uint32_t)se.x;
+        const int32_t len = se.y - se.x;  // (-1, -1): 0
+        const uint32_t o = s0 * (uint32_t)sizeof(uint2);
+        const G4 g0 = loadg(0 < len ? o : kNoGroup);
+        const G4 g1 = loadg(4 < len ? o + 32u : kNoGroup);
+        const G4 g2 = loadg(8 < len ? o + 64u : kNoGroup);
+        if (__any(0 < len)) visit4(s0, len, g0);
+        if (__any(4 < len)) visit4(s0 + 4u, len - 4, g1);
+        if (__any(8 < len)) visit4(s0 + 8u, len - 8, g2);
+#pragma unroll 1
+        for (int32_t r = 12; r < len; r += 4) visit4(s0 + (uint32_t)r, len - r, loadg(o + (uint32_t)r * 8u));
+        se = nx;
+        nx = nn;
     }
 #pragma unroll 1
-    for (; r < cnt; ++r) {
-        const uint32_t j = idx(r);
-        body(j, X[j]);
+    while (tail > done) copy4();
+    list_count[t] = tail;
+    if (TAPS) cand_out[k] = ncand;
+}
+
+// The walk of a gather (the second kernel): hands thread t's neighbours to
+// body(j, xj, r, r2, vj) in visit order, with xj = X[j], r = xi - xj and, when
+// V is given, vj = V[j] (loaded with xj, ahead of the test).  Every list
+// entry gets the exact test (j != self, r^2 <= h^2; the reference adds exact
+// zeros for the pairs it rejects).  A list that overflowed kNbrCap is not
+// walked: the 27 buckets are scanned again.  Four entries per 16-byte load,
+// and their records through a buffer resource over X (n slots): entries past
+// the count are never used, and a stale one past n reads zeros.
+template <bool TAPS, class Body>
+__device__ __forceinline__ void gather_list(V3 xi, float h, const Domain& dm, const int2* __restrict__ cells,
+                                            const float4* __restrict__ X, const float4* __restrict__ V, uint32_t n,
+                                            uint32_t self, uint32_t t, const uint32_t* __restrict__ list_count,
+                                            const uint32_t* __restrict__ list, Body&& body, uint32_t& nnbr) {
+    const float h2 = h * h;
+    const uint32_t cnt = list_count[t];
+    if (cnt <= (uint32_t)kNbrCap) {
+        const uint4* run = wave_run(list, t);
+        const __amdgpu_buffer_rsrc_t xr =
+            __builtin_amdgcn_make_buffer_rsrc((void*)X, (short)0, (int)(n * (uint32_t)sizeof(float4)), 0x00020000);
+        const __amdgpu_buffer_rsrc_t vr =
+            __builtin_amdgcn_make_buffer_rsrc((void*)V, (short)0, (int)(n * (uint32_t)sizeof(float4)), 0x00020000);
+        auto load = [&](const __amdgpu_buffer_rsrc_t& rs, uint32_t j) {
+            const auto v = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)(j * (uint32_t)sizeof(float4)), 0, 0);
+            return make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]),
+                               __uint_as_float(v[3]));
+        };
+        const bool withv = V != nullptr;
+        auto exact = [&](uint32_t j, float4 xj, float4 vj) {
+            const V3 r = sub(xi, xyz(xj));
+            const float r2 = dot(r, r);
+            if (j != self && r2 <= h2) {
+                if (TAPS) ++nnbr;
+                body(j, xj, r, r2, vj);
+            }
+        };
+#pragma unroll 1
+        for (uint32_t e = 0; e < cnt; e += 4) {
+            const uint4 j = run[(e >> 2) * 64u];
+            const float4 a0 = load(xr, j.x), a1 = load(xr, j.y), a2 = load(xr, j.z), a3 = load(xr, j.w);
+            float4 v0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), v1 = v0, v2 = v0, v3 = v0;
+            if (withv) {
+                v0 = load(vr, j.x);
+                v1 = load(vr, j.y);
+                v2 = load(vr, j.z);
+                v3 = load(vr, j.w);
+            }
+            exact(j.x, a0, v0);
+            if (e + 1 < cnt) exact(j.y, a1, v1);
+            if (e + 2 < cnt) exact(j.z, a2, v2);
+            if (e + 3 < cnt) exact(j.w, a3, v3);
+        }
+    } else {
+        scan_buckets(xi, h, dm, cells, X, self, [&](uint32_t j, float4 xj, V3 r, float r2) {
+            if (r2 <= h2) {
+                if (TAPS) ++nnbr;
+                body(j, xj, r, r2, V != nullptr ? V[j] : make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+            }
+        });
     }
 }
 
 // Neighbour gather, bit-identical to the reference's loop in both forms:
 //   V = 0  single loop: body(j, xj, r, r2) for each candidate in the support.
-//   V = 1  two phases (collect_flush): a quantised pre-test drops most of
-//          the ~85% of candidates outside the support, the flush gives the
-//          rest the exact test, and the lanes of a wave execute the expensive
-//          arithmetic (square root, divisions) together on real neighbours
-//          instead of under divergence.
+//   V = 1  list walk (gather_list) over the list k_build_list wrote for this
+//          pass: the quantised pre-test of the builder drops most of the ~85%
+//          of candidates outside the support, and the lanes of a wave run the
+//          expensive arithmetic (square root, divisions) together on the
+//          entries instead of under divergence.
 // Pairs with r^2 > h^2 are skipped: in the reference they add exact zeros.
 template <int V, bool TAPS, class Body>
 __device__ __forceinline__ void gather(V3 xi, float h, const Domain& dm, const int2* __restrict__ cells,
-                                       const float4* __restrict__ X, const uint2* __restrict__ Q, float qscale,
-                                       uint32_t n, uint32_t self, uint32_t* col, Body&& body, uint32_t& ncand,
+                                       const float4* __restrict__ X, const float4* __restrict__ Vel, uint32_t n,
+                                       uint32_t self, uint32_t t, const uint32_t* __restrict__ list_count,
+                                       const uint32_t* __restrict__ list, Body&& body, uint32_t& ncand,
                                        uint32_t& nnbr) {
     if constexpr (V == 0) {
         const float h2 = h * h;
@@ -155,22 +257,11 @@ __device__ __forceinline__ void gather(V3 xi, float h, const Domain& dm, const i
             if (TAPS) ++ncand;
             if (r2 <= h2) {
                 if (TAPS) ++nnbr;
-                body(k, xk, r, r2);
+                body(k, xk, r, r2, Vel != nullptr ? Vel[k] : make_float4(0.0f, 0.0f, 0.0f, 0.0f));
             }
         });
     } else {
-        const float h2 = h * h;
-        collect_flush<TAPS>(xi, h, dm, cells, Q, n, self, quantize(xi, qscale), qpass_of(xi, qscale), col,
-                            [&](uint32_t cnt) {
-            walk_list(cnt, X, [&](uint32_t e) { return col[e * kBlock]; }, [&](uint32_t j, float4 xj) {
-                const V3 r = sub(xi, xyz(xj));
-                const float r2 = dot(r, r);
-                if (j != self && r2 <= h2) {
-                    if (TAPS) ++nnbr;
-                    body(j, xj, r, r2);
-                }
-            });
-        }, ncand);
+        gather_list<TAPS>(xi, h, dm, cells, X, Vel, n, self, t, list_count, list, body, nnbr);
     }
 }
 
@@ -295,29 +386,17 @@ __global__ __launch_bounds__(kBlock) void k_lod(StepConst c, const float4* __res
     lod[k] = l;
 }
 
-// Neighbour-list slot of entry r (< kNbrCap) of thread t.  The 64 threads of
-// a wave own one block of 64 x kNbrCap slots, in which entries r..r+3 (r a
-// multiple of four) of all 64 threads form one contiguous 1 KiB run, four
-// slots per thread.  The delta-p pass reads four entries per thread with one
-// 16-byte load, so each of its wave loads covers whole cache lines once.  (One
-// row per thread instead made every wave load touch 64 lines, re-fetched four
-// times each as the row was walked, and the lines of the waves in flight did
-// not fit in the L2.)
-__device__ __forceinline__ size_t list_slot(uint32_t t, uint32_t r) {
-    return (size_t)(t >> 6) * (64u * (uint32_t)kNbrCap) + (r >> 2) * 256u + (t & 63u) * 4u + (r & 3u);
-}
-
-// CS_ComputeLambdas.glsl:108-162.  In the two-phase form the neighbour list
-// is also written to HBM (list_slot), for the delta-p pass of the same
-// iteration, which sees the same positions.
+// CS_ComputeLambdas.glsl:108-162.  In the list form (V = 1) the neighbours
+// come from the list k_build_list wrote for these positions; the delta-p pass
+// of the same iteration walks it again.
 template <int V, bool TAPS, bool B>
 __global__ __launch_bounds__(kBlock) void k_lambda(StepConst c, const uint32_t* __restrict__ order,
-                                                   const float4* __restrict__ P, const uint2* __restrict__ Q,
-                                                   float4* __restrict__ PL,
+                                                   const float4* __restrict__ P, float4* __restrict__ PL,
                                                    float* __restrict__ rho_out, const uint32_t* __restrict__ lod,
                                                    const int2* __restrict__ cells, uint32_t iter,
-                                                   uint32_t* __restrict__ list_count, uint32_t* __restrict__ list,
-                                                   uint32_t* __restrict__ cand_out, uint32_t* __restrict__ nbr_out) {
+                                                   const uint32_t* __restrict__ list_count,
+                                                   const uint32_t* __restrict__ list, uint32_t* __restrict__ cand_out,
+                                                   uint32_t* __restrict__ nbr_out) {
     const uint32_t t = xcd_gid();
     if (t >= c.n) return;
     const uint32_t k = order[t];
@@ -332,33 +411,15 @@ __global__ __launch_bounds__(kBlock) void k_lambda(StepConst c, const uint32_t* 
     float sum_g2 = 0.0f;
     V3 gi = mk(0.0f, 0.0f, 0.0f);
     uint32_t ncand = 0, nnbr = 0;
-    auto body = [&](uint32_t, float4, V3 r, float r2) {
+    auto body = [&](uint32_t, float4, V3 r, float r2, float4) {
         rho = rho + poly6_r2(c, r2);
         const V3 gw = spiky(c, r, r2);
         const float gj2 = dot(gw, gw);
         sum_g2 = sum_g2 + gj2 * (inv * inv);
         gi = add(gi, gw);
     };
-    if constexpr (V == 0) {
-        gather<0, TAPS>(xi, c.h, domain_of<B>(c, k), cells, P, Q, c.qscale, c.n, k, nullptr, body, ncand, nnbr);
-    } else {
-        __shared__ uint32_t lst[kLdsCap * kBlock];
-        uint32_t* col = &lst[threadIdx.x];
-        const float h2 = c.h * c.h;
-        collect_flush<TAPS>(xi, c.h, domain_of<B>(c, k), cells, Q, c.n, k, quantize(xi, c.qscale),
-                            qpass_of(xi, c.qscale), col, [&](uint32_t cnt) {
-            walk_list(cnt, P, [&](uint32_t e) { return col[e * kBlock]; }, [&](uint32_t j, float4 xj) {
-                const V3 d = sub(xi, xyz(xj));
-                const float r2 = dot(d, d);
-                if (j != k && r2 <= h2) {
-                    if (nnbr < (uint32_t)kNbrCap) list[list_slot(t, nnbr)] = j;
-                    ++nnbr;
-                    body(j, xj, d, r2);
-                }
-            });
-        }, ncand);
-        list_count[t] = nnbr;
-    }
+    gather<V, TAPS>(xi, c.h, domain_of<B>(c, k), cells, P, nullptr, c.n, k, t, list_count, list, body, ncand,
+                    nnbr);
     gi = scl(gi, inv);
     float C = rho / c.rho0 - 1.0f;
     const float floorc = -c.cohesion;
@@ -368,16 +429,16 @@ __global__ __launch_bounds__(kBlock) void k_lambda(StepConst c, const uint32_t* 
     PL[k] = make_float4(xi.x, xi.y, xi.z, lambda);
     rho_out[k] = rho;
     if (TAPS) {
-        cand_out[k] = ncand;
+        if (V == 0) cand_out[k] = ncand;  // the list form counts them in k_build_list
         nbr_out[k] = nnbr;
     }
 }
 
 // CS_ComputeDeltaP.glsl:120-166 fused with the clamp and add of
 // CS_ApplyDeltaP.glsl:80-106: reads (x, lambda) from PL, writes the corrected
-// position to P (the SDF projection follows in k_apply_sdf).  In the two-phase
+// position to P (the SDF projection follows in k_apply_sdf).  In the list
 // form the neighbours come from the list the lambda pass of this iteration
-// wrote (same positions, same order).
+// walked (same positions, same order).
 template <int V, bool TAPS, bool B>
 __global__ __launch_bounds__(kBlock) void k_deltap(StepConst c, const uint32_t* __restrict__ order,
                                                    const float4* __restrict__ PL, float4* __restrict__ P,
@@ -392,48 +453,15 @@ __global__ __launch_bounds__(kBlock) void k_deltap(StepConst c, const uint32_t* 
     const float4 xi4 = PL[k];
     const V3 xi = xyz(xi4);
     const float li = xi4.w;
-    const float h2 = c.h * c.h;
     V3 dp = mk(0.0f, 0.0f, 0.0f);
     auto body = [&](float lj, V3 r, float r2) {
         const V3 gw = spiky(c, r, r2);
         const float sc = scorr_r2(c, r2);
         dp = add(dp, scl(gw, li + lj + sc));
     };
-    auto single = [&]() {
-        scan_buckets(xi, c.h, domain_of<B>(c, k), cells, PL, k, [&](uint32_t, float4 xj, V3 r, float r2) {
-            if (r2 <= h2) body(xj.w, r, r2);
-        });
-    };
-    if constexpr (V == 0) {
-        single();
-    } else {
-        const uint32_t cnt = list_count[t];
-        if (cnt <= (uint32_t)kNbrCap) {
-            // four slots per 16-byte load (list_slot keeps them together)
-            uint32_t r = 0;
-#pragma unroll 1
-            for (; r + 4 <= cnt; r += 4) {
-                const uint4 j = *reinterpret_cast<const uint4*>(list + list_slot(t, r));
-                const float4 a0 = PL[j.x], a1 = PL[j.y], a2 = PL[j.z], a3 = PL[j.w];
-                const V3 d0 = sub(xi, xyz(a0));
-                body(a0.w, d0, dot(d0, d0));
-                const V3 d1 = sub(xi, xyz(a1));
-                body(a1.w, d1, dot(d1, d1));
-                const V3 d2 = sub(xi, xyz(a2));
-                body(a2.w, d2, dot(d2, d2));
-                const V3 d3 = sub(xi, xyz(a3));
-                body(a3.w, d3, dot(d3, d3));
-            }
-#pragma unroll 1
-            for (; r < cnt; ++r) {
-                const float4 a = PL[list[list_slot(t, r)]];
-                const V3 d = sub(xi, xyz(a));
-                body(a.w, d, dot(d, d));
-            }
-        } else {
-            single();
-        }
-    }
+    uint32_t ncand = 0, nnbr = 0;
+    gather<V, false>(xi, c.h, domain_of<B>(c, k), cells, PL, nullptr, c.n, k, t, list_count, list,
+                     [&](uint32_t, float4 xj, V3 r, float r2, float4) { body(xj.w, r, r2); }, ncand, nnbr);
     dp = scl(dp, c.inv_rho0);
     if (TAPS) dp_tap[k] = make_float4(dp.x, dp.y, dp.z, 0.0f);
     // apply: clamp |dp| <= h/2, x += dp, SDF projection
@@ -476,9 +504,10 @@ __global__ __launch_bounds__(kBlock) void k_apply_sdf(StepConst c, const uint32_
 // at :171).  It writes the XSPH velocity change and the curl.
 template <int V, bool B>
 __global__ __launch_bounds__(kBlock) void k_integrate(StepConst c, const uint32_t* __restrict__ order,
-                                                      const float4* __restrict__ P, const uint2* __restrict__ Q,
-                                                      const float4* __restrict__ vs,
-                                                      const int2* __restrict__ cells, float4* __restrict__ vdelta,
+                                                      const float4* __restrict__ P, const float4* __restrict__ vs,
+                                                      const int2* __restrict__ cells,
+                                                      const uint32_t* __restrict__ list_count,
+                                                      const uint32_t* __restrict__ list, float4* __restrict__ vdelta,
                                                       float4* __restrict__ curl_out) {
     const uint32_t t = xcd_gid();
     if (t >= c.n) return;
@@ -491,10 +520,9 @@ __global__ __launch_bounds__(kBlock) void k_integrate(StepConst c, const uint32_
     V3 visc = mk(0.0f, 0.0f, 0.0f);
     V3 curl = mk(0.0f, 0.0f, 0.0f);
     uint32_t ncand = 0, nnbr = 0;
-    __shared__ uint32_t lst[kLdsCap * kBlock];
-    gather<V, false>(xp, c.h, domain_of<B>(c, k), cells, P, Q, c.qscale, c.n, k, &lst[threadIdx.x],
-                     [&](uint32_t j, float4, V3 r, float r2) {
-        const V3 vij = sub(xyz(vs[j]), old_vel);
+    gather<V, false>(xp, c.h, domain_of<B>(c, k), cells, P, vs, c.n, k, t, list_count, list,
+                     [&](uint32_t, float4, V3 r, float r2, float4 vj) {
+        const V3 vij = sub(xyz(vj), old_vel);
         if (do_visc) {
             const float w = poly6_r2(c, r2);
             visc = add(visc, scl(scl(vij, w), inv));
@@ -574,10 +602,11 @@ __global__ __launch_bounds__(kBlock) void k_integrate_finish(StepConst c, const 
 // CS_VorticityConfinement.glsl:133-167 (pass 1)
 template <int V, bool B>
 __global__ __launch_bounds__(kBlock) void k_vorticity(StepConst c, const uint32_t* __restrict__ order,
-                                                      const float4* __restrict__ posS, const uint2* __restrict__ Q,
-                                                      const float4* __restrict__ curl,
+                                                      const float4* __restrict__ posS, const float4* __restrict__ curl,
                                                       const uint32_t* __restrict__ perm,
-                                                      float4* __restrict__ vel, const int2* __restrict__ cells) {
+                                                      float4* __restrict__ vel, const int2* __restrict__ cells,
+                                                      const uint32_t* __restrict__ list_count,
+                                                      const uint32_t* __restrict__ list) {
     const uint32_t t = xcd_gid();
     if (t >= c.n) return;
     const uint32_t k = order[t];
@@ -585,9 +614,8 @@ __global__ __launch_bounds__(kBlock) void k_vorticity(StepConst c, const uint32_
     const V3 pi = xyz(posS[k]);
     V3 eta = mk(0.0f, 0.0f, 0.0f);
     uint32_t ncand = 0, nnbr = 0;
-    __shared__ uint32_t lst[kLdsCap * kBlock];
-    gather<V, false>(pi, c.h, domain_of<B>(c, k), cells, posS, Q, c.qscale, c.n, k, &lst[threadIdx.x],
-                     [&](uint32_t, float4 xj, V3 r, float r2) {
+    gather<V, false>(pi, c.h, domain_of<B>(c, k), cells, posS, nullptr, c.n, k, t, list_count, list,
+                     [&](uint32_t, float4 xj, V3 r, float r2, float4) {
         const V3 gw = spiky(c, r, r2);
         eta = add(eta, scl(gw, xj.w));  // xj.w = |omega_j|
     }, ncand, nnbr);
@@ -1055,6 +1083,18 @@ hipError_t launch_lod(const StepConst& c, DeviceState& d, hipStream_t st) {
     return hipGetLastError();
 }
 
+// The list of the list-form gathers over the positions X (quantised in d.Q).
+template <bool B>
+static void build_list_t(const StepConst& c, DeviceState& d, const float4* X, const uint32_t* lod, uint32_t iter,
+                         bool taps, hipStream_t st) {
+    if (taps)
+        hipLaunchKernelGGL((k_build_list<true, B>), grid_for(c.n), dim3(kBlock), 0, st, c, d.order, X, d.Q, lod,
+                           iter, d.cells, d.list_count, d.list, d.cand);
+    else
+        hipLaunchKernelGGL((k_build_list<false, B>), grid_for(c.n), dim3(kBlock), 0, st, c, d.order, X, d.Q, lod,
+                           iter, d.cells, d.list_count, d.list, d.cand);
+}
+
 // gather form V (tuning knob) x taps x batched
 #define FSI_GATHER_SWITCH(CALL)                         \
     if (d.variant >= 1) {                               \
@@ -1079,12 +1119,17 @@ hipError_t launch_lambda(const StepConst& c, DeviceState& d, uint32_t iter, bool
 #undef L
         return hipGetLastError();
     }
-#define L(V, B)                                                                                                 \
-    if (taps)                                                                                                   \
-        hipLaunchKernelGGL((k_lambda<V, true, B>), grid_for(c.n), dim3(kBlock), 0, st, c, d.order, d.P, d.Q, d.PL, \
-                           d.rho, d.lod, d.cells, iter, d.list_count, d.list, d.cand, d.nbr);                   \
-    else                                                                                                        \
-        hipLaunchKernelGGL((k_lambda<V, false, B>), grid_for(c.n), dim3(kBlock), 0, st, c, d.order, d.P, d.Q, d.PL, \
+    if (d.variant >= 1) {
+#define L(B) build_list_t<B>(c, d, d.P, d.lod, iter, taps, st);
+        FSI_B_SWITCH(d.batched, L)
+#undef L
+    }
+#define L(V, B)                                                                                                  \
+    if (taps)                                                                                                    \
+        hipLaunchKernelGGL((k_lambda<V, true, B>), grid_for(c.n), dim3(kBlock), 0, st, c, d.order, d.P, d.PL,    \
+                           d.rho, d.lod, d.cells, iter, d.list_count, d.list, d.cand, d.nbr);                    \
+    else                                                                                                         \
+        hipLaunchKernelGGL((k_lambda<V, false, B>), grid_for(c.n), dim3(kBlock), 0, st, c, d.order, d.P, d.PL,   \
                            d.rho, d.lod, d.cells, iter, d.list_count, d.list, d.cand, d.nbr);
     FSI_GATHER_SWITCH(L)
 #undef L
@@ -1126,9 +1171,14 @@ hipError_t launch_deltap(const StepConst& c, DeviceState& d, uint32_t iter, bool
 
 hipError_t launch_integrate(const StepConst& c, DeviceState& d, hipStream_t st) {
     if (c.enable_visc == 1u || c.enable_vort == 1u) {
-#define L(V, B)                                                                                                     \
-    hipLaunchKernelGGL((k_integrate<V, B>), grid_for(c.n), dim3(kBlock), 0, st, c, d.order, d.P, d.Q, d.vs, d.cells, \
-                       d.vdelta, d.curl);
+        if (d.variant >= 1) {
+#define L(B) build_list_t<B>(c, d, d.P, nullptr, 0u, false, st);
+            FSI_B_SWITCH(d.batched, L)
+#undef L
+        }
+#define L(V, B)                                                                                                  \
+    hipLaunchKernelGGL((k_integrate<V, B>), grid_for(c.n), dim3(kBlock), 0, st, c, d.order, d.P, d.vs, d.cells, \
+                       d.list_count, d.list, d.vdelta, d.curl);
         FSI_GATHER_SWITCH(L)
 #undef L
     }
@@ -1141,9 +1191,14 @@ hipError_t launch_integrate(const StepConst& c, DeviceState& d, hipStream_t st) 
 }
 
 hipError_t launch_vorticity(const StepConst& c, DeviceState& d, hipStream_t st) {
-#define L(V, B)                                                                                                   \
-    hipLaunchKernelGGL((k_vorticity<V, B>), grid_for(c.n), dim3(kBlock), 0, st, c, d.order, d.posS, d.Q, d.curl, d.perm, \
-                       d.vel, d.cells);
+    if (d.variant >= 1) {
+#define L(B) build_list_t<B>(c, d, d.posS, nullptr, 0u, false, st);
+        FSI_B_SWITCH(d.batched, L)
+#undef L
+    }
+#define L(V, B)                                                                                                  \
+    hipLaunchKernelGGL((k_vorticity<V, B>), grid_for(c.n), dim3(kBlock), 0, st, c, d.order, d.posS, d.curl,    \
+                       d.perm, d.vel, d.cells, d.list_count, d.list);
     FSI_GATHER_SWITCH(L)
 #undef L
     return hipGetLastError();

@@ -68,7 +68,7 @@ struct DeviceState {
     uint32_t* bin_rank = nullptr;  // sorted slot -> rank inside its bin
     uint32_t* order = nullptr; // processing order: thread t handles sorted slot order[t]
     uint32_t* list_count = nullptr;  // by thread: neighbours found by the last lambda pass
-    uint32_t* list = nullptr;  // kNbrCap slots per thread, n rounded up to 64 (layout: list_slot in pbf_kernels.hip)
+    uint32_t* list = nullptr;  // kNbrCap slots per thread, n rounded up to 64 (layout: wave_run in pbf_kernels.hip)
     uint32_t* tflag = nullptr; // n: 1 where a tile starts in the processing order
     uint32_t* tidx = nullptr;  // n: exclusive scan of tflag
     uint32_t* tiles = nullptr; // n + 1: processing-order position where tile i starts (then the end)
