@@ -351,6 +351,15 @@ __global__ __launch_bounds__(kMB) void k_zero32(uint32_t* __restrict__ p, size_t
 
 // ---- contact forces ---------------------------------------------------------
 
+// counter += the number of this wave's lanes with hit: one atomic per wave,
+// by its first active lane, instead of one per lane on the same address.
+// Every active lane of the wave calls it.
+__device__ __forceinline__ void count_wave(uint32_t* counter, bool hit) {
+    const unsigned long long b = __ballot(hit);
+    const int first = __ffsll((long long)__ballot(1)) - 1;
+    if (b != 0ull && (int)__lane_id() == first) atomicAdd(counter, (uint32_t)__popcll(b));
+}
+
 // barycentric (src/main.cpp:557-571), FP64.
 __device__ void barycentric(Q3 pt, Q3 A, Q3 B, Q3 C, float& u, float& v, float& w) {
     const Q3 v0 = qsub(B, A), v1 = qsub(C, A), v2 = qsub(pt, A);
@@ -392,6 +401,7 @@ __global__ __launch_bounds__(kMB) void k_contact(MeshView m, const double* __res
     uint32_t key[3] = {nv, nv, nv};
     float4 f0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), f1 = f0;
     const uint32_t n = min(*count, cap);
+    bool contact = false;
     if (c < n) {
         const float4 p4 = cand[2 * (size_t)c], v4 = cand[2 * (size_t)c + 1];
         const DHit hit = bvh_closest(m.nodes, m.tris, Q3{(double)p4.x, (double)p4.y, (double)p4.z});
@@ -442,9 +452,10 @@ __global__ __launch_bounds__(kMB) void k_contact(MeshView m, const double* __res
             key[0] = (uint32_t)f[0];
             key[1] = (uint32_t)f[1];
             key[2] = (uint32_t)f[2];
-            atomicAdd(&counters[0], 1u);
+            contact = true;
         }
     }
+    count_wave(&counters[0], contact);
     cf[2 * (size_t)c] = f0;
     cf[2 * (size_t)c + 1] = f1;
     keys[3 * (size_t)c] = key[0];
@@ -487,7 +498,7 @@ __global__ __launch_bounds__(kMB) void k_vertex_sum(const int2* __restrict__ seg
     }
     forces[v] = make_float4(fx, fy, fz, 0.0f);
     const float mg = sqrtf(fx * fx + fy * fy + fz * fz);
-    if (!(mg < 1e-5f) && mg >= threshold) atomicAdd(&counters[1], 1u);
+    count_wave(&counters[1], !(mg < 1e-5f) && mg >= threshold);
 }
 
 }  // namespace
