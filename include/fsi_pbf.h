@@ -465,6 +465,24 @@ int fsi_pbf_set_sdf_from_mesh(fsi_pbf* h, fsi_mesh* m);
 int fsi_pbf_contact_forces(fsi_pbf* h, fsi_mesh* m, float band, int use_vel_magnitude,
                            float control_threshold, float* forces, uint32_t counts[2]);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops): the contact
+ * forces of fsi_pbf_contact_forces, same kernels and the same results, on
+ * `count` candidates given in device memory (32-byte fsi_contact_candidate
+ * records, e.g. those fsi_pbf_read_contacts returns), with every scratch
+ * buffer in one fixed device array (6.6 MiB per device, kept until the process
+ * ends) instead of buffers allocated on demand.  It calls no allocator.
+ * Limits: count <= 65536 candidates and a mesh of <= 65535 vertices, else
+ * FSI_ERR_INVALID_ARG.  Out, in device memory: forces, 4 floats per vertex
+ * (x, y, z, 0; 16-byte aligned); counts[2] as fsi_pbf_contact_forces.
+ * Enqueued on `stream` (NULL: the mesh's own) after the mesh's last update;
+ * the results are ready when the stream reaches this point.  Calls on one
+ * device share the scratch: calls that overlap in time on one device race
+ * (order them on one stream). */
+int fsi_mesh_contact_forces_noalloc(fsi_mesh* m, const fsi_contact_candidate* candidates, uint32_t count,
+                                    float band, int use_vel_magnitude, float control_threshold, float* forces,
+                                    uint32_t* counts, void* stream);
+
 /* ------------------------------------------------------------------------- */
 /* Differentiable frames.  No counterpart in the reference (its PBFluids has  */
 /* no gradient): the reverse-mode derivative of `frames` whole frames of      */

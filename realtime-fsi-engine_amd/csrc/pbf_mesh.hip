@@ -29,6 +29,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -390,7 +391,8 @@ __device__ void barycentric(Q3 pt, Q3 A, Q3 B, Q3 C, float& u, float& v, float& 
 
 // One candidate of computeContactForces (src/main.cpp:630-675): its force
 // (dir * mag) and barycentric weights, and the keys (vertex, or nverts when
-// it makes no contact) of its three corners.
+// it makes no contact) of its three corners.  The candidates are the first
+// min(*count, cap) of cand (all cap when count is NULL).
 __global__ __launch_bounds__(kMB) void k_contact(MeshView m, const double* __restrict__ verts,
                                                  const float4* __restrict__ cand, const uint32_t* __restrict__ count,
                                                  uint32_t cap, float band, int use_vel, uint32_t nv,
@@ -400,7 +402,7 @@ __global__ __launch_bounds__(kMB) void k_contact(MeshView m, const double* __res
     if (c >= cap) return;
     uint32_t key[3] = {nv, nv, nv};
     float4 f0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), f1 = f0;
-    const uint32_t n = min(*count, cap);
+    const uint32_t n = count ? min(*count, cap) : cap;
     bool contact = false;
     if (c < n) {
         const float4 p4 = cand[2 * (size_t)c], v4 = cand[2 * (size_t)c + 1];
@@ -707,6 +709,61 @@ int mesh_grid_for_solver(fsi_mesh* m, const float** grid, int32_t dims[3], doubl
 
 }  // namespace fsi
 
+// ---- contact forces without allocation --------------------------------------
+// fsi_mesh_contact_forces_noalloc runs the kernels of fsi_pbf_contact_forces
+// on caller-given device candidates, with its scratch (contact records, sort
+// buffers, histogram, vertex segments) in one fixed device array instead of
+// buffers the mesh allocates.  Limits: kNaMaxContacts candidates per call and
+// kNaMaxVerts mesh vertices (keys 0..nverts fit 16 bits).
+namespace {
+
+constexpr uint32_t kNaMaxContacts = 65536;
+constexpr uint32_t kNaMaxVerts = 65535;
+constexpr size_t kNaKeys = 3 * (size_t)kNaMaxContacts;
+constexpr size_t kNaHist = ((size_t)1 << fsi::kSortMaxDigitBits) * ((kNaKeys + 2047) / 2048);
+constexpr size_t kNaPartial = 1024;
+constexpr size_t kNaSeg = (size_t)1 << 16;
+constexpr size_t kNaOffCf = 0;
+constexpr size_t kNaOffKeys = kNaOffCf + 2 * (size_t)kNaMaxContacts * sizeof(float4);
+constexpr size_t kNaOffK0 = kNaOffKeys + kNaKeys * sizeof(uint32_t);
+constexpr size_t kNaOffK1 = kNaOffK0 + kNaKeys * sizeof(uint32_t);
+constexpr size_t kNaOffI0 = kNaOffK1 + kNaKeys * sizeof(uint32_t);
+constexpr size_t kNaOffI1 = kNaOffI0 + kNaKeys * sizeof(uint32_t);
+constexpr size_t kNaOffHist = kNaOffI1 + kNaKeys * sizeof(uint32_t);
+constexpr size_t kNaOffPartial = kNaOffHist + kNaHist * sizeof(uint32_t);
+constexpr size_t kNaOffSeg = kNaOffPartial + kNaPartial * sizeof(uint32_t);
+constexpr size_t kNaBytes = kNaOffSeg + kNaSeg * sizeof(int2);  // 6.6 MiB
+static_assert(kNaOffKeys % 256 == 0 && kNaOffK0 % 256 == 0 && kNaOffK1 % 256 == 0 && kNaOffI0 % 256 == 0 &&
+                  kNaOffI1 % 256 == 0 && kNaOffHist % 256 == 0 && kNaOffPartial % 256 == 0 && kNaOffSeg % 256 == 0,
+              "scratch sections start on 256-byte boundaries");
+
+// The scratch: one copy per device, held until the process ends.  Host code
+// never uses this name as an address (it is a host-side stand-in); it uses
+// the device address noalloc_scratch returns.
+__device__ __align__(256) unsigned char scratch[kNaBytes];
+
+constexpr int kNaDevices = 64;
+std::mutex g_na_mu;
+unsigned char* g_na_base[kNaDevices] = {};
+
+// The address of scratch on `device`, which must be current: looked up once
+// per device (hipGetSymbolAddress) and kept.
+int noalloc_scratch(int device, unsigned char** out) {
+    if (device < 0 || device >= kNaDevices) return fsi::fail(FSI_ERR_INVALID_ARG, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_na_mu);
+    if (!g_na_base[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess)
+            return fsi::fail(FSI_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_na_base[device] = static_cast<unsigned char*>(p);
+    }
+    *out = g_na_base[device];
+    return FSI_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 int fsi_mesh_create(const double* verts, uint32_t nverts, const int32_t* faces, uint32_t nfaces, double cell,
@@ -857,6 +914,10 @@ int fsi_mesh_create(const double* verts, uint32_t nverts, const int32_t* faces, 
     if (e == hipSuccess) e = record_ready(m);
     if (e == hipSuccess) e = hipStreamSynchronize(st);  // the host images go out of scope
     if (e != hipSuccess) return bail(fail(FSI_ERR_HIP, std::string("mesh upload: ") + hipGetErrorString(e)));
+    // set-up of fsi_mesh_contact_forces_noalloc on this device (no allocation)
+    unsigned char* na = nullptr;
+    const int rc = noalloc_scratch(m->device, &na);
+    if (rc) return bail(rc);
     *out = m;
     return FSI_OK;
 }
@@ -953,6 +1014,68 @@ int fsi_mesh_refit_dirty(fsi_mesh* m, const double* verts, double band_radius, d
     for (int k = 0; k < 3; ++k) {
         if (dirty_min) dirty_min[k] = d[0] ? d[1 + k] : 0;
         if (dirty_max) dirty_max[k] = d[0] ? d[4 + k] : -1;
+    }
+    return FSI_OK;
+}
+
+int fsi_mesh_contact_forces_noalloc(fsi_mesh* m, const fsi_contact_candidate* candidates, uint32_t count,
+                                    float band, int use_vel_magnitude, float control_threshold, float* forces,
+                                    uint32_t* counts, void* stream) {
+    int rc = bind(m);
+    if (rc) return rc;
+    if (!forces || !counts) return fail(FSI_ERR_INVALID_ARG, "forces and counts (device memory) are required");
+    if ((reinterpret_cast<uintptr_t>(forces) & 15u) != 0) return fail(FSI_ERR_INVALID_ARG, "forces must be 16-byte aligned");
+    if (count > 0 && !candidates) return fail(FSI_ERR_INVALID_ARG, "candidates is NULL");
+    if (count > kNaMaxContacts)
+        return fail(FSI_ERR_INVALID_ARG, "fsi_mesh_contact_forces_noalloc: " + std::to_string(count) +
+                                             " candidates, at most " + std::to_string(kNaMaxContacts));
+    if (m->nv > kNaMaxVerts)
+        return fail(FSI_ERR_INVALID_ARG, "fsi_mesh_contact_forces_noalloc: mesh of " + std::to_string(m->nv) +
+                                             " vertices, at most " + std::to_string(kNaMaxVerts));
+    if (sort_hist_entries((uint32_t)kNaKeys, kSortMaxDigitBits) > kNaHist ||
+        scan_partials_needed(std::max(kNaHist, kNaKeys)) > kNaPartial)
+        return fail(FSI_ERR_STATE, "no-allocation scratch is smaller than the sort needs");
+    unsigned char* base = nullptr;
+    if ((rc = noalloc_scratch(m->device, &base))) return rc;
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : m->stream;
+    if (st != m->stream) {  // after the mesh's last update
+        FSI_HIP_TRY(hipEventRecord(m->ev, m->stream));
+        FSI_HIP_TRY(hipStreamWaitEvent(st, m->ev, 0));
+    }
+    float4* cf = reinterpret_cast<float4*>(base + kNaOffCf);
+    uint32_t* keys = reinterpret_cast<uint32_t*>(base + kNaOffKeys);
+    SortScratch ss{};
+    ss.keys[0] = reinterpret_cast<uint32_t*>(base + kNaOffK0);
+    ss.keys[1] = reinterpret_cast<uint32_t*>(base + kNaOffK1);
+    ss.ids[0] = reinterpret_cast<uint32_t*>(base + kNaOffI0);
+    ss.ids[1] = reinterpret_cast<uint32_t*>(base + kNaOffI1);
+    ss.hist = reinterpret_cast<uint32_t*>(base + kNaOffHist);
+    ss.partial = reinterpret_cast<uint32_t*>(base + kNaOffPartial);
+    int2* seg = reinterpret_cast<int2*>(base + kNaOffSeg);
+    float4* fout = reinterpret_cast<float4*>(forces);
+    // every call starts from its own inputs: the counters and (for no
+    // candidates) the forces are cleared here, the rest is overwritten
+    hipLaunchKernelGGL(k_zero32, dim3(1), dim3(kMB), 0, st, counts, (size_t)2);
+    if (count == 0) {
+        hipLaunchKernelGGL(k_zero32, mgrid(4 * (size_t)m->nv), dim3(kMB), 0, st, reinterpret_cast<uint32_t*>(forces),
+                           4 * (size_t)m->nv);
+    } else {
+        const uint32_t nk = 3 * count;
+        hipLaunchKernelGGL(k_contact, mgrid(count), dim3(kMB), 0, st, view(m), m->verts,
+                           reinterpret_cast<const float4*>(candidates), nullptr, count, band,
+                           use_vel_magnitude ? 1 : 0, m->nv, cf, keys, counts);
+        uint32_t *skeys = nullptr, *sids = nullptr;
+        FSI_HIP_TRY(radix_sort_pairs(keys, nk, m->key_bits, kSortMaxDigitBits, ss, &skeys, &sids, st));
+        const uint32_t nseg = 1u << m->key_bits;
+        hipLaunchKernelGGL(k_seg_clear, mgrid(nseg), dim3(kMB), 0, st, seg, nseg);
+        hipLaunchKernelGGL(k_seg_build, mgrid(nk), dim3(kMB), 0, st, skeys, nk, seg);
+        hipLaunchKernelGGL(k_vertex_sum, mgrid(m->nv), dim3(kMB), 0, st, seg, sids, cf, m->nv, control_threshold, fout,
+                           counts);
+    }
+    FSI_HIP_TRY(hipGetLastError());
+    if (st != m->stream) {  // a later mesh update waits for this pass
+        FSI_HIP_TRY(hipEventRecord(m->ev, st));
+        FSI_HIP_TRY(hipStreamWaitEvent(m->stream, m->ev, 0));
     }
     return FSI_OK;
 }

@@ -234,6 +234,7 @@ def load() -> C.CDLL:
         "fsi_mesh_read_forces": ([P, f32p], C.c_int),
         "fsi_pbf_set_sdf_from_mesh": ([P, P], C.c_int),
         "fsi_pbf_contact_forces": ([P, P, C.c_float, C.c_int, C.c_float, f32p, u32p], C.c_int),
+        "fsi_mesh_contact_forces_noalloc": ([P, P, C.c_uint32, C.c_float, C.c_int, C.c_float, P, P, P], C.c_int),
         "fsi_deform_params_default": ([C.POINTER(CDeformParams)], C.c_int),
         "fsi_deform_create": ([f64p, C.c_uint32, i32p, C.c_uint32, C.POINTER(P)], C.c_int),
         "fsi_deform_destroy": ([P], C.c_int),
