@@ -427,21 +427,47 @@ def fsi_leg(fsi, frames):
             "finite": bool(np.all(np.isfinite(got[:, 0:3])))}
 
 
-def cpu_baseline(fsi, cfg, parts, sdf, frames=6):
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(fsi, cfg, parts, sdf, warmup=2, runs=5):
     """The CPU oracle (a C restatement of the reference shaders, OpenMP over
-    particles) on the same workload, timed over `frames` frames."""
-    threads = min(16, os.cpu_count() or 1)
+    particles; its -O3 x86-64-v3 timing build, same FP policy and bits) on the
+    same workload: warm-up frames, then the median of one-frame runs, on every
+    thread of the process's host share (OMP_NUM_THREADS where the host sets
+    it, else the CPUs this process may run on)."""
+    affinity = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    threads = int(os.environ.get("OMP_NUM_THREADS") or affinity)
     os.environ["OMP_NUM_THREADS"] = str(threads)
+    os.environ["FSI_ORACLE_LIB"] = os.path.join(ROOT, "oracle", "_build", "liboracle_fast.so")
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
 
     sim = pyoracle.OracleSim(cfg, parts, sdf=sdf, sdf_padding=0.025)
-    t0 = time.perf_counter()
-    sim.step(frames)
-    wall = time.perf_counter() - t0
+    sim.step(warmup)
+    times = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        sim.step(1)
+        times.append(time.perf_counter() - t0)
+    med = float(np.median(times))
     n = len(parts)
-    return {"value": n * SUBSTEPS * ITERS * frames / wall, "unit": "particle-iterations/s", "cores": threads,
-            "kind": "port", "sample": f"{frames} frames of the same 1,048,576-particle workload ({wall:.1f} s)"}
+    return {"value": n * SUBSTEPS * ITERS / med, "unit": "particle-iterations/s", "cores": threads,
+            "kind": "port",
+            "sample": f"median of {runs} one-frame runs (frames {warmup + 1}-{warmup + runs}, after {warmup} "
+                      f"warm-up frames) of the same 1,048,576-particle workload: {med:.2f} s per frame "
+                      f"(min {min(times):.2f}, max {max(times):.2f})",
+            "build": "gcc -O3 -march=x86-64-v3 -fopenmp -ffp-contract=off (oracle/_build/liboracle_fast.so)",
+            "cpu_model": cpu_model(), "host_cpus": os.cpu_count(), "process_cpus": affinity,
+            "omp_threads": threads}
 
 
 def main():
