@@ -21,10 +21,13 @@ particle counts log-uniform in [4096, 131072] (seeded), each a lattice block
 in the corner of its own box next to a static sphere obstacle of radius
 U[0.3, 1.0] given as an SDF grid; 4 iterations, 2 substeps.  The multi-scene
 handle (one launch per pass for all scenes, frames replayed from a HIP graph,
-knobs picked by its autotuner) is timed against a loop of one single-scene
-handle per scene, twice: each handle on its own stream (frames of different
-scenes overlap on the four hardware queues), and every handle on one stream
-(frames in submission order, as on the reference's one GL queue).
+knobs -- gather form, radix digit width, workgroup size, tile shape -- picked
+by its autotuner) is timed against a loop of one single-scene handle per
+scene under the same knobs, each replaying its frames from a HIP graph,
+twice: each handle on its own stream (frames of different scenes overlap on
+the four hardware queues), and every handle on one stream (frames in
+submission order, as on the reference's one GL queue).  Each form runs the
+median of five windows of --batch-steps frames.
 
 Third leg (reported under "adjoint"): the differentiable workload, 65,536
 particles (a 64x32x32 lattice block falling freely in the middle of a 6.4^3
@@ -38,11 +41,15 @@ Fourth leg (reported under "fsi"): the FSI coupling workload, 262,144
 particles (a 64^3 lattice block) falling onto a cantilever plate of 128 x 32
 quads (4.0 x 1.0, 8,192 triangles, 4,257 vertices) whose SDF (cell 0.05) lives
 on the device; every frame: contact forces from the last frame's candidates,
-a prescribed bending of the plate, dirty SDF refit, step, contact filter.
+the host Laplacian deformation of the plate under them (first column
+anchored), the SDF refresh, the fluid step and the contact filter.  It runs
+twice: with the SDF refreshed on every 3rd deformation, the reference's
+cadence, and (under "fsi.refit_every_frame") on every deformation.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch-steps K2]
                        [--fsi-steps K3] [--no-batched] [--no-adjoint] [--no-fsi]
-                       [--no-cpu-baseline] [--gather G] [--order O]
+                       [--no-cpu-baseline] [--gather G] [--order O] [--block B]
+                       [--tile T] [--no-autotune]
 """
 import argparse
 import json
@@ -135,22 +142,38 @@ def batch_config(fsi, box=(1.0, 1.0, 1.0), hash_size=1 << 17):
                            bounds_min=(0.0, 0.0, 0.0), bounds_max=box, hash_size=hash_size)
 
 
-def batched_leg(fsi, steps, warmup):
-    """Multi-scene handle vs a loop of single-scene handles on the same scenes."""
+def timed_runs(step, sync, steps, runs):
+    """Wall times of `runs` consecutive windows of `steps` frames each."""
+    walls = []
+    for _ in range(runs):
+        sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step()
+        sync()
+        walls.append(time.perf_counter() - t0)
+    return walls
+
+
+def batched_leg(fsi, steps, warmup, runs=5):
+    """Multi-scene handle vs a loop of single-scene handles on the same scenes,
+    every handle replaying its frames from a HIP graph under the knobs the
+    multi-scene autotuner picked.  Each form runs warmup frames, then `runs`
+    windows of `steps` frames; the median window is reported, and the final
+    states are compared bit for bit."""
     scenes = make_batch_scenes(fsi)
     total = sum(len(sc.particles) for sc in scenes)
     work = total * BATCH_SUBSTEPS * BATCH_ITERS * steps
+    counts = [len(x.particles) for x in scenes]
+    offsets = np.cumsum([0] + counts)
 
     b = fsi.Batch(batch_config(fsi))
     b.set_scenes(scenes)
-    gather, digit_bits = b.autotune(1)
+    b.autotune(1)
+    knobs = b.knobs()
     b.set_graph(True)
     b.step(warmup)
-    b.sync()
-    t0 = time.perf_counter()
-    b.step(steps)
-    b.sync()
-    wall_b = time.perf_counter() - t0
+    walls_b = timed_runs(lambda: b.step(1), b.sync, steps, runs)
     got = b.read_particles()
     b.close()
 
@@ -159,24 +182,26 @@ def batched_leg(fsi, steps, warmup):
         sv = fsi.Solver(batch_config(fsi, sc.bounds_max, sc.hash_size))
         sv.set_particles(sc.particles)
         sv.set_sdf(sc.sdf, sc.sdf_padding, False)
-        sv.set_tuning(fsi.TUNE_GATHER, gather)
-        sv.set_tuning(fsi.TUNE_SORT_DIGIT_BITS, digit_bits)
+        for name, k in fsi.KNOBS.items():
+            sv.set_tuning(k, knobs[name])
+        sv.set_graph(True)
         solvers.append(sv)
+
+    def loop_step():
+        for sv in solvers:
+            sv.step()
+
+    def loop_sync():
+        for sv in solvers:
+            sv.sync()
+
+    def same_as_batch():
+        return all(np.array_equal(sv.read_particles(), got[o:o + c]) for sv, o, c in zip(solvers, offsets, counts))
+
     for _ in range(warmup):
-        for sv in solvers:
-            sv.step()
-    for sv in solvers:
-        sv.sync()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        for sv in solvers:
-            sv.step()
-    for sv in solvers:
-        sv.sync()
-    wall_l = time.perf_counter() - t0
-    same = all(np.array_equal(sv.read_particles(), got[o:o + c])
-               for sv, o, c in zip(solvers, np.cumsum([0] + [len(x.particles) for x in scenes]),
-                                   [len(x.particles) for x in scenes]))
+        loop_step()
+    walls_l = timed_runs(loop_step, loop_sync, steps, runs)
+    same = same_as_batch()
 
     # The same loop with every handle on one stream: the scenes' frames run in
     # submission order, as the reference's dispatches on its one GL queue.
@@ -185,29 +210,27 @@ def batched_leg(fsi, steps, warmup):
         sv.set_particles(sc.particles)
         sv.set_stream(shared)
     for _ in range(warmup):
-        for sv in solvers:
-            sv.step()
-    solvers[0].sync()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        for sv in solvers:
-            sv.step()
-    solvers[0].sync()
-    wall_q = time.perf_counter() - t0
-    same_q = all(np.array_equal(sv.read_particles(), got[o:o + c])
-                 for sv, o, c in zip(solvers, np.cumsum([0] + [len(x.particles) for x in scenes]),
-                                     [len(x.particles) for x in scenes]))
+        loop_step()
+    walls_q = timed_runs(loop_step, solvers[0].sync, steps, runs)
+    same_q = same_as_batch()
     for sv in solvers[1:]:
         sv.set_stream(0)
     for sv in solvers:
         sv.close()
+    wall_b, wall_l, wall_q = (float(np.median(w)) for w in (walls_b, walls_l, walls_q))
     return {
         "scenes": BATCH_SCENES, "particles": total, "iterations": BATCH_ITERS, "substeps": BATCH_SUBSTEPS,
-        "steps": steps, "warmup": warmup, "value": work / wall_b, "ms_per_step": wall_b / steps * 1e3,
-        "loop_value": work / wall_l, "loop_ms_per_step": wall_l / steps * 1e3, "speedup": wall_l / wall_b,
+        "steps": steps, "warmup": warmup, "runs": runs, "of": "median window of `steps` frames over `runs` windows",
+        "value": work / wall_b, "ms_per_step": wall_b / steps * 1e3,
+        "ms_per_step_runs": [round(w / steps * 1e3, 3) for w in walls_b],
+        "loop_value": work / wall_l, "loop_ms_per_step": wall_l / steps * 1e3,
+        "loop_ms_per_step_runs": [round(w / steps * 1e3, 3) for w in walls_l],
+        "speedup": wall_l / wall_b,
         "loop_one_stream_value": work / wall_q, "loop_one_stream_ms_per_step": wall_q / steps * 1e3,
         "speedup_vs_one_stream_loop": wall_q / wall_b,
-        "autotuned": {"gather": gather, "sort_digit_bits": digit_bits}, "graph": True,
+        "autotuned": {"gather": knobs["gather"], "sort_digit_bits": knobs["sort_digit_bits"],
+                      "block": knobs["block"], "tile": knobs["tile"]},
+        "graph": {"batch": True, "loop": True, "loop_one_stream": True},
         "identical_to_loop": bool(same and same_q),
     }
 
@@ -319,7 +342,7 @@ def plate_mesh(nx=128, nz=32, size=(4.0, 1.0), origin=(1.2, 1.0, 2.7)):
     return verts, faces
 
 
-def fsi_leg(fsi, frames):
+def fsi_leg(fsi, frames, every=3):
     """The two-way coupling loop of src/main.cpp:1674-1707: contact forces on
     the device from the last frame's candidates, the host Laplacian
     deformation of the plate (deformStep, src/main.cpp:862-944: the plate's
@@ -331,7 +354,7 @@ def fsi_leg(fsi, frames):
     verts, faces = plate_mesh()
     cell, band, thresh, padding = 0.05, 0.12, 0.5, 0.025
     params = fsi.deform_params()  # stiffness 1, 0.02 per frame, 0.4 total, restore 0.05, thresholds 0.5 / 0.3
-    every, full_every = 3, 100  # g_sdfUpdateEvery, g_sdfFullRebuildEvery
+    full_every = 100  # g_sdfFullRebuildEvery; `every` is g_sdfUpdateEvery (3 in the reference)
     dirty_band = max(padding, 2.0 * cell) + every * float(params.max_displacement) + cell
     cfg = fsi.FluidConfig(dt=1.0 / 120.0, gravity=(0.0, -9.81, 0.0), h=0.1, rho0=6378.0, solver_iterations=6,
                           substep_iterations=2, eps=600.0, bounds_min=(0.0, 0.0, 0.0), bounds_max=(6.4, 6.4, 6.4),
@@ -408,7 +431,7 @@ def fsi_leg(fsi, frames):
     s.close()
     deform.close()
     mesh.close()
-    return {"particles": n, "frames": frames, "iterations": 6, "substeps": 2,
+    return {"particles": n, "frames": frames, "iterations": 6, "substeps": 2, "sdf_refresh_every": every,
             "plate": {"vertices": len(verts), "triangles": len(faces), "anchors": int(len(anchors)), "sdf_cell": cell,
                       "sdf_dims": list(mesh.dims)},
             "ms_per_frame": round(wall / frames * 1e3, 3),
@@ -475,7 +498,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--batch-steps", type=int, default=5)
+    ap.add_argument("--batch-steps", type=int, default=8)
+    ap.add_argument("--block", type=int, default=None, help="workgroup size of the gathers (128 or 256)")
+    ap.add_argument("--tile", type=int, default=None, help="tile shape of the tiled gathers (0, 1, 2)")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default knobs in the plain leg")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -516,12 +542,14 @@ def main():
     s = fsi.Solver(cfg)
     s.set_particles(parts)
     s.set_sdf(sdf, 0.025, False)
-    if args.gather is not None:
-        s.set_tuning(fsi.TUNE_GATHER, args.gather)
-    if args.order is not None:
-        s.set_tuning(fsi.TUNE_ORDER, args.order)
-    knobs = {"gather": s.get_tuning(fsi.TUNE_GATHER), "sort_digit_bits": s.get_tuning(fsi.TUNE_SORT_DIGIT_BITS),
-             "order": s.get_tuning(fsi.TUNE_ORDER)}
+    fixed = {"gather": args.gather, "order": args.order, "block": args.block, "tile": args.tile}
+    if all(v is None for v in fixed.values()) and not args.no_autotune:
+        s.autotune(2)  # times every knob setting from this state, which it restores
+    for name, v in fixed.items():
+        if v is not None:
+            s.set_tuning(fsi.KNOBS[name], v)
+    knobs = s.knobs()
+    knobs["autotuned"] = all(v is None for v in fixed.values()) and not args.no_autotune
     s.step(args.warmup)
     s.sync()
     # timed region: events bracket the launches of the dominant pass (lambda)
@@ -601,8 +629,10 @@ def main():
         out["adjoint"] = adjoint_leg(fsi)
         progress(f"adjoint leg: ratio {out['adjoint']['adjoint_forward_ratio']:.2f}")
     if not args.no_fsi:
-        out["fsi"] = fsi_leg(fsi, args.fsi_steps)
+        out["fsi"] = fsi_leg(fsi, args.fsi_steps, every=3)
         progress(f"fsi leg: {out['fsi']['ms_per_frame']:.3f} ms/frame")
+        out["fsi"]["refit_every_frame"] = fsi_leg(fsi, args.fsi_steps, every=1)
+        progress(f"fsi leg, refit every frame: {out['fsi']['refit_every_frame']['ms_per_frame']:.3f} ms/frame")
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(fsi, cfg, parts, sdf)
         progress("cpu baseline done")

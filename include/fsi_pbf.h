@@ -281,22 +281,29 @@ int fsi_pbf_read_timing(fsi_pbf* h, fsi_pbf_timing* out);
  * same results; only speed differs. */
 enum fsi_pbf_knob {
     FSI_TUNE_GATHER = 0,          /* neighbour-gather form: 0 single loop, 1 two-phase list, */
-                                  /* 2 LDS-staged 4x4x4-cell tiles                           */
+                                  /* 2 LDS-staged cell tiles                                 */
     FSI_TUNE_SORT_DIGIT_BITS = 1, /* widest radix digit, 1..10                                */
     FSI_TUNE_ORDER = 2,           /* thread -> particle order of the gather passes: 0 sorted  */
                                   /* slots as the sort leaves them, 1 Morton order of the      */
                                   /* predicted cells in per-XCD ranges (default; form 2 needs  */
                                   /* it and runs as form 1 under 0)                            */
-    FSI_TUNE_COUNT = 3
+    FSI_TUNE_BLOCK = 3,           /* workgroup size of the form-0/1 gathers: 128 or 256        */
+                                  /* (default; the reference's local_size_x)                  */
+    FSI_TUNE_TILE = 4,            /* cell-tile shape of form 2: 0 4x4x4 cells (default),      */
+                                  /* 1 8x4x4, 2 8x8x4                                          */
+    FSI_TUNE_COUNT = 5
 };
 int fsi_pbf_set_tuning(fsi_pbf* h, int knob, int value);
 int fsi_pbf_get_tuning(fsi_pbf* h, int knob, int* value);
 
-/* Autotuner: times `frames` frames of the current state under every
- * combination of gather form 0, 1 or 2 (2 when the processing order's bins
- * are 4x4x4-cell tiles) and digit width 6, 8 or 10 (restoring the particle
- * state after each trial), keeps the fastest and returns it.  No
- * counterpart in the reference. */
+/* Autotuner: times `frames` frames of the current state (after one
+ * untimed frame) under every combination of digit width 6, 8 or 10 with
+ * gather form 0 or 1 at workgroup size 128 or 256, and with gather form 2 at
+ * each tile shape (when the scene's box is binned by 4x4x4-cell tiles),
+ * restoring the particle state after each trial; keeps the fastest and
+ * returns its gather form and digit width (the workgroup size and tile shape
+ * through fsi_pbf_get_tuning).  The reference hard-codes its launch shape
+ * (src/fluid/PBFluids.cpp:330-332,353); no counterpart. */
 int fsi_pbf_autotune(fsi_pbf* h, uint32_t frames, int* gather, int* digit_bits);
 
 /* Replay each frame from a HIP graph captured on first use (re-captured when

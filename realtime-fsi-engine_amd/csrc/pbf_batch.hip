@@ -40,6 +40,7 @@ struct fsi_batch {
     uint16_t* d_scene_of = nullptr;
     uint32_t total_hash = 0;
     uint32_t axis_bits = 1;  // Morton bits per axis of every scene's bins
+    bool binned_tiles = false;  // doms are binned for the tiled gathers
     bool any_sdf = false;
     std::vector<float> host_pred;
 };
@@ -145,6 +146,7 @@ int fsi_batch_set_scenes(fsi_batch* b, const fsi_particle* ps, uint32_t total, c
     FSI_HIP_TRY(hipStreamSynchronize(c.stream));
     free_scenes(b);
     // Morton bins: split the 2^21 bins evenly, 3 * axis_bits per scene
+    b->binned_tiles = c.d.variant == 2;
     int sbits = 0;
     while ((1u << sbits) < nscenes) ++sbits;
     const uint32_t axis_bits = (uint32_t)std::max(1, std::min(7, (21 - sbits) / 3));
@@ -168,7 +170,7 @@ int fsi_batch_set_scenes(fsi_batch* b, const fsi_particle* ps, uint32_t total, c
         }
         d.hash_mask = h.hash_size - 1u;
         d.hash_base = h.hash_base;
-        set_binning(d, d.bmin, d.bmax, b->cfg.h, axis_bits, s << (3 * axis_bits), c.d.variant == 2);
+        set_binning(d, d.bmin, d.bmax, b->cfg.h, axis_bits, s << (3 * axis_bits), b->binned_tiles);
         if (sc[s].sdf_grid) {
             const size_t cnt = (size_t)sc[s].sdf_dims[0] * sc[s].sdf_dims[1] * sc[s].sdf_dims[2];
             FSI_HIP_TRY(dev_alloc(&h.sdf_grid, cnt));
@@ -299,23 +301,26 @@ int fsi_batch_set_graph(fsi_batch* b, int enable) {
     return FSI_OK;
 }
 
+// Re-derives every scene's binning for the tiled gathers (bins of
+// 2^kTileShift cells) or for the other forms, when it changes.
+static int rebin_scenes(fsi_batch* b, bool tiles) {
+    if (b->doms.empty() || b->binned_tiles == tiles) return FSI_OK;
+    FSI_HIP_TRY(hipStreamSynchronize(b->core.stream));
+    for (size_t s = 0; s < b->doms.size(); ++s) {
+        Domain& d = b->doms[s];
+        set_binning(d, d.bmin, d.bmax, b->cfg.h, b->axis_bits, (uint32_t)s << (3 * b->axis_bits), tiles);
+    }
+    FSI_HIP_TRY(hipMemcpy(b->d_doms, b->doms.data(), sizeof(Domain) * b->doms.size(), hipMemcpyHostToDevice));
+    b->binned_tiles = tiles;
+    core_invalidate_graph(b->core);
+    return FSI_OK;
+}
+
 int fsi_batch_set_tuning(fsi_batch* b, int knob, int value) {
     int rc = bind(b);
     if (rc) return rc;
-    const bool tiles_before = b->core.d.variant == 2;
     if ((rc = core_set_tuning(b->core, knob, value))) return rc;
-    const bool tiles = b->core.d.variant == 2;
-    if (tiles != tiles_before && !b->doms.empty()) {
-        // the tiled gathers take bins of 2^kTileShift cells
-        FSI_HIP_TRY(hipStreamSynchronize(b->core.stream));
-        for (size_t s = 0; s < b->doms.size(); ++s) {
-            Domain& d = b->doms[s];
-            set_binning(d, d.bmin, d.bmax, b->cfg.h, b->axis_bits, (uint32_t)s << (3 * b->axis_bits), tiles);
-        }
-        FSI_HIP_TRY(hipMemcpy(b->d_doms, b->doms.data(), sizeof(Domain) * b->doms.size(), hipMemcpyHostToDevice));
-        core_invalidate_graph(b->core);
-    }
-    return FSI_OK;
+    return rebin_scenes(b, b->core.d.variant == 2);
 }
 
 int fsi_batch_get_tuning(fsi_batch* b, int knob, int* value) {
@@ -326,8 +331,13 @@ int fsi_batch_get_tuning(fsi_batch* b, int knob, int* value) {
 int fsi_batch_autotune(fsi_batch* b, uint32_t frames, int* gather, int* digit_bits) {
     int rc = bind(b);
     if (rc) return rc;
-    prepare(b);
-    return core_autotune(b->core, step_const(b), frame_plan(b), frames, gather, digit_bits);
+    const Rebin rebin = [b](bool tiles, StepConst& k) {
+        const int rc = rebin_scenes(b, tiles);
+        prepare(b);
+        k = step_const(b);
+        return rc;
+    };
+    return core_autotune(b->core, rebin, frame_plan(b), frames, gather, digit_bits);
 }
 
 int fsi_batch_set_timing(fsi_batch* b, int enable) {

@@ -628,11 +628,15 @@ int fsi_pbf_get_tuning(fsi_pbf* h, int knob, int* value) {
 int fsi_pbf_autotune(fsi_pbf* h, uint32_t frames, int* gather, int* digit_bits) {
     int rc = bind(h);
     if (rc) return rc;
-    const StepConst c = step_const(h);
     h->core.d.batched = false;
-    h->core.d.any_sdf = c.dom.enable_sdf == 1u;
-    h->core.d.tiled = c.dom.dshift == kTileShift;
-    return core_autotune(h->core, c, frame_plan(h), frames, gather, digit_bits);
+    // the binning follows the gather form in step_const
+    const Rebin rebin = [h](bool, StepConst& k) {
+        k = step_const(h);
+        h->core.d.any_sdf = k.dom.enable_sdf == 1u;
+        h->core.d.tiled = k.dom.dshift == kTileShift;
+        return (int)FSI_OK;
+    };
+    return core_autotune(h->core, rebin, frame_plan(h), frames, gather, digit_bits);
 }
 
 int fsi_pbf_set_graph(fsi_pbf* h, int enable) {

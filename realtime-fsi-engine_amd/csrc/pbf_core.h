@@ -7,6 +7,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -100,10 +101,16 @@ int core_drain_timing(Core& c);
 int core_set_tuning(Core& c, int knob, int value);
 int core_get_tuning(const Core& c, int knob, int* value);
 void core_invalidate_graph(Core& c);
-// Times `frames` frames under the gather forms 0 and 1 and every radix digit
-// width, restoring the particle state (pos, vel) after each trial; keeps and
-// returns the fastest.
-int core_autotune(Core& c, const StepConst& k, const FramePlan& plan, uint32_t frames, int* gather, int* digit_bits);
+// The step constants of the frames to time, with the binning the tiled
+// gathers need (tiles) or the one the other forms use; also sets the device
+// state's `tiled`.
+using Rebin = std::function<int(bool tiles, StepConst& out)>;
+// Times `frames` frames under every combination of gather form, radix digit
+// width, workgroup size (forms 0 and 1) and tile shape (form 2, when the
+// binning gives tiles), restoring the particle state (pos, vel) after each
+// trial; keeps the fastest, leaves the binning for it, and returns its gather
+// form and digit width (the rest through core_get_tuning).
+int core_autotune(Core& c, const Rebin& rebin, const FramePlan& plan, uint32_t frames, int* gather, int* digit_bits);
 
 // Morton binning of one scene's processing order: cover its bounds (plus two
 // cells) with at most 2^axis_bits bins per axis, numbered from bin_base, each

@@ -298,7 +298,7 @@ hipError_t core_alloc_particles(Core& c, uint32_t n) {
     ALLOC(d.tiles, (size_t)n + 1);
     ALLOC(d.ntiles, 1);
     ALLOC(d.defer, n);
-    ALLOC(d.ndefer, 1);
+    ALLOC(d.ndefer, kDeferWords);
     ALLOC(d.sort.keys[0], n);
     ALLOC(d.sort.keys[1], n);
     ALLOC(d.sort.ids[0], n);
@@ -432,70 +432,98 @@ int core_frame(Core& c, const StepConst& k, const FramePlan& plan) {
     return FSI_OK;
 }
 
-int core_autotune(Core& c, const StepConst& k, const FramePlan& plan, uint32_t frames, int* gather, int* digit_bits) {
-    const uint32_t n = c.d.n;
-    if (n == 0 || frames == 0) {
-        if (gather) *gather = c.d.variant;
-        if (digit_bits) *digit_bits = c.sort_digit_bits;
-        return FSI_OK;
-    }
+namespace {
+// What core_autotune holds while it runs, released and restored on every exit.
+struct TuneScope {
+    Core& c;
     float4 *pos0 = nullptr, *vel0 = nullptr;
-    FSI_HIP_TRY(dev_alloc(&pos0, n));
-    FSI_HIP_TRY(dev_alloc(&vel0, n));
-    FSI_HIP_TRY(hipMemcpyAsync(pos0, c.d.pos, sizeof(float4) * n, hipMemcpyDeviceToDevice, c.stream));
-    FSI_HIP_TRY(hipMemcpyAsync(vel0, c.d.vel, sizeof(float4) * n, hipMemcpyDeviceToDevice, c.stream));
-    const bool timing = c.timing, graph = c.use_graph;
-    const fsi_pbf_timing acc = c.acc;
-    c.timing = false;
-    c.use_graph = false;
     hipEvent_t ea = nullptr, eb = nullptr;
-    FSI_HIP_TRY(hipEventCreate(&ea));
-    FSI_HIP_TRY(hipEventCreate(&eb));
-    float best = 1e30f;
-    int best_g = c.d.variant, best_b = c.sort_digit_bits;
+    bool timing, graph;
+    fsi_pbf_timing acc;
+    explicit TuneScope(Core& core) : c(core), timing(core.timing), graph(core.use_graph), acc(core.acc) {
+        c.timing = false;
+        c.use_graph = false;
+    }
+    void restore_state() {
+        const uint32_t n = c.d.n;
+        if (pos0) (void)hipMemcpyAsync(c.d.pos, pos0, sizeof(float4) * n, hipMemcpyDeviceToDevice, c.stream);
+        if (vel0) (void)hipMemcpyAsync(c.d.vel, vel0, sizeof(float4) * n, hipMemcpyDeviceToDevice, c.stream);
+    }
+    ~TuneScope() {
+        restore_state();
+        (void)hipStreamSynchronize(c.stream);
+        if (ea) (void)hipEventDestroy(ea);
+        if (eb) (void)hipEventDestroy(eb);
+        dev_free(&pos0);
+        dev_free(&vel0);
+        c.timing = timing;
+        c.use_graph = graph;
+        c.acc = acc;
+        core_invalidate_graph(c);
+    }
+};
+}  // namespace
+
+int core_autotune(Core& c, const Rebin& rebin, const FramePlan& plan, uint32_t frames, int* gather, int* digit_bits) {
+    const uint32_t n = c.d.n;
+    struct Knobs {
+        int g, bits, block, tile;
+    };
+    Knobs best{c.d.variant, c.sort_digit_bits, c.d.block, c.d.tile_shape};
     int rc = FSI_OK;
-    const int bits[] = {6, 8, 10};
-    // the staged-tile form (2) is a candidate when the processing order's
-    // bins are tiles; otherwise it would run as form 1
-    const int gmax = c.d.tiled ? 2 : 1;
-    for (int g = 0; g <= gmax && rc == FSI_OK; ++g)
+    if (n > 0 && frames > 0) {
+        TuneScope scope(c);
+        FSI_HIP_TRY(dev_alloc(&scope.pos0, n));
+        FSI_HIP_TRY(dev_alloc(&scope.vel0, n));
+        FSI_HIP_TRY(hipMemcpyAsync(scope.pos0, c.d.pos, sizeof(float4) * n, hipMemcpyDeviceToDevice, c.stream));
+        FSI_HIP_TRY(hipMemcpyAsync(scope.vel0, c.d.vel, sizeof(float4) * n, hipMemcpyDeviceToDevice, c.stream));
+        FSI_HIP_TRY(hipEventCreate(&scope.ea));
+        FSI_HIP_TRY(hipEventCreate(&scope.eb));
+        // the trials: forms 0 and 1 at both workgroup sizes, form 2 at every
+        // tile shape when its binning gives tiles (else it would run as form 1)
+        std::vector<Knobs> trials;
+        const int bits[] = {6, 8, 10};
         for (int b : bits) {
-            c.d.variant = g;
-            c.sort_digit_bits = b;
+            for (int g = 0; g <= 1; ++g)
+                for (int bs : {128, 256}) trials.push_back({g, b, bs, c.d.tile_shape});
+            for (int t = 0; t < kTileShapes; ++t) trials.push_back({2, b, c.d.block, t});
+        }
+        float best_ms = 1e30f;
+        for (const Knobs& t : trials) {
+            c.d.variant = t.g;
+            c.sort_digit_bits = t.bits;
+            c.d.block = t.block;
+            c.d.tile_shape = t.tile;
+            StepConst k{};
+            if ((rc = rebin(t.g == 2, k)) != FSI_OK) break;
+            if (t.g == 2 && !c.d.tiled) continue;
             if ((rc = enqueue_frame(c, k, plan, false)) != FSI_OK) break;  // warm-up
-            (void)hipEventRecord(ea, c.stream);
+            (void)hipEventRecord(scope.ea, c.stream);
             for (uint32_t f = 0; f < frames && rc == FSI_OK; ++f) rc = enqueue_frame(c, k, plan, false);
-            (void)hipEventRecord(eb, c.stream);
+            (void)hipEventRecord(scope.eb, c.stream);
             if (rc != FSI_OK) break;
-            if (hipEventSynchronize(eb) != hipSuccess) {
+            if (hipEventSynchronize(scope.eb) != hipSuccess) {
                 rc = fail(FSI_ERR_HIP, "autotune: event wait failed");
                 break;
             }
             float ms = 0.0f;
-            (void)hipEventElapsedTime(&ms, ea, eb);
-            if (ms < best) {
-                best = ms;
-                best_g = g;
-                best_b = b;
+            (void)hipEventElapsedTime(&ms, scope.ea, scope.eb);
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = t;
             }
-            (void)hipMemcpyAsync(c.d.pos, pos0, sizeof(float4) * n, hipMemcpyDeviceToDevice, c.stream);
-            (void)hipMemcpyAsync(c.d.vel, vel0, sizeof(float4) * n, hipMemcpyDeviceToDevice, c.stream);
+            scope.restore_state();
         }
-    (void)hipMemcpyAsync(c.d.pos, pos0, sizeof(float4) * n, hipMemcpyDeviceToDevice, c.stream);
-    (void)hipMemcpyAsync(c.d.vel, vel0, sizeof(float4) * n, hipMemcpyDeviceToDevice, c.stream);
-    (void)hipStreamSynchronize(c.stream);
-    (void)hipEventDestroy(ea);
-    (void)hipEventDestroy(eb);
-    dev_free(&pos0);
-    dev_free(&vel0);
-    c.d.variant = best_g;
-    c.sort_digit_bits = best_b;
-    c.timing = timing;
-    c.use_graph = graph;
-    c.acc = acc;
-    core_invalidate_graph(c);
-    if (gather) *gather = best_g;
-    if (digit_bits) *digit_bits = best_b;
+    }
+    c.d.variant = best.g;
+    c.sort_digit_bits = best.bits;
+    c.d.block = best.block;
+    c.d.tile_shape = best.tile;
+    StepConst k{};
+    const int rc2 = rebin(best.g == 2, k);
+    if (rc == FSI_OK) rc = rc2;
+    if (gather) *gather = best.g;
+    if (digit_bits) *digit_bits = best.bits;
     return rc;
 }
 
@@ -528,6 +556,14 @@ int core_set_tuning(Core& c, int knob, int value) {
             if (value < 0 || value > 1) return fail(FSI_ERR_INVALID_ARG, "processing order must be 0 or 1");
             c.d.morton = value == 1;
             return FSI_OK;
+        case FSI_TUNE_BLOCK:
+            if (value != 128 && value != 256) return fail(FSI_ERR_INVALID_ARG, "workgroup size must be 128 or 256");
+            c.d.block = value;
+            return FSI_OK;
+        case FSI_TUNE_TILE:
+            if (value < 0 || value >= kTileShapes) return fail(FSI_ERR_INVALID_ARG, "tile shape must be 0, 1 or 2");
+            c.d.tile_shape = value;
+            return FSI_OK;
         default:
             return fail(FSI_ERR_INVALID_ARG, "unknown knob");
     }
@@ -538,6 +574,8 @@ int core_get_tuning(const Core& c, int knob, int* value) {
         case FSI_TUNE_GATHER: *value = c.d.variant; return FSI_OK;
         case FSI_TUNE_SORT_DIGIT_BITS: *value = c.sort_digit_bits; return FSI_OK;
         case FSI_TUNE_ORDER: *value = c.d.morton ? 1 : 0; return FSI_OK;
+        case FSI_TUNE_BLOCK: *value = c.d.block; return FSI_OK;
+        case FSI_TUNE_TILE: *value = c.d.tile_shape; return FSI_OK;
         default: return fail(FSI_ERR_INVALID_ARG, "unknown knob");
     }
 }

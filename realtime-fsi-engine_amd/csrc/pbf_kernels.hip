@@ -40,6 +40,18 @@ __device__ __forceinline__ uint32_t xcd_gid() {
 
 inline dim3 grid_for(uint32_t n) { return dim3((n + kBlock - 1) / kBlock); }
 
+// The same for the gather kernels, whose workgroup size BS (128 or 256) is a
+// tuning knob (DeviceState::block).
+template <int BS>
+__device__ __forceinline__ uint32_t xcd_gid_bs() {
+    const uint32_t b = blockIdx.x, nb = gridDim.x;
+    const uint32_t q = nb >> 3, r = nb & 7u, x = b & 7u, l = b >> 3;
+    const uint32_t lb = (x < r) ? x * (q + 1) + l : r * (q + 1) + (x - r) * q + l;
+    return lb * BS + threadIdx.x;
+}
+template <int BS>
+inline dim3 grid_bs(uint32_t n) { return dim3((n + BS - 1) / BS); }
+
 // Per-lane LDS list of the two-phase form (entries), and the fill level at
 // which the whole wave flushes.
 constexpr uint32_t kLdsCap = 24;
@@ -83,21 +95,21 @@ constexpr uint32_t kLRing = 16;
 // buckets instead).  With lod, an APBF-inactive particle builds no list.
 // With TAPS, cand_out[k] counts the candidates other than the particle
 // itself, as the reference visits them.  X and Q hold n slots (n < 2^27).
-template <bool TAPS, bool B>
-__global__ __launch_bounds__(kBlock) void k_build_list(StepConst c, const uint32_t* __restrict__ order,
+template <bool TAPS, bool B, int BS>
+__global__ __launch_bounds__(BS) void k_build_list(StepConst c, const uint32_t* __restrict__ order,
                                                        const float4* __restrict__ X, const uint2* __restrict__ Q,
                                                        const uint32_t* __restrict__ lod, uint32_t iter,
                                                        const int2* __restrict__ cells,
                                                        uint32_t* __restrict__ list_count, uint32_t* __restrict__ list,
                                                        uint32_t* __restrict__ cand_out) {
-    const uint32_t t = xcd_gid();
+    const uint32_t t = xcd_gid_bs<BS>();
     if (t >= c.n) return;
     const uint32_t k = order[t];
     if (lod != nullptr && c.enable_apbf != 0u && lod[k] < iter) return;
     const V3 x = xyz(X[k]);
     const uint2 qi = quantize(x, c.qscale);
     const int32_t qpass = qpass_of(x, c.qscale);
-    __shared__ uint32_t stage[kLRing * kBlock];
+    __shared__ uint32_t stage[kLRing * BS];
     uint32_t* ring = &stage[threadIdx.x];
     uint4* run = wave_run(list, t);
     int32_t cx, cy, cz;
@@ -123,8 +135,8 @@ __global__ __launch_bounds__(kBlock) void k_build_list(StepConst c, const uint32
     // writes entries done..done+3 (the last group may hold stale slots past
     // the count: they are never read as entries)
     auto copy4 = [&]() {
-        const uint32_t* r = ring + (done & (kLRing - 1u)) * kBlock;
-        const uint4 v = make_uint4(r[0], r[kBlock], r[2 * kBlock], r[3 * kBlock]);
+        const uint32_t* r = ring + (done & (kLRing - 1u)) * BS;
+        const uint4 v = make_uint4(r[0], r[BS], r[2 * BS], r[3 * BS]);
         if (done < (uint32_t)kNbrCap) run[(done >> 2) * 64u] = v;
         done += 4;
     };
@@ -132,7 +144,7 @@ __global__ __launch_bounds__(kBlock) void k_build_list(StepConst c, const uint32
     // only for a valid slot that passes the pre-test
     auto visit = [&](uint32_t s, uint2 qs, bool valid) {
         if (TAPS) ncand += (valid && s != k) ? 1u : 0u;
-        ring[(tail & (kLRing - 1u)) * kBlock] = s;
+        ring[(tail & (kLRing - 1u)) * BS] = s;
         // bitwise, so that the test is not made a branch (which would sink
         // the group's loads into it)
         tail += (uint32_t)(qdist2(qi, qs) <= qpass) & (uint32_t)valid;
@@ -389,15 +401,15 @@ __global__ __launch_bounds__(kBlock) void k_lod(StepConst c, const float4* __res
 // CS_ComputeLambdas.glsl:108-162.  In the list form (V = 1) the neighbours
 // come from the list k_build_list wrote for these positions; the delta-p pass
 // of the same iteration walks it again.
-template <int V, bool TAPS, bool B>
-__global__ __launch_bounds__(kBlock) void k_lambda(StepConst c, const uint32_t* __restrict__ order,
+template <int V, bool TAPS, bool B, int BS>
+__global__ __launch_bounds__(BS) void k_lambda(StepConst c, const uint32_t* __restrict__ order,
                                                    const float4* __restrict__ P, float4* __restrict__ PL,
                                                    float* __restrict__ rho_out, const uint32_t* __restrict__ lod,
                                                    const int2* __restrict__ cells, uint32_t iter,
                                                    const uint32_t* __restrict__ list_count,
                                                    const uint32_t* __restrict__ list, uint32_t* __restrict__ cand_out,
                                                    uint32_t* __restrict__ nbr_out) {
-    const uint32_t t = xcd_gid();
+    const uint32_t t = xcd_gid_bs<BS>();
     if (t >= c.n) return;
     const uint32_t k = order[t];
     const V3 xi = xyz(P[k]);
@@ -439,14 +451,14 @@ __global__ __launch_bounds__(kBlock) void k_lambda(StepConst c, const uint32_t* 
 // position to P (the SDF projection follows in k_apply_sdf).  In the list
 // form the neighbours come from the list the lambda pass of this iteration
 // walked (same positions, same order).
-template <int V, bool TAPS, bool B>
-__global__ __launch_bounds__(kBlock) void k_deltap(StepConst c, const uint32_t* __restrict__ order,
+template <int V, bool TAPS, bool B, int BS>
+__global__ __launch_bounds__(BS) void k_deltap(StepConst c, const uint32_t* __restrict__ order,
                                                    const float4* __restrict__ PL, float4* __restrict__ P,
                                                    uint2* __restrict__ Q, const uint32_t* __restrict__ lod,
                                                    const int2* __restrict__ cells, uint32_t iter,
                                                    const uint32_t* __restrict__ list_count,
                                                    const uint32_t* __restrict__ list, float4* __restrict__ dp_tap) {
-    const uint32_t t = xcd_gid();
+    const uint32_t t = xcd_gid_bs<BS>();
     if (t >= c.n) return;
     const uint32_t k = order[t];
     if (c.enable_apbf != 0u && lod[k] < iter) return;  // P[k] already holds x
@@ -502,14 +514,14 @@ __global__ __launch_bounds__(kBlock) void k_apply_sdf(StepConst c, const uint32_
 // over the neighbours (CS_Integrate.glsl:160-205); neighbour velocities come
 // from the sorted post-predict snapshot vs (the Jacobi form of the racy read
 // at :171).  It writes the XSPH velocity change and the curl.
-template <int V, bool B>
-__global__ __launch_bounds__(kBlock) void k_integrate(StepConst c, const uint32_t* __restrict__ order,
+template <int V, bool B, int BS>
+__global__ __launch_bounds__(BS) void k_integrate(StepConst c, const uint32_t* __restrict__ order,
                                                       const float4* __restrict__ P, const float4* __restrict__ vs,
                                                       const int2* __restrict__ cells,
                                                       const uint32_t* __restrict__ list_count,
                                                       const uint32_t* __restrict__ list, float4* __restrict__ vdelta,
                                                       float4* __restrict__ curl_out) {
-    const uint32_t t = xcd_gid();
+    const uint32_t t = xcd_gid_bs<BS>();
     if (t >= c.n) return;
     const uint32_t k = order[t];
     const V3 old_vel = xyz(vs[k]);
@@ -600,14 +612,14 @@ __global__ __launch_bounds__(kBlock) void k_integrate_finish(StepConst c, const 
 }
 
 // CS_VorticityConfinement.glsl:133-167 (pass 1)
-template <int V, bool B>
-__global__ __launch_bounds__(kBlock) void k_vorticity(StepConst c, const uint32_t* __restrict__ order,
+template <int V, bool B, int BS>
+__global__ __launch_bounds__(BS) void k_vorticity(StepConst c, const uint32_t* __restrict__ order,
                                                       const float4* __restrict__ posS, const float4* __restrict__ curl,
                                                       const uint32_t* __restrict__ perm,
                                                       float4* __restrict__ vel, const int2* __restrict__ cells,
                                                       const uint32_t* __restrict__ list_count,
                                                       const uint32_t* __restrict__ list) {
-    const uint32_t t = xcd_gid();
+    const uint32_t t = xcd_gid_bs<BS>();
     if (t >= c.n) return;
     const uint32_t k = order[t];
     
@@ -665,59 +677,78 @@ __global__ __launch_bounds__(kBlock) void k_contact_write(StepConst c, const flo
 }
 
 // ---- tiled gathers (gather form 2) -----------------------------------------
-// The Morton bins of 4x4x4 cells are tiles.  One workgroup takes a tile: it
-// loads the cell-table entries of the 6x6x6 cells around the tile and copies
-// those buckets' records into LDS once, in bucket order; every particle of the
-// tile whose current cell is inside the tile then visits its 27 buckets from
-// LDS, in the reference order (an aliased bucket appears once per halo cell
-// that hashes to it, as it is visited once per occurrence).  A particle that
-// has moved out of its tile, or a tile whose buckets exceed the LDS capacity,
-// takes the global single-loop gather.  Tiles are spread over the XCDs in
-// contiguous Morton ranges.
+// The domains are binned by Morton bins of 4x4x4 cells; a tile is a run of 2^M
+// consecutive bins (the cell-tile shape, a tuning knob): 4x4x4 cells for M = 0,
+// 8x4x4 for M = 1, 8x8x4 for M = 2.  One workgroup takes a tile: it loads the
+// cell-table entries of the tile's cells and a one-cell halo around them and
+// copies those buckets' records into LDS once, in bucket order; every particle
+// of the tile whose current cell is inside the tile then visits its 27 buckets
+// from LDS, in the reference order (an aliased bucket appears once per halo
+// cell that hashes to it, as it is visited once per occurrence).  A particle
+// that has moved out of its tile, or a tile whose buckets exceed the LDS
+// capacity, takes the global single-loop gather.  The pass is persistent:
+// kTileGrid workgroups take tiles from a queue per XCD, each XCD over a
+// contiguous Morton range, until the range is done, so tiles of uneven
+// occupancy (uneven scenes in a batch) balance within the XCD.
 
-constexpr int kTileW = 1 << kTileShift;
-constexpr int kHaloW = kTileW + 2;
-constexpr int kHaloCells = kHaloW * kHaloW * kHaloW;
-constexpr uint32_t kTileCap = 2560;  // staged records per tile (40 KiB)
+template <int M>
+struct TileShape {
+    static constexpr int TX = (1 << kTileShift) << ((M + 2) / 3);
+    static constexpr int TY = (1 << kTileShift) << ((M + 1) / 3);
+    static constexpr int TZ = (1 << kTileShift) << (M / 3);
+    static constexpr int HX = TX + 2, HY = TY + 2, HZ = TZ + 2;
+    static constexpr int HC = HX * HY * HZ;                   // halo cells
+    static constexpr uint32_t Cap = M == 0 ? 2560u : M == 1 ? 3584u : 5120u;  // staged records
+};
 constexpr uint32_t kTileGrid = 768;  // persistent workgroups: 3 per CU, a multiple of 8
 constexpr uint32_t kDeferGrid = 256; // workgroups of the deferred passes
 
+template <int M>
 struct TileStage {
     int32_t o[3];  // first cell of the tile
     uint32_t total;
-    int32_t hstart[kHaloCells];  // first slot of the bucket of halo cell h, -1 empty
-    uint32_t hcnt[kHaloCells];
-    uint32_t hoff[kHaloCells];   // its first record in x[]
-    float4 x[kTileCap];
+    uint32_t next;  // the tile this workgroup took from its queue
+    int32_t hstart[TileShape<M>::HC];  // first slot of the bucket of halo cell h, -1 empty
+    uint32_t hcnt[TileShape<M>::HC];
+    uint32_t hoff[TileShape<M>::HC];   // its first record in x[]
+    float4 x[TileShape<M>::Cap];
 };
 
-// Tile range of this workgroup's XCD and its first tile and stride in it.
-__device__ __forceinline__ void tile_span(uint32_t nt, uint32_t& t, uint32_t& end, uint32_t& stride) {
-    const uint32_t x = blockIdx.x & 7u, lb = blockIdx.x >> 3;
-    t = (uint32_t)(((uint64_t)nt * x) >> 3) + lb;
-    end = (uint32_t)(((uint64_t)nt * (x + 1)) >> 3);
-    stride = gridDim.x >> 3;
+// The next tile of this workgroup's XCD range (tiles [nt*x/8, nt*(x+1)/8) for
+// XCD x), taken from the XCD's queue head heads[x] (zeroed before the pass),
+// or UINT32_MAX when the range is done; the same for the whole workgroup.  The
+// caller's previous tile ended with a barrier, so ts.next is free.
+template <int M>
+__device__ __forceinline__ uint32_t next_tile(TileStage<M>& ts, uint32_t nt, uint32_t* __restrict__ heads) {
+    const uint32_t x = blockIdx.x & 7u;
+    const uint32_t begin = (uint32_t)(((uint64_t)nt * x) >> 3);
+    const uint32_t end = (uint32_t)(((uint64_t)nt * (x + 1)) >> 3);
+    if (threadIdx.x == 0) ts.next = begin + atomicAdd(&heads[x], 1u);
+    __syncthreads();
+    const uint32_t t = ts.next;
+    return t < end ? t : UINT32_MAX;
 }
 
-// Stages tile [start, ...) of the processing order; returns whether its
-// buckets fit.  Ends with a barrier.
-template <bool B>
-__device__ bool stage_tile(const StepConst& c, TileStage& ts, const float4* __restrict__ X,
+// Stages the tile whose first particle is at [start] of the processing order;
+// returns whether its buckets fit.  Ends with a barrier.
+template <int M, bool B>
+__device__ bool stage_tile(const StepConst& c, TileStage<M>& ts, const float4* __restrict__ X,
                            const int2* __restrict__ cells, const uint32_t* __restrict__ order,
                            const uint32_t* __restrict__ bin_of, uint32_t start) {
+    using S = TileShape<M>;
     const uint32_t k0 = order[start];
     const Domain& dm = domain_of<B>(c, k0);
-    const uint32_t code = bin_of[k0] - dm.bin_base;
+    const uint32_t code = (bin_of[k0] - dm.bin_base) & ~((1u << M) - 1u);  // the tile's first bin
     const int32_t ox = dm.dcmin[0] + (int32_t)(compact7(code) << kTileShift);
     const int32_t oy = dm.dcmin[1] + (int32_t)(compact7(code >> 1) << kTileShift);
     const int32_t oz = dm.dcmin[2] + (int32_t)(compact7(code >> 2) << kTileShift);
     const uint32_t tid = threadIdx.x;
-    if (tid < (uint32_t)kHaloCells) {
-        const int32_t hx = (int32_t)(tid % kHaloW), hy = (int32_t)((tid / kHaloW) % kHaloW),
-                      hz = (int32_t)(tid / (kHaloW * kHaloW));
+    for (uint32_t hc = tid; hc < (uint32_t)S::HC; hc += kBlock) {
+        const int32_t hx = (int32_t)(hc % S::HX), hy = (int32_t)((hc / S::HX) % S::HY),
+                      hz = (int32_t)(hc / (S::HX * S::HY));
         const int2 se = cells[dm.hash_base + cell_hash(ox - 1 + hx, oy - 1 + hy, oz - 1 + hz, dm.hash_mask)];
-        ts.hstart[tid] = se.x;
-        ts.hcnt[tid] = se.x < 0 ? 0u : (uint32_t)(se.y - se.x);
+        ts.hstart[hc] = se.x;
+        ts.hcnt[hc] = se.x < 0 ? 0u : (uint32_t)(se.y - se.x);
     }
     if (tid == 0) {
         ts.o[0] = ox;
@@ -725,11 +756,12 @@ __device__ bool stage_tile(const StepConst& c, TileStage& ts, const float4* __re
         ts.o[2] = oz;
     }
     __syncthreads();
-    if (tid < 64) {  // wave 0: exclusive scan of the counts, four per lane
+    if (tid < 64) {  // wave 0: exclusive scan of the counts, kPer per lane
+        constexpr int kPer = (S::HC + 63) / 64;
         uint32_t local = 0;
-        for (int j = 0; j < 4; ++j) {
-            const uint32_t h = 4 * tid + j;
-            if (h < (uint32_t)kHaloCells) local += ts.hcnt[h];
+        for (int j = 0; j < kPer; ++j) {
+            const uint32_t h = kPer * tid + j;
+            if (h < (uint32_t)S::HC) local += ts.hcnt[h];
         }
         uint32_t v = local;
         for (int d = 1; d < 64; d <<= 1) {
@@ -737,9 +769,9 @@ __device__ bool stage_tile(const StepConst& c, TileStage& ts, const float4* __re
             if ((int)tid >= d) v += u;
         }
         uint32_t run = v - local;
-        for (int j = 0; j < 4; ++j) {
-            const uint32_t h = 4 * tid + j;
-            if (h < (uint32_t)kHaloCells) {
+        for (int j = 0; j < kPer; ++j) {
+            const uint32_t h = kPer * tid + j;
+            if (h < (uint32_t)S::HC) {
                 ts.hoff[h] = run;
                 run += ts.hcnt[h];
             }
@@ -748,10 +780,10 @@ __device__ bool stage_tile(const StepConst& c, TileStage& ts, const float4* __re
     }
     __syncthreads();
     const uint32_t total = ts.total;
-    const bool ok = total <= kTileCap;
+    const bool ok = total <= S::Cap;
     if (ok) {
         for (uint32_t e = tid; e < total; e += kBlock) {
-            uint32_t lo = 0, hi = kHaloCells - 1;  // last halo cell whose records start at or before e
+            uint32_t lo = 0, hi = S::HC - 1;  // last halo cell whose records start at or before e
             while (lo < hi) {
                 const uint32_t mid = (lo + hi + 1) >> 1;
                 if (ts.hoff[mid] <= e) lo = mid;
@@ -764,18 +796,26 @@ __device__ bool stage_tile(const StepConst& c, TileStage& ts, const float4* __re
     return ok;
 }
 
+// Whether the cell (lx, ly, lz) relative to the tile is inside it.
+template <int M>
+__device__ __forceinline__ bool in_tile(int32_t lx, int32_t ly, int32_t lz) {
+    using S = TileShape<M>;
+    return lx >= 0 && lx < S::TX && ly >= 0 && ly < S::TY && lz >= 0 && lz < S::TZ;
+}
+
 // Phase 1 of the two-phase gather over the staged tile: as collect_flush, with
 // candidates read from LDS and their record numbers in x[] collected.  (lx,
-// ly, lz) is the particle's cell relative to the tile, each in [0, kTileW).
-template <bool TAPS, class Flush>
-__device__ __forceinline__ uint32_t collect_flush_tile(const TileStage& ts, V3 x, float h, int32_t lx, int32_t ly,
+// ly, lz) is the particle's cell relative to the tile, inside it.
+template <int M, bool TAPS, class Flush>
+__device__ __forceinline__ uint32_t collect_flush_tile(const TileStage<M>& ts, V3 x, float h, int32_t lx, int32_t ly,
                                                        int32_t lz, uint32_t self, uint16_t* col, Flush&& flush,
                                                        uint32_t& ncand) {
     const float h2 = h * h;
     uint32_t cnt = 0, total = 0;
 #pragma unroll 1
     for (int q = 0; q < 27; ++q) {
-        const int hc = (lx + q % 3) + kHaloW * (ly + (q / 3) % 3) + kHaloW * kHaloW * (lz + q / 9);
+        const int hc = (lx + q % 3) + TileShape<M>::HX * (ly + (q / 3) % 3) +
+                       TileShape<M>::HX * TileShape<M>::HY * (lz + q / 9);
         const uint32_t nb = ts.hcnt[hc];
         const uint32_t base = ts.hoff[hc];
         const uint32_t s0 = (uint32_t)ts.hstart[hc];
@@ -824,7 +864,7 @@ __device__ __forceinline__ void lambda_store(const StepConst& c, uint32_t k, V3 
 
 // CS_ComputeLambdas.glsl:108-162, tiled.  Particles outside their tile are
 // appended to defer[] for k_lambda_deferred.
-template <bool TAPS, bool B>
+template <int M, bool TAPS, bool B>
 __global__ __launch_bounds__(kBlock) void k_lambda_tile(StepConst c, const uint32_t* __restrict__ tiles,
                                                         const uint32_t* __restrict__ ntiles,
                                                         const uint32_t* __restrict__ order,
@@ -834,15 +874,14 @@ __global__ __launch_bounds__(kBlock) void k_lambda_tile(StepConst c, const uint3
                                                         const int2* __restrict__ cells, uint32_t iter,
                                                         uint32_t* __restrict__ cand_out, uint32_t* __restrict__ nbr_out,
                                                         uint32_t* __restrict__ defer, uint32_t* __restrict__ ndefer) {
-    __shared__ TileStage ts;
+    __shared__ TileStage<M> ts;
     __shared__ uint16_t lst[kLdsCap * kBlock];
     uint16_t* col = &lst[threadIdx.x];
-    uint32_t tile, tend, stride;
-    tile_span(*ntiles, tile, tend, stride);
+    const uint32_t nt = *ntiles;
     const float inv = c.inv_rho0;
-    for (; tile < tend; tile += stride) {
+    for (uint32_t tile; (tile = next_tile<M>(ts, nt, ndefer + 1)) != UINT32_MAX;) {
         const uint32_t start = tiles[tile], end = tiles[tile + 1];
-        const bool ok = stage_tile<B>(c, ts, P, cells, order, bin_of, start);
+        const bool ok = stage_tile<M, B>(c, ts, P, cells, order, bin_of, start);
         for (uint32_t t = start + threadIdx.x; t < end; t += kBlock) {
             const uint32_t k = order[t];
             const V3 xi = xyz(P[k]);
@@ -853,7 +892,7 @@ __global__ __launch_bounds__(kBlock) void k_lambda_tile(StepConst c, const uint3
             int32_t cx, cy, cz;
             cell_of(xi, c.h, cx, cy, cz);
             const int32_t lx = cx - ts.o[0], ly = cy - ts.o[1], lz = cz - ts.o[2];
-            if (!(ok && lx >= 0 && lx < kTileW && ly >= 0 && ly < kTileW && lz >= 0 && lz < kTileW)) {
+            if (!(ok && in_tile<M>(lx, ly, lz))) {
                 defer[atomicAdd(ndefer, 1u)] = k;
                 continue;
             }
@@ -861,7 +900,7 @@ __global__ __launch_bounds__(kBlock) void k_lambda_tile(StepConst c, const uint3
             float sum_g2 = 0.0f;
             V3 gi = mk(0.0f, 0.0f, 0.0f);
             uint32_t ncand = 0;
-            const uint32_t nnbr = collect_flush_tile<TAPS>(ts, xi, c.h, lx, ly, lz, k, col, [&](uint32_t, uint32_t cnt) {
+            const uint32_t nnbr = collect_flush_tile<M, TAPS>(ts, xi, c.h, lx, ly, lz, k, col, [&](uint32_t, uint32_t cnt) {
 #pragma unroll 1
                 for (uint32_t e = 0; e < cnt; ++e) {
                     const V3 d = sub(xi, xyz(ts.x[col[e * kBlock]]));
@@ -932,7 +971,7 @@ __device__ __forceinline__ void deltap_store(const StepConst& c, uint32_t k, V3 
 
 // CS_ComputeDeltaP.glsl:120-166 + the clamp and add of CS_ApplyDeltaP.glsl:80-106,
 // tiled; particles outside their tile go to defer[] for k_deltap_deferred.
-template <bool TAPS, bool B>
+template <int M, bool TAPS, bool B>
 __global__ __launch_bounds__(kBlock) void k_deltap_tile(StepConst c, const uint32_t* __restrict__ tiles,
                                                         const uint32_t* __restrict__ ntiles,
                                                         const uint32_t* __restrict__ order,
@@ -942,14 +981,13 @@ __global__ __launch_bounds__(kBlock) void k_deltap_tile(StepConst c, const uint3
                                                         const int2* __restrict__ cells, uint32_t iter,
                                                         float4* __restrict__ dp_tap, uint32_t* __restrict__ defer,
                                                         uint32_t* __restrict__ ndefer) {
-    __shared__ TileStage ts;
+    __shared__ TileStage<M> ts;
     __shared__ uint16_t lst[kLdsCap * kBlock];
     uint16_t* col = &lst[threadIdx.x];
-    uint32_t tile, tend, stride;
-    tile_span(*ntiles, tile, tend, stride);
-    for (; tile < tend; tile += stride) {
+    const uint32_t nt = *ntiles;
+    for (uint32_t tile; (tile = next_tile<M>(ts, nt, ndefer + 1)) != UINT32_MAX;) {
         const uint32_t start = tiles[tile], end = tiles[tile + 1];
-        const bool ok = stage_tile<B>(c, ts, PL, cells, order, bin_of, start);
+        const bool ok = stage_tile<M, B>(c, ts, PL, cells, order, bin_of, start);
         for (uint32_t t = start + threadIdx.x; t < end; t += kBlock) {
             const uint32_t k = order[t];
             if (c.enable_apbf != 0u && lod[k] < iter) continue;  // P[k] already holds x
@@ -958,14 +996,14 @@ __global__ __launch_bounds__(kBlock) void k_deltap_tile(StepConst c, const uint3
             int32_t cx, cy, cz;
             cell_of(xi, c.h, cx, cy, cz);
             const int32_t lx = cx - ts.o[0], ly = cy - ts.o[1], lz = cz - ts.o[2];
-            if (!(ok && lx >= 0 && lx < kTileW && ly >= 0 && ly < kTileW && lz >= 0 && lz < kTileW)) {
+            if (!(ok && in_tile<M>(lx, ly, lz))) {
                 defer[atomicAdd(ndefer, 1u)] = k;
                 continue;
             }
             const float li = xi4.w;
             V3 dp = mk(0.0f, 0.0f, 0.0f);
             uint32_t ncand = 0;
-            collect_flush_tile<false>(ts, xi, c.h, lx, ly, lz, k, col, [&](uint32_t, uint32_t cnt) {
+            collect_flush_tile<M, false>(ts, xi, c.h, lx, ly, lz, k, col, [&](uint32_t, uint32_t cnt) {
 #pragma unroll 1
                 for (uint32_t e = 0; e < cnt; ++e) {
                     const float4 a = ts.x[col[e * kBlock]];
@@ -1007,13 +1045,14 @@ __global__ __launch_bounds__(kBlock) void k_deltap_deferred(StepConst c, const u
     }
 }
 
-// Tile list: the positions of the processing order where the bin changes.
-__global__ __launch_bounds__(kBlock) void k_tile_flags(uint32_t n, const uint32_t* __restrict__ order,
+// Tile list: the positions of the processing order where the tile (the bin
+// without its low m Morton bits) changes.
+__global__ __launch_bounds__(kBlock) void k_tile_flags(uint32_t n, uint32_t m, const uint32_t* __restrict__ order,
                                                        const uint32_t* __restrict__ bin_of,
                                                        uint32_t* __restrict__ tflag) {
     const uint32_t t = gid();
     if (t >= n) return;
-    tflag[t] = (t == 0 || bin_of[order[t]] != bin_of[order[t - 1]]) ? 1u : 0u;
+    tflag[t] = (t == 0 || (bin_of[order[t]] >> m) != (bin_of[order[t - 1]] >> m)) ? 1u : 0u;
 }
 
 __global__ __launch_bounds__(kBlock) void k_tile_list(uint32_t n, const uint32_t* __restrict__ tflag,
@@ -1070,7 +1109,8 @@ hipError_t launch_reorder(const StepConst& c, DeviceState& d, hipStream_t st) {
     else
         hipLaunchKernelGGL(k_order_slots, grid_for(c.n), dim3(kBlock), 0, st, c.n, d.order);
     if (tiled_form(d)) {
-        hipLaunchKernelGGL(k_tile_flags, grid_for(c.n), dim3(kBlock), 0, st, c.n, d.order, d.bin_of, d.tflag);
+        hipLaunchKernelGGL(k_tile_flags, grid_for(c.n), dim3(kBlock), 0, st, c.n, (uint32_t)d.tile_shape, d.order,
+                           d.bin_of, d.tflag);
         e = scan_exclusive_u32(d.tflag, d.tidx, c.n, d.sort.partial, st);
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL(k_tile_list, grid_for(c.n), dim3(kBlock), 0, st, c.n, d.tflag, d.tidx, d.tiles, d.ntiles);
@@ -1084,31 +1124,47 @@ hipError_t launch_lod(const StepConst& c, DeviceState& d, hipStream_t st) {
 }
 
 // The list of the list-form gathers over the positions X (quantised in d.Q).
+template <bool B, int BS>
+static void build_list_bs(const StepConst& c, DeviceState& d, const float4* X, const uint32_t* lod, uint32_t iter,
+                          bool taps, hipStream_t st) {
+    if (taps)
+        hipLaunchKernelGGL((k_build_list<true, B, BS>), grid_bs<BS>(c.n), dim3(BS), 0, st, c, d.order, X, d.Q, lod,
+                           iter, d.cells, d.list_count, d.list, d.cand);
+    else
+        hipLaunchKernelGGL((k_build_list<false, B, BS>), grid_bs<BS>(c.n), dim3(BS), 0, st, c, d.order, X, d.Q, lod,
+                           iter, d.cells, d.list_count, d.list, d.cand);
+}
 template <bool B>
 static void build_list_t(const StepConst& c, DeviceState& d, const float4* X, const uint32_t* lod, uint32_t iter,
                          bool taps, hipStream_t st) {
-    if (taps)
-        hipLaunchKernelGGL((k_build_list<true, B>), grid_for(c.n), dim3(kBlock), 0, st, c, d.order, X, d.Q, lod,
-                           iter, d.cells, d.list_count, d.list, d.cand);
-    else
-        hipLaunchKernelGGL((k_build_list<false, B>), grid_for(c.n), dim3(kBlock), 0, st, c, d.order, X, d.Q, lod,
-                           iter, d.cells, d.list_count, d.list, d.cand);
+    if (d.block == 128) build_list_bs<B, 128>(c, d, X, lod, iter, taps, st);
+    else build_list_bs<B, 256>(c, d, X, lod, iter, taps, st);
 }
 
-// gather form V (tuning knob) x taps x batched
-#define FSI_GATHER_SWITCH(CALL)                         \
-    if (d.variant >= 1) {                               \
-        if (d.batched) { CALL(1, true); } else { CALL(1, false); } \
-    } else {                                            \
-        if (d.batched) { CALL(0, true); } else { CALL(0, false); } \
+// gather form V (tuning knob) x batched x workgroup size (tuning knob)
+#define FSI_GATHER_SWITCH(CALL)                                                         \
+    if (d.variant >= 1) {                                                               \
+        if (d.batched) {                                                                \
+            if (d.block == 128) { CALL(1, true, 128); } else { CALL(1, true, 256); }    \
+        } else {                                                                        \
+            if (d.block == 128) { CALL(1, false, 128); } else { CALL(1, false, 256); }  \
+        }                                                                               \
+    } else {                                                                            \
+        if (d.batched) {                                                                \
+            if (d.block == 128) { CALL(0, true, 128); } else { CALL(0, true, 256); }    \
+        } else {                                                                        \
+            if (d.block == 128) { CALL(0, false, 128); } else { CALL(0, false, 256); }  \
+        }                                                                               \
     }
 
 hipError_t launch_lambda(const StepConst& c, DeviceState& d, uint32_t iter, bool taps, hipStream_t st) {
     if (tiled_form(d)) {
-        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(kBlock), 0, st, d.ndefer, 1u, 0u);
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(kBlock), 0, st, d.ndefer, kDeferWords, 0u);
+#define LM(M, T, B)                                                                                              \
+    hipLaunchKernelGGL((k_lambda_tile<M, T, B>), dim3(kTileGrid), dim3(kBlock), 0, st, c, d.tiles, d.ntiles,     \
+                       d.order, d.bin_of, d.P, d.PL, d.rho, d.lod, d.cells, iter, d.cand, d.nbr, d.defer, d.ndefer);
 #define L(T, B)                                                                                                  \
-    hipLaunchKernelGGL((k_lambda_tile<T, B>), dim3(kTileGrid), dim3(kBlock), 0, st, c, d.tiles, d.ntiles, d.order, \
-                       d.bin_of, d.P, d.PL, d.rho, d.lod, d.cells, iter, d.cand, d.nbr, d.defer, d.ndefer);     \
+    if (d.tile_shape == 2) { LM(2, T, B) } else if (d.tile_shape == 1) { LM(1, T, B) } else { LM(0, T, B) }      \
     hipLaunchKernelGGL((k_lambda_deferred<T, B>), dim3(kDeferGrid), dim3(kBlock), 0, st, c, d.defer, d.ndefer, d.P, \
                        d.PL, d.rho, d.cells, d.cand, d.nbr);
         if (taps) {
@@ -1117,6 +1173,7 @@ hipError_t launch_lambda(const StepConst& c, DeviceState& d, uint32_t iter, bool
             FSI_B_SWITCH(d.batched, L_F)
         }
 #undef L
+#undef LM
         return hipGetLastError();
     }
     if (d.variant >= 1) {
@@ -1124,12 +1181,12 @@ hipError_t launch_lambda(const StepConst& c, DeviceState& d, uint32_t iter, bool
         FSI_B_SWITCH(d.batched, L)
 #undef L
     }
-#define L(V, B)                                                                                                  \
+#define L(V, B, BS)                                                                                              \
     if (taps)                                                                                                    \
-        hipLaunchKernelGGL((k_lambda<V, true, B>), grid_for(c.n), dim3(kBlock), 0, st, c, d.order, d.P, d.PL,    \
+        hipLaunchKernelGGL((k_lambda<V, true, B, BS>), grid_bs<BS>(c.n), dim3(BS), 0, st, c, d.order, d.P, d.PL, \
                            d.rho, d.lod, d.cells, iter, d.list_count, d.list, d.cand, d.nbr);                    \
     else                                                                                                         \
-        hipLaunchKernelGGL((k_lambda<V, false, B>), grid_for(c.n), dim3(kBlock), 0, st, c, d.order, d.P, d.PL,   \
+        hipLaunchKernelGGL((k_lambda<V, false, B, BS>), grid_bs<BS>(c.n), dim3(BS), 0, st, c, d.order, d.P, d.PL, \
                            d.rho, d.lod, d.cells, iter, d.list_count, d.list, d.cand, d.nbr);
     FSI_GATHER_SWITCH(L)
 #undef L
@@ -1138,10 +1195,12 @@ hipError_t launch_lambda(const StepConst& c, DeviceState& d, uint32_t iter, bool
 
 hipError_t launch_deltap(const StepConst& c, DeviceState& d, uint32_t iter, bool taps, hipStream_t st) {
     if (tiled_form(d)) {
-        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(kBlock), 0, st, d.ndefer, 1u, 0u);
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(kBlock), 0, st, d.ndefer, kDeferWords, 0u);
+#define LM(M, T, B)                                                                                               \
+    hipLaunchKernelGGL((k_deltap_tile<M, T, B>), dim3(kTileGrid), dim3(kBlock), 0, st, c, d.tiles, d.ntiles,      \
+                       d.order, d.bin_of, d.PL, d.P, d.Q, d.lod, d.cells, iter, d.dp_tap, d.defer, d.ndefer);
 #define L(T, B)                                                                                                   \
-    hipLaunchKernelGGL((k_deltap_tile<T, B>), dim3(kTileGrid), dim3(kBlock), 0, st, c, d.tiles, d.ntiles, d.order,   \
-                       d.bin_of, d.PL, d.P, d.Q, d.lod, d.cells, iter, d.dp_tap, d.defer, d.ndefer);             \
+    if (d.tile_shape == 2) { LM(2, T, B) } else if (d.tile_shape == 1) { LM(1, T, B) } else { LM(0, T, B) }       \
     hipLaunchKernelGGL((k_deltap_deferred<T, B>), dim3(kDeferGrid), dim3(kBlock), 0, st, c, d.defer, d.ndefer, d.PL, \
                        d.P, d.Q, d.cells, d.dp_tap);
         if (taps) {
@@ -1150,14 +1209,15 @@ hipError_t launch_deltap(const StepConst& c, DeviceState& d, uint32_t iter, bool
             FSI_B_SWITCH(d.batched, L_F)
         }
 #undef L
+#undef LM
     } else {
-#define L(V, B)                                                                                                 \
+#define L(V, B, BS)                                                                                             \
     if (taps)                                                                                                   \
-        hipLaunchKernelGGL((k_deltap<V, true, B>), grid_for(c.n), dim3(kBlock), 0, st, c, d.order, d.PL, d.P,   \
+        hipLaunchKernelGGL((k_deltap<V, true, B, BS>), grid_bs<BS>(c.n), dim3(BS), 0, st, c, d.order, d.PL, d.P, \
                            d.Q, d.lod, d.cells, iter, d.list_count, d.list, d.dp_tap);                          \
     else                                                                                                        \
-        hipLaunchKernelGGL((k_deltap<V, false, B>), grid_for(c.n), dim3(kBlock), 0, st, c, d.order, d.PL, d.P,  \
-                           d.Q, d.lod, d.cells, iter, d.list_count, d.list, d.dp_tap);
+        hipLaunchKernelGGL((k_deltap<V, false, B, BS>), grid_bs<BS>(c.n), dim3(BS), 0, st, c, d.order, d.PL,     \
+                           d.P, d.Q, d.lod, d.cells, iter, d.list_count, d.list, d.dp_tap);
         FSI_GATHER_SWITCH(L)
 #undef L
     }
@@ -1176,9 +1236,9 @@ hipError_t launch_integrate(const StepConst& c, DeviceState& d, hipStream_t st) 
             FSI_B_SWITCH(d.batched, L)
 #undef L
         }
-#define L(V, B)                                                                                                  \
-    hipLaunchKernelGGL((k_integrate<V, B>), grid_for(c.n), dim3(kBlock), 0, st, c, d.order, d.P, d.vs, d.cells, \
-                       d.list_count, d.list, d.vdelta, d.curl);
+#define L(V, B, BS)                                                                                              \
+    hipLaunchKernelGGL((k_integrate<V, B, BS>), grid_bs<BS>(c.n), dim3(BS), 0, st, c, d.order, d.P, d.vs,       \
+                       d.cells, d.list_count, d.list, d.vdelta, d.curl);
         FSI_GATHER_SWITCH(L)
 #undef L
     }
@@ -1196,8 +1256,8 @@ hipError_t launch_vorticity(const StepConst& c, DeviceState& d, hipStream_t st) 
         FSI_B_SWITCH(d.batched, L)
 #undef L
     }
-#define L(V, B)                                                                                                  \
-    hipLaunchKernelGGL((k_vorticity<V, B>), grid_for(c.n), dim3(kBlock), 0, st, c, d.order, d.posS, d.curl,    \
+#define L(V, B, BS)                                                                                              \
+    hipLaunchKernelGGL((k_vorticity<V, B, BS>), grid_bs<BS>(c.n), dim3(BS), 0, st, c, d.order, d.posS, d.curl, \
                        d.perm, d.vel, d.cells, d.list_count, d.list);
     FSI_GATHER_SWITCH(L)
 #undef L

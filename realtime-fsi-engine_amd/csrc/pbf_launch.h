@@ -40,6 +40,9 @@ hipError_t radix_sort_pairs(const uint32_t* keys_in, uint32_t n, int key_bits, i
 hipError_t scan_exclusive_u32(const uint32_t* in, uint32_t* out, size_t n, uint32_t* partial, hipStream_t st);
 
 // Device buffers of one solver handle (particle-id order unless "sorted").
+constexpr uint32_t kDeferWords = 9;
+constexpr int kTileShapes = 3;  // tiles of 1, 2 or 4 Morton bins of 4x4x4 cells
+
 struct DeviceState {
     uint32_t n = 0, hash_size = 0;
     float4* pos = nullptr;     // pos.xyz, pad0
@@ -74,9 +77,11 @@ struct DeviceState {
     uint32_t* tiles = nullptr; // n + 1: processing-order position where tile i starts (then the end)
     uint32_t* ntiles = nullptr;  // 1: number of tiles
     uint32_t* defer = nullptr;   // n: slots the tiled pass left for the deferred pass
-    uint32_t* ndefer = nullptr;  // 1
+    uint32_t* ndefer = nullptr;  // kDeferWords: the deferred count, then the tile queue heads of the 8 XCDs
     SortScratch sort{};
     int variant = 1;           // gather form (tuning knob, see gather() in pbf_kernels.hip)
+    int block = 256;           // workgroup size of the gather kernels, 128 or 256 (tuning knob)
+    int tile_shape = 0;        // tiled gathers: tiles of 2^tile_shape bins, 0..2 (tuning knob)
     bool morton = true;        // processing order: Morton order of the cells (else sorted slots)
     bool tiled = false;        // every domain bins by 2^kTileShift cells (tiled gathers possible)
     bool batched = false;      // several scenes (StepConst::doms / scene_of)

@@ -24,7 +24,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libfsi_pbf.so")
 
 FSI_OK = 0
-TUNE_GATHER, TUNE_SORT_DIGIT_BITS, TUNE_ORDER = 0, 1, 2
+TUNE_GATHER, TUNE_SORT_DIGIT_BITS, TUNE_ORDER, TUNE_BLOCK, TUNE_TILE = 0, 1, 2, 3, 4
+KNOBS = {"gather": TUNE_GATHER, "sort_digit_bits": TUNE_SORT_DIGIT_BITS, "order": TUNE_ORDER,
+         "block": TUNE_BLOCK, "tile": TUNE_TILE}
 PASS_NAMES = ["predict", "sort", "tables", "lambda", "deltap", "integrate", "vorticity", "lod"]
 
 
@@ -659,10 +661,15 @@ class Solver:
         _check(self.lib.fsi_pbf_set_stream(self.h, C.c_void_p(stream or None)))
 
     def autotune(self, frames: int = 3):
-        """Time every (gather form, sort digit bits) pair; keeps the fastest. Particle state is restored."""
+        """Time every (gather form, sort digit bits, workgroup size / tile shape) setting; keeps the fastest and
+        returns (gather form, digit bits); knobs() has the rest. Particle state is restored."""
         g, d = C.c_int(0), C.c_int(0)
         _check(self.lib.fsi_pbf_autotune(self.h, frames, C.byref(g), C.byref(d)))
         return g.value, d.value
+
+    def knobs(self) -> dict:
+        """Every launch-shape knob's current value."""
+        return {name: self.get_tuning(k) for name, k in KNOBS.items()}
 
     def set_timing(self, on: bool):
         _check(self.lib.fsi_pbf_set_timing(self.h, 1 if on else 0))
@@ -810,9 +817,14 @@ class Batch:
         return v.value
 
     def autotune(self, frames: int = 3):
+        """As Solver.autotune, over every scene at once."""
         g, d = C.c_int(0), C.c_int(0)
         _check(self.lib.fsi_batch_autotune(self.h, frames, C.byref(g), C.byref(d)))
         return g.value, d.value
+
+    def knobs(self) -> dict:
+        """Every launch-shape knob's current value."""
+        return {name: self.get_tuning(k) for name, k in KNOBS.items()}
 
     def set_timing(self, on: bool):
         _check(self.lib.fsi_batch_set_timing(self.h, 1 if on else 0))

@@ -57,13 +57,18 @@ def run_single(base, sc, frames):
     return out, grid
 
 
-@pytest.mark.parametrize("frames,graph,gather", [(1, False, 1), (3, True, 1), (3, True, 2)])
-def test_batch_scenes_equal_single_scene_runs(gpu, frames, graph, gather):
+@pytest.mark.parametrize("frames,graph,gather,block,tile", [(1, False, 1, 256, 0), (3, True, 1, 256, 0),
+                                                             (3, True, 2, 256, 0), (2, False, 0, 128, 0),
+                                                             (2, True, 1, 128, 0), (2, False, 2, 256, 1),
+                                                             (3, True, 2, 256, 2)])
+def test_batch_scenes_equal_single_scene_runs(gpu, frames, graph, gather, block, tile):
     fsi = _fsi()
     base = dam_break_config(n_iter=3, substeps=2)
     scenes = make_scenes()
     b = fsi.Batch(base)
     b.set_tuning(fsi.TUNE_GATHER, gather)
+    b.set_tuning(fsi.TUNE_BLOCK, block)
+    b.set_tuning(fsi.TUNE_TILE, tile)
     b.set_graph(graph)
     b.set_scenes(scenes)
     b.step(frames)
@@ -118,7 +123,9 @@ def test_batch_autotune_restores_state_and_graph_matches(gpu):
     b.set_scenes(scenes)
     gather, bits = b.autotune(1)
     assert gather in (0, 1, 2) and bits in (6, 8, 10)
-    assert b.get_tuning(fsi.TUNE_GATHER) == gather and b.get_tuning(fsi.TUNE_SORT_DIGIT_BITS) == bits
+    k = b.knobs()
+    assert k["gather"] == gather and k["sort_digit_bits"] == bits
+    assert k["block"] in (128, 256) and k["tile"] in (0, 1, 2), k
     b.set_graph(True)
     b.set_timing(True)
     b.step(2)

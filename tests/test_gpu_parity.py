@@ -238,6 +238,13 @@ def test_error_paths(gpu):
         s.step(1)
         keys, ids, cells = s.read_grid()
         assert cells.shape == (1 << 10, 2)
+        for knob, value in ((fsi.TUNE_GATHER, 3), (fsi.TUNE_SORT_DIGIT_BITS, 11), (fsi.TUNE_ORDER, 2),
+                            (fsi.TUNE_BLOCK, 64), (fsi.TUNE_BLOCK, 512), (fsi.TUNE_TILE, 3), (fsi.TUNE_TILE, -1),
+                            (5, 0)):
+            with pytest.raises(fsi.FsiError) as e:
+                s.set_tuning(knob, value)
+            assert e.value.code == -1, (knob, value)
+        assert s.knobs() == {"gather": 1, "sort_digit_bits": 10, "order": 1, "block": 256, "tile": 0}
 
 
 def test_full_size_properties(gpu):
@@ -270,9 +277,9 @@ def test_full_size_properties(gpu):
 
 @pytest.mark.parametrize("graph", [False, True])
 def test_knobs_and_graph_do_not_change_results(gpu, graph):
-    """Every gather form, radix digit width and processing order, direct or
-    replayed from a HIP graph, gives the same bits (the oracle fixes the
-    summation order)."""
+    """Every gather form, radix digit width, processing order, workgroup size
+    and tile shape, direct or replayed from a HIP graph, gives the same bits
+    (the oracle fixes the summation order)."""
     fsi = _fsi()
     cfg = dam_break_config(n_iter=3, substeps=2, hash_size=1 << 14)
     parts = fsi.lattice(24, 20, 16, (0.0, 0.0, 0.0), 0.05, 0.001, 99)
@@ -281,23 +288,47 @@ def test_knobs_and_graph_do_not_change_results(gpu, graph):
     ref = _oracle().OracleSim(cfg, parts, taps=False, sdf=sdf, sdf_padding=0.025)
     ref.step(2)
     exp = ref.particles()
-    knobs = [(g, b, 1) for g in (0, 1, 2) for b in (6, 8, 10)] + [(g, 8, 0) for g in (0, 1, 2)]
-    for gather, bits, order in knobs:
+    knobs = ([(g, b, 1, 256, 0) for g in (0, 1, 2) for b in (6, 8, 10)] + [(g, 8, 0, 256, 0) for g in (0, 1, 2)]
+             + [(g, 8, 1, 128, 0) for g in (0, 1)] + [(2, b, 1, 256, t) for b in (6, 10) for t in (1, 2)])
+    for gather, bits, order, block, tile in knobs:
         with fsi.Solver(cfg) as s:
             s.set_particles(parts)
             s.set_sdf(sdf, 0.025, False)
             s.set_tuning(fsi.TUNE_GATHER, gather)
             s.set_tuning(fsi.TUNE_SORT_DIGIT_BITS, bits)
             s.set_tuning(fsi.TUNE_ORDER, order)
-            assert s.get_tuning(fsi.TUNE_ORDER) == order
+            s.set_tuning(fsi.TUNE_BLOCK, block)
+            s.set_tuning(fsi.TUNE_TILE, tile)
+            assert s.knobs() == {"gather": gather, "sort_digit_bits": bits, "order": order, "block": block,
+                                 "tile": tile}
             s.set_graph(graph)
             s.step(2)
             got = s.read_particles()
             keys, ids, cells = s.read_grid()
-        case = (gather, bits, order)
+        case = (gather, bits, order, block, tile)
         assert np.array_equal(keys, ref.keys) and np.array_equal(ids, ref.ids), case
         assert np.array_equal(cells, ref.cells), case
         assert np.abs(got[:, 0:3] - exp[:, 0:3]).max() <= 1e-5 * cfg.h, case
+
+
+def test_autotune_restores_state(gpu):
+    """The autotuner times every knob setting from the same state and puts it
+    back: the frames after it equal frames run without it."""
+    fsi = _fsi()
+    cfg = dam_break_config(n_iter=3, substeps=2, hash_size=1 << 14)
+    parts = fsi.lattice(24, 20, 16, (0.0, 0.0, 0.0), 0.05, 0.001, 5)
+    with fsi.Solver(cfg) as s:
+        s.set_particles(parts)
+        s.step(2)
+        want = s.read_particles()
+    with fsi.Solver(cfg) as s:
+        s.set_particles(parts)
+        gather, bits = s.autotune(1)
+        k = s.knobs()
+        assert (k["gather"], k["sort_digit_bits"]) == (gather, bits)
+        assert gather in (0, 1, 2) and bits in (6, 8, 10) and k["block"] in (128, 256) and k["tile"] in (0, 1, 2)
+        s.step(2)
+        assert np.array_equal(s.read_particles(), want), k
 
 
 def test_density_error_report_against_fp64(gpu):
