@@ -453,6 +453,12 @@ int fsi_mesh_refit(fsi_mesh* m, const double* verts);
  * coordinates (min > max when none). */
 int fsi_mesh_refit_dirty(fsi_mesh* m, const double* verts, double band_radius, double move_eps,
                          int32_t dirty_min[3], int32_t dirty_max[3], uint32_t* dirty_voxels);
+/* SDFBoundary::closestTriangle (src/fluid/SDFBoundary.cpp:600-610) of n
+ * points (3 floats each, host): per point the index of the nearest triangle
+ * in the face list the mesh was created with (-1 for none) and the closest
+ * point on it (3 floats), from the BVH of the mesh's current vertices, in
+ * FP64 as the reference's query. */
+int fsi_mesh_closest(fsi_mesh* m, const float* points, uint32_t n, int32_t* faces, float* closest);
 /* The per-vertex forces of the last fsi_pbf_contact_forces (3 floats each). */
 int fsi_mesh_read_forces(fsi_mesh* m, float* forces);
 

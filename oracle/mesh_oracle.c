@@ -9,6 +9,7 @@
  *     refit keeps (updateGeometryAndRefit :438-452: face normals refreshed,
  *     edge and vertex normals from the build);
  *   - the dirty-voxel marking of refitDirty (:480-525);
+ *   - closestTriangle (:600-610);
  *   - computeContactForces (src/main.cpp:604-710) with barycentric
  *     (:557-571), per-vertex sums in candidate order.
  * Only tests/ and bench.py's CPU baseline use it.  Among equidistant
@@ -196,6 +197,21 @@ static hit_t brute_closest(const double* verts, const int32_t* faces, uint32_t n
         }
     }
     return best;
+}
+
+/* SDFBoundary::closestTriangle (src/fluid/SDFBoundary.cpp:600-610): the
+ * nearest triangle of each float point (promoted to double), by brute force;
+ * the closest point is rounded to float as the reference returns it. */
+void oracle_closest_triangles(const double* verts, const int32_t* faces, uint32_t nf, const float* pts, uint32_t n,
+                              int32_t* face_out, float* point_out, double* d2_out) {
+    for (uint32_t i = 0; i < n; ++i) {
+        const hit_t h = brute_closest(verts, faces, nf, mk(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]));
+        face_out[i] = (int32_t)h.tri;
+        point_out[3 * i] = (float)h.p.x;
+        point_out[3 * i + 1] = (float)h.p.y;
+        point_out[3 * i + 2] = (float)h.p.z;
+        d2_out[i] = h.d2;
+    }
 }
 
 void oracle_mesh_sdf(const double* verts, const double* verts_build, uint32_t nv, const int32_t* faces, uint32_t nf,

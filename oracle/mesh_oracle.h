@@ -32,6 +32,11 @@ uint32_t oracle_dirty_mask(const double* old_verts, const double* new_verts, uin
                            double move_eps, const double origin[3], double cell, const int32_t dims[3],
                            uint8_t* mask, int32_t box[6]);
 
+/* Nearest triangle of each point (3 floats): its index (lowest among
+ * equidistant ones), the closest point as float and its squared distance. */
+void oracle_closest_triangles(const double* verts, const int32_t* faces, uint32_t nf, const float* pts, uint32_t n,
+                              int32_t* face_out, float* point_out, double* d2_out);
+
 /* Per-vertex forces (3 floats each) of the candidates (8 floats each:
  * pos xyzw, vel xyzw) in candidate order; counts = contacts, control points. */
 void oracle_contact_forces(const double* verts, uint32_t nv, const int32_t* faces, uint32_t nf, const float* cand,

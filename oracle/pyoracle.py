@@ -115,6 +115,8 @@ def load() -> C.CDLL:
     lib.oracle_dirty_mask.restype = C.c_uint32
     lib.oracle_contact_forces.argtypes = [d3, C.c_uint32, i32p, C.c_uint32, f32p, C.c_uint32, C.c_float, C.c_int,
                                           C.c_float, f32p, u32p]
+    lib.oracle_closest_triangles.argtypes = [d3, i32p, C.c_uint32, f32p, C.c_uint32, i32p, f32p,
+                                             C.POINTER(C.c_double)]
     _lib = lib
     return lib
 
@@ -150,6 +152,20 @@ def dirty_mask(old_verts, new_verts, band, move_eps, origin, cell, dims):
                                  _p(o, C.c_double), float(cell), _p(d, C.c_int32), _p(mask, C.c_uint8),
                                  _p(box, C.c_int32))
     return int(n), mask, box
+
+
+def closest_triangles(verts, faces, points):
+    """(face (n,) int32, closest point (n, 3) float32, squared distance (n,)
+    float64) of every float point, by brute force over the triangles."""
+    v = np.ascontiguousarray(verts, dtype=np.float64).reshape(-1, 3)
+    f = np.ascontiguousarray(faces, dtype=np.int32).reshape(-1, 3)
+    p = np.ascontiguousarray(points, dtype=np.float32).reshape(-1, 3)
+    face = np.zeros(len(p), dtype=np.int32)
+    out = np.zeros((len(p), 3), dtype=np.float32)
+    d2 = np.zeros(len(p), dtype=np.float64)
+    load().oracle_closest_triangles(_p(v, C.c_double), _p(f, C.c_int32), len(f), _p(p, C.c_float), len(p),
+                                    _p(face, C.c_int32), _p(out, C.c_float), _p(d2, C.c_double))
+    return face, out, d2
 
 
 def contact_forces(verts, faces, cand, band, use_vel=True, threshold=0.0):

@@ -503,6 +503,21 @@ __global__ __launch_bounds__(kMB) void k_vertex_sum(const int2* __restrict__ seg
     count_wave(&counters[1], !(mg < 1e-5f) && mg >= threshold);
 }
 
+// SDFBoundary::closestTriangle (src/fluid/SDFBoundary.cpp:600-610) of every
+// query point: the BVH walk in FP64 from the float point, the triangle's index
+// in the build's face list and the closest point rounded to float.
+__global__ __launch_bounds__(kMB) void k_closest(MeshView m, const float* __restrict__ pts, uint32_t n,
+                                                 int32_t* __restrict__ face, float* __restrict__ out) {
+    const uint32_t i = mgid();
+    if (i >= n) return;
+    const Q3 p{(double)pts[3 * (size_t)i], (double)pts[3 * (size_t)i + 1], (double)pts[3 * (size_t)i + 2]};
+    const DHit hit = bvh_closest(m.nodes, m.tris, p);
+    face[i] = hit.tri;
+    out[3 * (size_t)i] = (float)hit.p.x;
+    out[3 * (size_t)i + 1] = (float)hit.p.y;
+    out[3 * (size_t)i + 2] = (float)hit.p.z;
+}
+
 }  // namespace
 
 struct fsi_mesh {
@@ -537,6 +552,11 @@ struct fsi_mesh {
     int2* seg = nullptr;
     float4* forces = nullptr;
     uint32_t* counters = nullptr;
+    // closest-triangle queries, sized for `qcap` points
+    uint32_t qcap = 0;
+    float* qpts = nullptr;
+    int32_t* qface = nullptr;
+    float* qout = nullptr;
 };
 
 namespace {
@@ -586,6 +606,10 @@ void free_mesh(fsi_mesh* m) {
     dev_free(&m->seg);
     dev_free(&m->forces);
     dev_free(&m->counters);
+    dev_free(&m->qpts);
+    dev_free(&m->qface);
+    dev_free(&m->qout);
+    m->qcap = 0;
 }
 
 int bind(fsi_mesh* m) {
@@ -979,6 +1003,33 @@ int fsi_mesh_refit(fsi_mesh* m, const double* verts) {
                                m->stream));
     FSI_HIP_TRY(record_ready(m));
     FSI_HIP_TRY(hipStreamSynchronize(m->stream));  // verts is the caller's memory
+    return FSI_OK;
+}
+
+int fsi_mesh_closest(fsi_mesh* m, const float* points, uint32_t n, int32_t* faces, float* closest) {
+    int rc = bind(m);
+    if (rc) return rc;
+    if (n == 0) return FSI_OK;
+    if (!points || !faces || !closest) return fail(FSI_ERR_INVALID_ARG, "NULL argument");
+    if (n > (1u << 28)) return fail(FSI_ERR_INVALID_ARG, "more than 2^28 query points");
+    hipStream_t st = m->stream;
+    if (n > m->qcap) {
+        FSI_HIP_TRY(hipStreamSynchronize(st));
+        dev_free(&m->qpts);
+        dev_free(&m->qface);
+        dev_free(&m->qout);
+        m->qcap = 0;
+        FSI_HIP_TRY(dev_alloc(&m->qpts, 3 * (size_t)n));
+        FSI_HIP_TRY(dev_alloc(&m->qface, (size_t)n));
+        FSI_HIP_TRY(dev_alloc(&m->qout, 3 * (size_t)n));
+        m->qcap = n;
+    }
+    FSI_HIP_TRY(hipMemcpyAsync(m->qpts, points, sizeof(float) * 3 * (size_t)n, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_closest, mgrid(n), dim3(kMB), 0, st, view(m), m->qpts, n, m->qface, m->qout);
+    FSI_HIP_TRY(hipGetLastError());
+    FSI_HIP_TRY(hipMemcpyAsync(faces, m->qface, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost, st));
+    FSI_HIP_TRY(hipMemcpyAsync(closest, m->qout, sizeof(float) * 3 * (size_t)n, hipMemcpyDeviceToHost, st));
+    FSI_HIP_TRY(hipStreamSynchronize(st));
     return FSI_OK;
 }
 

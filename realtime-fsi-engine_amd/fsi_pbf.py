@@ -234,6 +234,7 @@ def load() -> C.CDLL:
         "fsi_mesh_refit": ([P, f64p], C.c_int),
         "fsi_mesh_refit_dirty": ([P, f64p, C.c_double, C.c_double, i32p, i32p, u32p], C.c_int),
         "fsi_mesh_read_forces": ([P, f32p], C.c_int),
+        "fsi_mesh_closest": ([P, f32p, C.c_uint32, i32p, f32p], C.c_int),
         "fsi_pbf_set_sdf_from_mesh": ([P, P], C.c_int),
         "fsi_pbf_contact_forces": ([P, P, C.c_float, C.c_int, C.c_float, f32p, u32p], C.c_int),
         "fsi_mesh_contact_forces_noalloc": ([P, P, C.c_uint32, C.c_float, C.c_int, C.c_float, P, P, P], C.c_int),
@@ -483,6 +484,15 @@ class Mesh:
         out = np.zeros((len(self.verts), 3), dtype=np.float32)
         _check(self.lib.fsi_mesh_read_forces(self.h, _ptr(out, C.c_float)))
         return out
+
+    def closest(self, points: np.ndarray):
+        """SDFBoundary::closestTriangle of every point: (face index per point, closest points (n, 3) float32)."""
+        p = np.ascontiguousarray(points, dtype=np.float32).reshape(-1, 3)
+        faces = np.zeros(len(p), dtype=np.int32)
+        out = np.zeros((len(p), 3), dtype=np.float32)
+        _check(self.lib.fsi_mesh_closest(self.h, _ptr(p, C.c_float), len(p), _ptr(faces, C.c_int32),
+                                         _ptr(out, C.c_float)))
+        return faces, out
 
     def close(self):
         if self.h:

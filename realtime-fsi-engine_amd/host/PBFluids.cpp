@@ -131,12 +131,11 @@ void PBFluids::setInteraction(bool active, const PVec3& pos) {
 
 void PBFluids::syncSDF() {
     if (_sdf == _uploadedSdf && (!_sdf || _sdf->version() == _uploadedVersion)) return;
-    if (!_sdf || !_sdf->valid()) {
+    if (!_sdf || !_sdf->valid() || !_sdf->deviceMesh()) {
         ok(fsi_pbf_set_sdf(_h, nullptr, nullptr, nullptr, 0.0f), "setSDFBoundary");
     } else {
-        const int32_t dims[3] = {_sdf->dimensions().x(), _sdf->dimensions().y(), _sdf->dimensions().z()};
-        const float o[3] = {(float)_sdf->origin().x(), (float)_sdf->origin().y(), (float)_sdf->origin().z()};
-        ok(fsi_pbf_set_sdf(_h, _sdf->textureOrder().data(), dims, o, (float)_sdf->cellSize()), "setSDFBoundary");
+        // the boundary's device grid, copied device to device
+        ok(fsi_pbf_set_sdf_from_mesh(_h, _sdf->deviceMesh()), "setSDFBoundary");
     }
     _uploadedSdf = _sdf;
     _uploadedVersion = _sdf ? _sdf->version() : 0;

@@ -11,27 +11,10 @@
 #include <vector>
 
 #include "SDFBoundary.h"
+#include "Types.h"
 
 struct fsi_pbf;
 
-using F32 = float;
-using I32 = std::int32_t;
-using U32 = std::uint32_t;
-
-struct PVec3 {
-    F32 x, y, z;
-};
-struct PVec4 {
-    F32 x, y, z, w;
-};
-struct UVec2 {
-    U32 x, y;
-};
-struct IVec2 {
-    I32 x, y;
-};
-
-inline PVec3 make_pvec3(F32 x, F32 y, F32 z) { return {x, y, z}; }
 
 // 48 bytes, std430-padded (include/fluid/Particle.h:8-12).
 struct Particle {

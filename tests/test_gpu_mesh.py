@@ -44,6 +44,41 @@ def test_device_grid_is_the_host_build(gpu, which):
     assert np.array_equal(dev.grid, host.grid)
 
 
+@pytest.mark.parametrize("which", ["icosphere", "plate"])
+def test_closest_triangle_matches_the_oracle(gpu, which):
+    """SDFBoundary::closestTriangle on the device (BVH walk) against the
+    brute-force oracle, on the built and on a refit (bent) mesh: the closest
+    points agree to float rounding, the distances to 1e-6, and the faces
+    wherever no other triangle is as close."""
+    import fsi_pbf
+
+    (verts, faces), cell = [(m, c) for n, m, c in meshes() if n == which][0]
+    rng = np.random.default_rng(17)
+    lo, hi = verts.min(0) - 0.5, verts.max(0) + 0.5
+    pts = rng.uniform(lo, hi, size=(5000, 3)).astype(np.float32)
+    with fsi_pbf.Mesh(verts, faces, cell) as m:
+        for stage in ("built", "refit"):
+            cur = verts
+            if stage == "refit":
+                cur = verts.copy()
+                cur[:, 2] += 0.1 * np.sin(np.pi * cur[:, 0]) * cur[:, 1]
+                m.refit(cur)
+            face, got = m.closest(pts)
+            of, op, od2 = pyoracle.closest_triangles(cur, faces, pts)
+            d2 = np.sum((pts.astype(np.float64) - got.astype(np.float64)) ** 2, axis=1)
+            print(f"{which} {stage}: max |p - p_oracle| = {np.abs(got - op).max():.2e}, "
+                  f"faces equal {np.mean(face == of) * 100:.2f}%", flush=True)
+            assert face.min() >= 0 and face.max() < len(faces)
+            assert np.abs(got - op).max() <= 2e-6 * max(1.0, float(np.abs(op).max()))
+            assert np.abs(np.sqrt(d2) - np.sqrt(od2)).max() <= 1e-6
+            # a different face only where it is as close as the oracle's (a shared edge or vertex)
+            for i in np.nonzero(face != of)[0]:
+                _, _, d2f = pyoracle.closest_triangles(cur, faces[face[i]][None], pts[i][None])
+                assert abs(np.sqrt(d2f[0]) - np.sqrt(od2[i])) <= 1e-12, (i, face[i], of[i])
+        face, got = m.closest(np.zeros((0, 3), dtype=np.float32))
+        assert len(face) == 0
+
+
 def test_refit_matches_the_oracle_with_build_normals(gpu):
     import fsi_pbf
 

@@ -118,3 +118,47 @@ def test_contact_forces_closed_form_on_a_plane(use_vel):
     # the split conserves the push: the weights of a point sum to one
     assert abs(forces.sum(axis=0) - expect.sum(axis=0)).max() < 1e-5
     assert controls == int((np.linalg.norm(expect, axis=1) >= 0.5).sum())
+
+
+@pytest.mark.parametrize("p, q", [
+    ((0.25, 0.25, 0.5), (0.25, 0.25, 0.0)),         # face region: the foot of the perpendicular
+    ((-1.0, -1.0, 0.0), (0.0, 0.0, 0.0)),           # vertex A region
+    ((2.0, -0.5, 0.25), (1.0, 0.0, 0.0)),           # vertex B region
+    ((0.5, -1.0, 0.0), (0.5, 0.0, 0.0)),            # edge AB region
+    ((1.0, 1.0, 0.3), (0.5, 0.5, 0.0)),             # edge BC region
+    ((-0.5, 0.5, -2.0), (0.0, 0.5, 0.0)),           # edge CA region
+])
+def test_closest_triangle_feature_regions(p, q):
+    """closestTriangle on one triangle: the closest point of each Voronoi
+    region (Ericson 5.1.5) in closed form, its squared distance, face 0."""
+    face, pt, d2 = pyoracle.closest_triangles(TRI, TRI_F, np.array([p], dtype=np.float32))
+    assert face[0] == 0
+    assert np.allclose(pt[0], q, atol=1e-7)
+    exp = float(np.sum((np.asarray(p, dtype=np.float32).astype(np.float64) - np.asarray(q)) ** 2))
+    assert abs(d2[0] - exp) <= 1e-12 * max(1.0, exp)
+
+
+def test_closest_triangle_of_a_cube_is_the_nearest_face():
+    """Points inside and outside a unit cube (12 triangles): the closest point
+    is the projection on the nearest face plane (clamped to the square), and
+    the reported face lies in that plane."""
+    v = np.array([[x, y, z] for z in (0.0, 1.0) for y in (0.0, 1.0) for x in (0.0, 1.0)])
+    quads = [(0, 2, 3, 1), (4, 5, 7, 6), (0, 1, 5, 4), (2, 6, 7, 3), (0, 4, 6, 2), (1, 3, 7, 5)]
+    f = np.array([t for a, b, c, d in quads for t in ((a, b, c), (a, c, d))], dtype=np.int32)
+    rng = np.random.default_rng(4)
+    pts = rng.uniform(-0.5, 1.5, size=(400, 3)).astype(np.float32)
+    face, got, d2 = pyoracle.closest_triangles(v, f, pts)
+    p64 = pts.astype(np.float64)
+    inside = np.all((p64 > 0) & (p64 < 1), axis=1)
+    exp = np.clip(p64, 0.0, 1.0)
+    # inside: move the nearest coordinate onto its face
+    dist = np.concatenate([p64, 1.0 - p64], axis=1)
+    k = np.argmin(dist, axis=1)
+    for i in np.nonzero(inside)[0]:
+        exp[i, k[i] % 3] = 0.0 if k[i] < 3 else 1.0
+    assert np.allclose(got, exp, atol=2e-7)
+    assert np.allclose(d2, np.sum((p64 - exp) ** 2, axis=1), atol=1e-12)
+    for i in range(len(pts)):
+        tri = v[f[face[i]]]
+        normal = np.cross(tri[1] - tri[0], tri[2] - tri[0])
+        assert abs(np.dot(normal, exp[i] - tri[0])) <= 1e-9
