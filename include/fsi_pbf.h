@@ -558,7 +558,9 @@ int fsi_adj_frame_norms(fsi_adjoint* a, float* out);
 /* Host mesh deformation of the two-way coupling: LaplacianDeformation        */
 /* (src/deformation/LaplacianDeformation.cpp, include/deformation/            */
 /* LaplacianDeformation.h) and its per-frame driver deformStep                */
-/* (src/main.cpp:862-944).  Host code, FP64, with a project-written sparse    */
+/* (src/main.cpp:862-944).  Host code, FP64 arithmetic on vertex positions  */
+/* kept as floats (the reference's mesh holds Vec3f points), with a          */
+/* project-written sparse                                                    */
 /* LDL^T (reverse Cuthill-McKee order) where the reference uses Eigen's        */
 /* SimplicialLDLT, CHOLMOD and its IncrementalCholesky.                       */
 /* ------------------------------------------------------------------------- */
@@ -596,7 +598,9 @@ int fsi_deform_set_control_points(fsi_deform* d, const int32_t* idx, const doubl
  * the current constrained set, then the positions for the current targets. */
 int fsi_deform_precompute(fsi_deform* d);
 int fsi_deform_solve(fsi_deform* d);
-/* The current vertices (3 doubles each). */
+/* The current vertices (3 doubles each, every one a float value: the
+ * rest pose and each solve's result are rounded to float, as the
+ * reference's OpenMesh points are, LaplacianDeformation.cpp:388-389,568,617). */
 int fsi_deform_read_vertices(fsi_deform* d, double* verts);
 /* deformStep (src/main.cpp:862-944) given the per-vertex contact forces
  * (3 floats each, as fsi_pbf_contact_forces returns them) and the frame dt:

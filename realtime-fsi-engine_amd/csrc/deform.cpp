@@ -343,8 +343,19 @@ void Ldlt::solve(double* b) const {
 
 // ---------------------------------------------------------------- Laplacian
 
+// The reference's mesh stores its points in single precision (OpenMesh's
+// default Vec3f points, read at LaplacianDeformation.cpp:179,350 and written
+// at :388-389,568,617): every vertex position the deformation reads or
+// writes is a float, promoted to double for the arithmetic.
+static inline double mesh_point(double x) { return (double)(float)x; }
+
+static std::vector<double> mesh_points(std::vector<double> v) {
+    for (double& x : v) x = mesh_point(x);
+    return v;
+}
+
 Laplacian::Laplacian(std::vector<double> verts, const std::vector<int32_t>& faces)
-    : v_(std::move(verts)), rest_(v_) {
+    : v_(mesh_points(std::move(verts))), rest_(v_) {
     const int nv = n();
     nbr_.assign((size_t)nv, {});
     for (size_t f = 0; f + 2 < faces.size(); f += 3)
@@ -559,7 +570,8 @@ bool Laplacian::solve_local() {
     }
     for (int a = 0; a < 3; ++a) {
         L.ldlt.solve(rhs[a].data());
-        for (int li = 0; li < ni; ++li) v_[3 * (size_t)L.interior[(size_t)li] + (size_t)a] = rhs[a][(size_t)li];
+        for (int li = 0; li < ni; ++li)
+            v_[3 * (size_t)L.interior[(size_t)li] + (size_t)a] = mesh_point(rhs[a][(size_t)li]);
     }
     return true;
 }
@@ -664,7 +676,7 @@ bool Laplacian::solve() {
             for (int k = 0; k < nv; ++k)
                 if (mask_[(size_t)k]) rhs[(size_t)k] = tg[3 * (size_t)k + (size_t)a];
             inc_.solve(rhs.data());
-            for (int k = 0; k < nv; ++k) v_[3 * (size_t)k + (size_t)a] = rhs[(size_t)k];
+            for (int k = 0; k < nv; ++k) v_[3 * (size_t)k + (size_t)a] = mesh_point(rhs[(size_t)k]);
         }
         return true;
     }
@@ -678,7 +690,7 @@ bool Laplacian::solve() {
         for (size_t i = 0; i < ctrl_.size(); ++i) b[(size_t)row++] = ctrl_t_[3 * i + (size_t)a];
         gemv(At_, b.data(), rhs.data());
         glob_.solve(rhs.data());
-        for (int k = 0; k < nv; ++k) v_[3 * (size_t)k + (size_t)a] = rhs[(size_t)k];
+        for (int k = 0; k < nv; ++k) v_[3 * (size_t)k + (size_t)a] = mesh_point(rhs[(size_t)k]);
     }
     return true;
 }

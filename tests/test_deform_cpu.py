@@ -6,18 +6,36 @@ The restatement builds the cotangent Laplacian densely from its definition,
 pinned here by closed forms on a square-grid plate (an axis edge between two
 45-degree corners weighs 1, a diagonal opposite two right angles takes the 1e-4
 floor), and solves with numpy's dense LU or least squares instead of the
-library's sparse LDL^T."""
+library's sparse LDL^T.  The reference's mesh keeps its points as floats, so
+the library rounds the rest pose and every solved position to float: the
+plates here have float vertices, and a solve is compared with the dense
+solution rounded to float, to one float ulp (the two FP64 solutions differ
+far below it, but may straddle a rounding boundary)."""
 import numpy as np
 import pytest
 
 import fsi_pbf
 
 
+def as_mesh(x):
+    """Positions as the reference's mesh stores them: rounded to float."""
+    return np.asarray(x, dtype=np.float64).astype(np.float32).astype(np.float64)
+
+
+def assert_mesh_close(got, want, extra=0.0):
+    """got holds floats and is the FP64 solution want rounded to float, to one
+    float ulp (plus `extra` for a restatement that differs more)."""
+    assert np.array_equal(got, as_mesh(got)), "vertices must be float values"
+    ulp = np.spacing(np.abs(want).astype(np.float32)).astype(np.float64)
+    err = np.abs(got - want) - ulp - extra
+    assert err.max() <= 1e-13, float(err.max())
+
+
 def plate(nx=12, ny=6, size=(2.4, 1.2)):
     xs = np.linspace(0.0, size[0], nx + 1)
     ys = np.linspace(0.0, size[1], ny + 1)
     X, Y = np.meshgrid(xs, ys, indexing="xy")
-    verts = np.stack([X.ravel(), Y.ravel(), np.zeros(X.size)], axis=1)
+    verts = as_mesh(np.stack([X.ravel(), Y.ravel(), np.zeros(X.size)], axis=1))
     faces = []
     for j in range(ny):
         for i in range(nx):
@@ -132,7 +150,7 @@ def test_local_patch_matches_the_dense_restatement(k):
         patch = d.info()["patch"]
     want = local_solve(verts, verts, W, set(anchors), handles, targets, k)
     assert patch == len([v for v in kring(W, handles, k) if v not in anchors])
-    assert np.abs(got - want).max() < 1e-10
+    assert_mesh_close(got, want)
     assert np.array_equal(got[[v for v in range(len(verts)) if v not in kring(W, handles, k)]],
                           verts[[v for v in range(len(verts)) if v not in kring(W, handles, k)]])
 
@@ -144,7 +162,8 @@ def test_handles_at_rest_keep_the_mesh():
         d.set_control_points([40, 41], verts[[40, 41]])
         d.precompute()
         d.solve()
-        assert np.abs(d.vertices() - verts).max() < 1e-12
+        got = d.vertices()
+        assert np.array_equal(got, as_mesh(got)) and np.abs(got - verts).max() < 1e-12
 
 
 def test_global_soft_path_is_least_squares():
@@ -168,7 +187,7 @@ def test_global_soft_path_is_least_squares():
     A = np.vstack([L, C])
     want = np.stack([np.linalg.lstsq(A, np.concatenate([L @ verts[:, a], verts[anchors, a], targets[:, a]]), rcond=None)[0]
                      for a in range(3)], axis=1)
-    assert np.abs(got - want).max() < 1e-9
+    assert_mesh_close(got, want, 1e-9)
 
 
 def hard_solve(verts, W, constrained, targets_by_vertex):
@@ -212,8 +231,8 @@ def test_incremental_path_hard_constraints_and_updates():
             tv = {v: verts[v] for v in anchors}
             tv.update({h: t for h, t in zip(handles, targets)})
             want = hard_solve(verts, W, set(anchors) | set(handles), tv)
-            assert np.abs(got - want).max() < 1e-8
-            assert np.abs(got[handles] - targets).max() < 1e-12  # hard: the targets are hit
+            assert_mesh_close(got, want, 1e-8)
+            assert np.array_equal(got[handles], as_mesh(targets))  # hard: the targets are hit, as floats
             assert d.verify_incremental() < 1e-9
         assert steps[0] == len(verts)
     # a local change of the constrained set recomputes only the tree paths
@@ -287,7 +306,34 @@ def test_deform_step_follows_the_reference_rule():
             if hs or hs != last:
                 assert solved
                 if hs:
-                    cur = local_solve(cur, verts, W, anchors, hs, ts)
+                    want = local_solve(cur, verts, W, anchors, hs, ts)
+                    assert_mesh_close(d.vertices(), want, 1e-6 * frame)
+                    cur = as_mesh(want)
             last = hs
-            assert np.abs(d.vertices() - cur).max() < 1e-9
+            assert np.abs(d.vertices() - cur).max() < 1e-6
         assert d.info()["factorizations"] >= 2
+
+
+def test_vertices_are_the_reference_mesh_floats():
+    """LaplacianDeformation writes solved positions into an OpenMesh mesh of
+    float points (LaplacianDeformation.cpp:388-389), and its rest pose comes
+    from that mesh: vertices given in FP64 come back rounded to float, and a
+    solve moves every patch vertex to the float nearest its FP64 solution."""
+    xs = np.linspace(0.0, 1.0, 11)  # 0.1 steps: not floats
+    X, Y = np.meshgrid(xs, xs[:6], indexing="xy")
+    verts = np.stack([X.ravel(), Y.ravel(), 0.1 * X.ravel()], axis=1)
+    faces = plate(10, 5, (1.0, 0.5))[1]
+    assert not np.array_equal(verts, as_mesh(verts))
+    with fsi_pbf.Deform(verts, faces, kring=2) as d:
+        assert np.array_equal(d.vertices(), as_mesh(verts))
+        d.add_anchor(0)
+        targets = as_mesh(verts)[[27, 28]] + [[0.0, 0.0, 0.013], [0.0, 0.0, -0.007]]
+        d.set_control_points([27, 28], targets)
+        d.precompute()
+        d.solve()
+        got = d.vertices()
+    mv = as_mesh(verts)
+    want = local_solve(mv, mv, cot_weights(mv, faces), {0}, [27, 28], targets, 2)
+    moved = np.any(got != mv, axis=1)
+    assert moved.sum() > 2
+    assert_mesh_close(got, want)

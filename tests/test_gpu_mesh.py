@@ -225,9 +225,10 @@ def test_two_way_coupling_frames(gpu):
         cur = d.vertices()
         grid = m.read_sdf()
     assert moved > 0
-    assert np.array_equal(cur[anchors], verts[anchors])
+    # anchors stay where the deformation's float mesh put them
+    assert np.array_equal(cur[anchors], verts[anchors].astype(np.float32).astype(np.float64))
     disp = cur - verts
     assert disp[:, 1].min() < -1e-3  # pushed down where the fluid lands
-    assert np.abs(disp).max() <= 0.4 + 1e-9  # the total-displacement clamp
+    assert np.abs(disp).max() <= 0.4 + 1e-6  # the total-displacement clamp (float vertices)
     ref = pyoracle.mesh_sdf(cur, faces, grid.origin, grid.cell, grid.dims, verts_build=verts)
     assert np.abs(grid.grid - ref).max() <= 1e-6

@@ -137,7 +137,8 @@ def test_coupling_at_full_size_refit_every_frame(gpu):
             assert np.array_equal(c_got, c_exp), f"frame {f}: contact candidates"
         cur = d.vertices()
     assert contacts > 0 and solves > 0
-    assert np.array_equal(cur[anchors], verts[anchors])
+    # anchors stay where the deformation's float mesh put them
+    assert np.array_equal(cur[anchors], verts[anchors].astype(np.float32).astype(np.float64))
     assert np.abs(cur - verts).max() > 0.0
     brute = pyoracle.mesh_sdf(cur, faces, grid.origin, grid.cell, grid.dims, verts_build=verts)
     assert np.abs(grid.grid - brute).max() <= 1e-6
