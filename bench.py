@@ -512,8 +512,9 @@ def main():
     ap.add_argument("--order", type=int, default=None,
                     help="pin the processing order of the 1M solver (0 sorted slots, 1 Morton; default 1)")
     ap.add_argument("--calibrate-copy", action="store_true",
-                    help="only run a plain device copy of 256 MiB (3 launches), the reference point that scales "
-                         "the FETCH_SIZE / WRITE_SIZE counters to bytes")
+                    help="only run a plain device copy of --calibrate-mib MiB (3 launches), the reference point "
+                         "that scales the FETCH_SIZE / WRITE_SIZE counters to bytes")
+    ap.add_argument("--calibrate-mib", type=int, default=256)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the PBF substep of this build runs on a single MI355X "
@@ -523,7 +524,7 @@ def main():
     if args.calibrate_copy:
         import torch
 
-        n = 64 << 20
+        n = args.calibrate_mib << 18  # floats
         a = torch.full((n,), 1.5, device="cuda:0", dtype=torch.float32)
         b = torch.empty_like(a)
         for _ in range(3):

@@ -285,8 +285,10 @@ enum fsi_pbf_knob {
     FSI_TUNE_SORT_DIGIT_BITS = 1, /* widest radix digit, 1..10                                */
     FSI_TUNE_ORDER = 2,           /* thread -> particle order of the gather passes: 0 sorted  */
                                   /* slots as the sort leaves them, 1 Morton order of the      */
-                                  /* predicted cells in per-XCD ranges (default; form 2 needs  */
-                                  /* it and runs as form 1 under 0)                            */
+                                  /* predicted cells in per-XCD ranges over sorted-slot        */
+                                  /* storage, 2 the same with the particle arrays stored in    */
+                                  /* that order, whole buckets in Morton order of their cells  */
+                                  /* (default); form 2 needs 1 or 2 and runs as form 1 under 0 */
     FSI_TUNE_BLOCK = 3,           /* workgroup size of the form-0/1 gathers: 128 or 256        */
                                   /* (default; the reference's local_size_x)                  */
     FSI_TUNE_TILE = 4,            /* cell-tile shape of form 2: 0 4x4x4 cells (default),      */
@@ -297,9 +299,10 @@ int fsi_pbf_set_tuning(fsi_pbf* h, int knob, int value);
 int fsi_pbf_get_tuning(fsi_pbf* h, int knob, int* value);
 
 /* Autotuner: times `frames` frames of the current state (after one
- * untimed frame) under every combination of digit width 6, 8 or 10 with
- * gather form 0 or 1 at workgroup size 128 or 256, and with gather form 2 at
- * each tile shape (when the scene's box is binned by 4x4x4-cell tiles),
+ * untimed frame) under every combination of processing order 1 or 2 and
+ * digit width 6, 8 or 10 with gather form 0 or 1 at workgroup size 128 or
+ * 256, and with gather form 2 at each tile shape (when the scene's box is
+ * binned by 4x4x4-cell tiles),
  * restoring the particle state after each trial; keeps the fastest and
  * returns its gather form and digit width (the workgroup size and tile shape
  * through fsi_pbf_get_tuning).  The reference hard-codes its launch shape

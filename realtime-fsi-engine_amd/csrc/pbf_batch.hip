@@ -105,6 +105,7 @@ int fsi_batch_create(const fsi_fluid_config* cfg, fsi_batch** out) {
         delete b;
         return rc;
     }
+    b->core.d.relabel = true;  // storage in Morton order of the buckets (processing order 2)
     b->cfg = *cfg;
     const float zero[3] = {0, 0, 0};
     pack_constants(b->cfg, zero, zero, std::clamp(cfg->bound_damping, 0.0f, 1.0f), 0, b->k);

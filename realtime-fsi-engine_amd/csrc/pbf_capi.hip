@@ -183,6 +183,7 @@ int fsi_pbf_create(const fsi_fluid_config* cfg, fsi_pbf** out) {
         delete h;
         return rc;
     }
+    h->core.d.relabel = true;  // storage in Morton order of the buckets (processing order 2)
     h->cfg = *cfg;
     apply_bounds(h, cfg->bounds_min, cfg->bounds_max, cfg->bound_damping);
     repack(h);
@@ -538,7 +539,9 @@ int fsi_pbf_read_solver(fsi_pbf* h, float* ppl, float* dpr) {
         return FSI_OK;
     }
     std::vector<uint32_t> perm(n);
-    FSI_HIP_TRY(hipMemcpy(perm.data(), c.d.perm, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
+    // storage slot -> particle id of the last substep's arrays
+    FSI_HIP_TRY(hipMemcpy(perm.data(), c.d.stored_relabel ? c.d.mperm : c.d.perm, sizeof(uint32_t) * n,
+                          hipMemcpyDeviceToHost));
     std::vector<float> tmp(4 * (size_t)n);
     if (ppl) {
         FSI_HIP_TRY(hipMemcpy(tmp.data(), c.d.P, sizeof(float4) * n, hipMemcpyDeviceToHost));
@@ -572,7 +575,9 @@ int fsi_pbf_read_counts(fsi_pbf* h, uint32_t* cand, uint32_t* nbr) {
     FSI_HIP_TRY(hipStreamSynchronize(c.stream));
     if (!c.stepped) return fail(FSI_ERR_STATE, "no step has run");
     std::vector<uint32_t> perm(n), tmp(n);
-    FSI_HIP_TRY(hipMemcpy(perm.data(), c.d.perm, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
+    // storage slot -> particle id of the last substep's arrays
+    FSI_HIP_TRY(hipMemcpy(perm.data(), c.d.stored_relabel ? c.d.mperm : c.d.perm, sizeof(uint32_t) * n,
+                          hipMemcpyDeviceToHost));
     if (cand) {
         FSI_HIP_TRY(hipMemcpy(tmp.data(), c.d.cand, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
         for (uint32_t k = 0; k < n; ++k) cand[perm[k]] = tmp[k];

@@ -106,10 +106,11 @@ void core_invalidate_graph(Core& c);
 // state's `tiled`.
 using Rebin = std::function<int(bool tiles, StepConst& out)>;
 // Times `frames` frames under every combination of gather form, radix digit
-// width, workgroup size (forms 0 and 1) and tile shape (form 2, when the
-// binning gives tiles), restoring the particle state (pos, vel) after each
-// trial; keeps the fastest, leaves the binning for it, and returns its gather
-// form and digit width (the rest through core_get_tuning).
+// width, workgroup size (forms 0 and 1), tile shape (form 2, when the binning
+// gives tiles) and storage order (Morton processing order over sorted slots,
+// or relabelled), restoring the particle state (pos, vel) after each trial;
+// keeps the fastest, leaves the binning for it, and returns its gather form
+// and digit width (the rest through core_get_tuning).
 int core_autotune(Core& c, const Rebin& rebin, const FramePlan& plan, uint32_t frames, int* gather, int* digit_bits);
 
 // Morton binning of one scene's processing order: cover its bounds (plus two

@@ -208,6 +208,7 @@ void core_destroy(Core& c) {
     dev_free(&d.xstar);
     dev_free(&d.keys);
     dev_free(&d.cells);
+    dev_free(&d.mcells);
     dev_free(&d.P);
     dev_free(&d.Q);
     dev_free(&d.PL);
@@ -225,6 +226,8 @@ void core_destroy(Core& c) {
     dev_free(&d.bins);
     dev_free(&d.bin_of);
     dev_free(&d.bin_rank);
+    dev_free(&d.bin_m);
+    dev_free(&d.mperm);
     dev_free(&d.order);
     dev_free(&d.list_count);
     dev_free(&d.list);
@@ -253,15 +256,20 @@ void core_destroy(Core& c) {
 }
 
 hipError_t core_alloc_cells(Core& c, uint32_t entries) {
-    if (c.cells_capacity != entries || !c.d.cells) {
+    if (c.cells_capacity != entries || !c.d.cells || !c.d.mcells) {
         hipError_t e = dev_alloc(&c.d.cells, entries);
+        if (e == hipSuccess) e = dev_alloc(&c.d.mcells, entries);
         if (e != hipSuccess) return e;
         c.cells_capacity = entries;
         core_invalidate_graph(c);
     }
     c.d.hash_size = entries;
     std::vector<int32_t> neg(2 * (size_t)entries, -1);
-    return hipMemcpyAsync(c.d.cells, neg.data(), neg.size() * sizeof(int32_t), hipMemcpyHostToDevice, c.stream);
+    hipError_t e = hipMemcpyAsync(c.d.cells, neg.data(), neg.size() * sizeof(int32_t), hipMemcpyHostToDevice, c.stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(c.d.mcells, neg.data(), neg.size() * sizeof(int32_t), hipMemcpyHostToDevice, c.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c.stream);  // neg is freed on return
+    return e;
 }
 
 hipError_t core_alloc_particles(Core& c, uint32_t n) {
@@ -290,6 +298,8 @@ hipError_t core_alloc_particles(Core& c, uint32_t n) {
     ALLOC(d.bins, kOrderBins);
     ALLOC(d.bin_of, n);
     ALLOC(d.bin_rank, n);
+    ALLOC(d.bin_m, n);
+    ALLOC(d.mperm, n);
     ALLOC(d.order, n);
     ALLOC(d.list_count, n);
     ALLOC(d.list, (size_t)kNbrCap * (((size_t)n + 63) & ~(size_t)63));  // whole 64-thread blocks
@@ -401,11 +411,11 @@ int core_frame(Core& c, const StepConst& k, const FramePlan& plan) {
         rc = enqueue_frame(c, k, plan, c.timing);
     } else {
         // key: everything the launches of a frame depend on
-        std::vector<unsigned char> key(sizeof(StepConst) + sizeof(FramePlan) + 5 * sizeof(int));
+        const int knobs[8] = {c.d.variant, c.sort_digit_bits, c.taps ? 1 : 0, c.d.any_sdf ? 1 : 0,
+                              c.d.morton ? 1 : 0, c.d.relabel ? 1 : 0, c.d.block, c.d.tile_shape};
+        std::vector<unsigned char> key(sizeof(StepConst) + sizeof(FramePlan) + sizeof(knobs));
         std::memcpy(key.data(), &k, sizeof(StepConst));
         std::memcpy(key.data() + sizeof(StepConst), &plan, sizeof(FramePlan));
-        const int knobs[5] = {c.d.variant, c.sort_digit_bits, c.taps ? 1 : 0, c.d.any_sdf ? 1 : 0,
-                              c.d.morton ? 1 : 0};
         std::memcpy(key.data() + sizeof(StepConst) + sizeof(FramePlan), knobs, sizeof(knobs));
         if (!c.graph_valid || key != c.graph_key) {
             core_invalidate_graph(c);
@@ -467,9 +477,10 @@ struct TuneScope {
 int core_autotune(Core& c, const Rebin& rebin, const FramePlan& plan, uint32_t frames, int* gather, int* digit_bits) {
     const uint32_t n = c.d.n;
     struct Knobs {
-        int g, bits, block, tile;
+        int g, bits, block, tile, order;
     };
-    Knobs best{c.d.variant, c.sort_digit_bits, c.d.block, c.d.tile_shape};
+    const int order_now = c.d.relabel ? 2 : (c.d.morton ? 1 : 0);
+    Knobs best{c.d.variant, c.sort_digit_bits, c.d.block, c.d.tile_shape, order_now};
     int rc = FSI_OK;
     if (n > 0 && frames > 0) {
         TuneScope scope(c);
@@ -480,20 +491,25 @@ int core_autotune(Core& c, const Rebin& rebin, const FramePlan& plan, uint32_t f
         FSI_HIP_TRY(hipEventCreate(&scope.ea));
         FSI_HIP_TRY(hipEventCreate(&scope.eb));
         // the trials: forms 0 and 1 at both workgroup sizes, form 2 at every
-        // tile shape when its binning gives tiles (else it would run as form 1)
+        // tile shape when its binning gives tiles (else it would run as form
+        // 1), each in Morton processing order over sorted-slot storage (1) and
+        // over relabelled storage (2)
         std::vector<Knobs> trials;
         const int bits[] = {6, 8, 10};
-        for (int b : bits) {
-            for (int g = 0; g <= 1; ++g)
-                for (int bs : {128, 256}) trials.push_back({g, b, bs, c.d.tile_shape});
-            for (int t = 0; t < kTileShapes; ++t) trials.push_back({2, b, c.d.block, t});
-        }
+        for (int o = 1; o <= 2; ++o)
+            for (int b : bits) {
+                for (int g = 0; g <= 1; ++g)
+                    for (int bs : {128, 256}) trials.push_back({g, b, bs, c.d.tile_shape, o});
+                for (int t = 0; t < kTileShapes; ++t) trials.push_back({2, b, c.d.block, t, o});
+            }
         float best_ms = 1e30f;
         for (const Knobs& t : trials) {
             c.d.variant = t.g;
             c.sort_digit_bits = t.bits;
             c.d.block = t.block;
             c.d.tile_shape = t.tile;
+            c.d.morton = t.order >= 1;
+            c.d.relabel = t.order == 2;
             StepConst k{};
             if ((rc = rebin(t.g == 2, k)) != FSI_OK) break;
             if (t.g == 2 && !c.d.tiled) continue;
@@ -519,6 +535,8 @@ int core_autotune(Core& c, const Rebin& rebin, const FramePlan& plan, uint32_t f
     c.sort_digit_bits = best.bits;
     c.d.block = best.block;
     c.d.tile_shape = best.tile;
+    c.d.morton = best.order >= 1;
+    c.d.relabel = best.order == 2;
     StepConst k{};
     const int rc2 = rebin(best.g == 2, k);
     if (rc == FSI_OK) rc = rc2;
@@ -553,8 +571,9 @@ int core_set_tuning(Core& c, int knob, int value) {
             c.sort_digit_bits = value;
             return FSI_OK;
         case FSI_TUNE_ORDER:
-            if (value < 0 || value > 1) return fail(FSI_ERR_INVALID_ARG, "processing order must be 0 or 1");
-            c.d.morton = value == 1;
+            if (value < 0 || value > 2) return fail(FSI_ERR_INVALID_ARG, "processing order must be 0, 1 or 2");
+            c.d.morton = value >= 1;
+            c.d.relabel = value == 2;
             return FSI_OK;
         case FSI_TUNE_BLOCK:
             if (value != 128 && value != 256) return fail(FSI_ERR_INVALID_ARG, "workgroup size must be 128 or 256");
@@ -573,7 +592,7 @@ int core_get_tuning(const Core& c, int knob, int* value) {
     switch (knob) {
         case FSI_TUNE_GATHER: *value = c.d.variant; return FSI_OK;
         case FSI_TUNE_SORT_DIGIT_BITS: *value = c.sort_digit_bits; return FSI_OK;
-        case FSI_TUNE_ORDER: *value = c.d.morton ? 1 : 0; return FSI_OK;
+        case FSI_TUNE_ORDER: *value = c.d.relabel ? 2 : (c.d.morton ? 1 : 0); return FSI_OK;
         case FSI_TUNE_BLOCK: *value = c.d.block; return FSI_OK;
         case FSI_TUNE_TILE: *value = c.d.tile_shape; return FSI_OK;
         default: return fail(FSI_ERR_INVALID_ARG, "unknown knob");

@@ -52,6 +52,12 @@ __device__ __forceinline__ uint32_t xcd_gid_bs() {
 template <int BS>
 inline dim3 grid_bs(uint32_t n) { return dim3((n + BS - 1) / BS); }
 
+// The storage slot thread t takes: order[t], or t itself when the storage is
+// in processing order (order == nullptr).
+__device__ __forceinline__ uint32_t slot_at(const uint32_t* __restrict__ order, uint32_t t) {
+    return order != nullptr ? order[t] : t;
+}
+
 // Per-lane LDS list of the two-phase form (entries), and the fill level at
 // which the whole wave flushes.
 constexpr uint32_t kLdsCap = 24;
@@ -104,7 +110,7 @@ __global__ __launch_bounds__(BS) void k_build_list(StepConst c, const uint32_t* 
                                                        uint32_t* __restrict__ cand_out) {
     const uint32_t t = xcd_gid_bs<BS>();
     if (t >= c.n) return;
-    const uint32_t k = order[t];
+    const uint32_t k = slot_at(order, t);
     if (lod != nullptr && c.enable_apbf != 0u && lod[k] < iter) return;
     const V3 x = xyz(X[k]);
     const uint2 qi = quantize(x, c.qscale);
@@ -383,6 +389,62 @@ __global__ __launch_bounds__(kBlock) void k_order(uint32_t n, const uint32_t* __
     order[base[bin_of[k]] + bin_rank[k]] = k;
 }
 
+// Relabelled storage (processing order 2).  Every hash bucket goes whole
+// into the Morton bin of the cell of its first particle, and the particle
+// arrays of the substep are stored bin by bin, each bucket contiguous and in
+// its sorted-slot order: a bucket's slots [start, end) become the storage
+// slots [m, m + end - start), so a gather that visits buckets in the
+// reference order and each bucket in slot order sees the same sequence, while
+// the particles a wave takes (consecutive storage slots) and their neighbours
+// lie close together in memory.  The first kernel sizes the bins (one atomic
+// per bucket, on its first slot), the second places every slot and writes the
+// cell table over storage slots (mcells).
+template <bool B>
+__global__ __launch_bounds__(kBlock) void k_bucket_bins(StepConst c, const uint32_t* __restrict__ skeys,
+                                                        const uint32_t* __restrict__ perm,
+                                                        const float4* __restrict__ xstar,
+                                                        const int2* __restrict__ cells, uint32_t* __restrict__ bins,
+                                                        uint32_t* __restrict__ bin_of, uint32_t* __restrict__ bin_rank) {
+    const uint32_t k = gid();
+    if (k >= c.n) return;
+    const uint32_t key = skeys[k];
+    if (k != 0 && skeys[k - 1] == key) return;  // not the bucket's first slot
+    const int2 se = cells[key];
+    int32_t cx, cy, cz;
+    cell_of(xyz(xstar[perm[k]]), c.h, cx, cy, cz);
+    const uint32_t b = morton_bin(domain_of<B>(c, k), cx, cy, cz);
+    bin_of[k] = b;
+    bin_rank[k] = atomicAdd(&bins[b], (uint32_t)(se.y - se.x));
+}
+
+template <bool B>
+__global__ __launch_bounds__(kBlock) void k_relabel(StepConst c, const uint32_t* __restrict__ skeys,
+                                                    const uint32_t* __restrict__ perm,
+                                                    const float4* __restrict__ xstar, const float4* __restrict__ vel,
+                                                    const int2* __restrict__ cells, const uint32_t* __restrict__ base,
+                                                    const uint32_t* __restrict__ bin_of,
+                                                    const uint32_t* __restrict__ bin_rank, float4* __restrict__ P,
+                                                    uint2* __restrict__ Q, float4* __restrict__ PL,
+                                                    float4* __restrict__ vs, uint32_t* __restrict__ mperm,
+                                                    uint32_t* __restrict__ bin_m, int2* __restrict__ mcells) {
+    const uint32_t k = gid();
+    if (k >= c.n) return;
+    const uint32_t key = skeys[k];
+    const int2 se = cells[key];
+    const uint32_t head = (uint32_t)se.x;
+    const uint32_t b = bin_of[head];
+    const uint32_t m = base[b] + bin_rank[head] + (k - head);
+    const uint32_t id = perm[k];
+    const float4 x = xstar[id];
+    P[m] = x;
+    Q[m] = quantize(xyz(x), c.qscale);
+    PL[m] = x;
+    vs[m] = vel[id];
+    mperm[m] = id;
+    bin_m[m] = b;
+    if (k == head) mcells[key] = make_int2((int32_t)m, (int32_t)(m + (uint32_t)(se.y - se.x)));
+}
+
 // CS_ComputeLOD.glsl:66-79
 __global__ __launch_bounds__(kBlock) void k_lod(StepConst c, const float4* __restrict__ P,
                                                 uint32_t* __restrict__ lod) {
@@ -411,7 +473,7 @@ __global__ __launch_bounds__(BS) void k_lambda(StepConst c, const uint32_t* __re
                                                    uint32_t* __restrict__ nbr_out) {
     const uint32_t t = xcd_gid_bs<BS>();
     if (t >= c.n) return;
-    const uint32_t k = order[t];
+    const uint32_t k = slot_at(order, t);
     const V3 xi = xyz(P[k]);
     if (c.enable_apbf != 0u && lod[k] < iter) {
         // inactive: keep the last lambda, refresh the position
@@ -460,7 +522,7 @@ __global__ __launch_bounds__(BS) void k_deltap(StepConst c, const uint32_t* __re
                                                    const uint32_t* __restrict__ list, float4* __restrict__ dp_tap) {
     const uint32_t t = xcd_gid_bs<BS>();
     if (t >= c.n) return;
-    const uint32_t k = order[t];
+    const uint32_t k = slot_at(order, t);
     if (c.enable_apbf != 0u && lod[k] < iter) return;  // P[k] already holds x
     const float4 xi4 = PL[k];
     const V3 xi = xyz(xi4);
@@ -498,7 +560,7 @@ __global__ __launch_bounds__(kBlock) void k_apply_sdf(StepConst c, const uint32_
                                                       const uint32_t* __restrict__ lod, uint32_t iter) {
     const uint32_t t = xcd_gid();
     if (t >= c.n) return;
-    const uint32_t k = order[t];
+    const uint32_t k = slot_at(order, t);
     if (c.enable_apbf != 0u && lod[k] < iter) return;
     const Domain& dom = domain_of<B>(c, k);
     if (dom.enable_sdf != 1u) return;
@@ -523,7 +585,7 @@ __global__ __launch_bounds__(BS) void k_integrate(StepConst c, const uint32_t* _
                                                       float4* __restrict__ curl_out) {
     const uint32_t t = xcd_gid_bs<BS>();
     if (t >= c.n) return;
-    const uint32_t k = order[t];
+    const uint32_t k = slot_at(order, t);
     const V3 old_vel = xyz(vs[k]);
     const V3 xp = xyz(P[k]);
     const float inv = c.inv_rho0;
@@ -573,7 +635,7 @@ __global__ __launch_bounds__(kBlock) void k_integrate_finish(StepConst c, const 
                                                              float4* __restrict__ posS, uint2* __restrict__ Q) {
     const uint32_t t = xcd_gid();
     if (t >= c.n) return;
-    const uint32_t k = order[t];
+    const uint32_t k = slot_at(order, t);
     const Domain& dom = domain_of<B>(c, k);
     const uint32_t id = perm[k];
     const float4 op4 = pos[id];
@@ -621,7 +683,7 @@ __global__ __launch_bounds__(BS) void k_vorticity(StepConst c, const uint32_t* _
                                                       const uint32_t* __restrict__ list) {
     const uint32_t t = xcd_gid_bs<BS>();
     if (t >= c.n) return;
-    const uint32_t k = order[t];
+    const uint32_t k = slot_at(order, t);
     
     const V3 pi = xyz(posS[k]);
     V3 eta = mk(0.0f, 0.0f, 0.0f);
@@ -736,7 +798,7 @@ __device__ bool stage_tile(const StepConst& c, TileStage<M>& ts, const float4* _
                            const int2* __restrict__ cells, const uint32_t* __restrict__ order,
                            const uint32_t* __restrict__ bin_of, uint32_t start) {
     using S = TileShape<M>;
-    const uint32_t k0 = order[start];
+    const uint32_t k0 = slot_at(order, start);
     const Domain& dm = domain_of<B>(c, k0);
     const uint32_t code = (bin_of[k0] - dm.bin_base) & ~((1u << M) - 1u);  // the tile's first bin
     const int32_t ox = dm.dcmin[0] + (int32_t)(compact7(code) << kTileShift);
@@ -883,7 +945,7 @@ __global__ __launch_bounds__(kBlock) void k_lambda_tile(StepConst c, const uint3
         const uint32_t start = tiles[tile], end = tiles[tile + 1];
         const bool ok = stage_tile<M, B>(c, ts, P, cells, order, bin_of, start);
         for (uint32_t t = start + threadIdx.x; t < end; t += kBlock) {
-            const uint32_t k = order[t];
+            const uint32_t k = slot_at(order, t);
             const V3 xi = xyz(P[k]);
             if (c.enable_apbf != 0u && lod[k] < iter) {
                 PL[k] = make_float4(xi.x, xi.y, xi.z, PL[k].w);
@@ -989,7 +1051,7 @@ __global__ __launch_bounds__(kBlock) void k_deltap_tile(StepConst c, const uint3
         const uint32_t start = tiles[tile], end = tiles[tile + 1];
         const bool ok = stage_tile<M, B>(c, ts, PL, cells, order, bin_of, start);
         for (uint32_t t = start + threadIdx.x; t < end; t += kBlock) {
-            const uint32_t k = order[t];
+            const uint32_t k = slot_at(order, t);
             if (c.enable_apbf != 0u && lod[k] < iter) continue;  // P[k] already holds x
             const float4 xi4 = PL[k];
             const V3 xi = xyz(xi4);
@@ -1052,7 +1114,7 @@ __global__ __launch_bounds__(kBlock) void k_tile_flags(uint32_t n, uint32_t m, c
                                                        uint32_t* __restrict__ tflag) {
     const uint32_t t = gid();
     if (t >= n) return;
-    tflag[t] = (t == 0 || (bin_of[order[t]] >> m) != (bin_of[order[t - 1]] >> m)) ? 1u : 0u;
+    tflag[t] = (t == 0 || (bin_of[slot_at(order, t)] >> m) != (bin_of[slot_at(order, t - 1)] >> m)) ? 1u : 0u;
 }
 
 __global__ __launch_bounds__(kBlock) void k_tile_list(uint32_t n, const uint32_t* __restrict__ tflag,
@@ -1095,22 +1157,49 @@ hipError_t launch_cells(const StepConst& c, DeviceState& d, hipStream_t st) {
 // The staged-tile gathers take Morton tiles of the processing order.
 static inline bool tiled_form(const DeviceState& d) { return d.variant == 2 && d.tiled && d.morton; }
 
+// What the passes after the reorder read: the cell table over the storage
+// slots, the storage slot -> particle id map and the Morton bin of each
+// storage slot (sorted slots unless the storage is relabelled).
+static inline const int2* gcells(const DeviceState& d) { return d.relabel ? d.mcells : d.cells; }
+static inline const uint32_t* sperm(const DeviceState& d) { return d.relabel ? d.mperm : d.perm; }
+static inline const uint32_t* sbins(const DeviceState& d) { return d.relabel ? d.bin_m : d.bin_of; }
+static inline const uint32_t* sorder(const DeviceState& d) { return d.relabel ? nullptr : d.order; }
+
 hipError_t launch_reorder(const StepConst& c, DeviceState& d, hipStream_t st) {
     hipLaunchKernelGGL(k_fill_u32, grid_for(kOrderBins), dim3(kBlock), 0, st, d.bins, kOrderBins, 0u);
+    hipError_t e = hipSuccess;
+    d.stored_relabel = d.relabel;
+    if (d.relabel) {
+        hipLaunchKernelGGL(k_clear_cells, grid_for(d.hash_size), dim3(kBlock), 0, st, d.mcells, d.hash_size);
+#define L(B)                                                                                                        \
+    hipLaunchKernelGGL((k_bucket_bins<B>), grid_for(c.n), dim3(kBlock), 0, st, c, d.skeys, d.perm, d.xstar, d.cells, \
+                       d.bins, d.bin_of, d.bin_rank);
+        FSI_B_SWITCH(d.batched, L)
+#undef L
+        e = scan_exclusive_u32(d.bins, d.bins, kOrderBins, d.sort.partial, st);
+        if (e != hipSuccess) return e;
+#define L(B)                                                                                                    \
+    hipLaunchKernelGGL((k_relabel<B>), grid_for(c.n), dim3(kBlock), 0, st, c, d.skeys, d.perm, d.xstar, d.vel,  \
+                       d.cells, d.bins, d.bin_of, d.bin_rank, d.P, d.Q, d.PL, d.vs, d.mperm, d.bin_m, d.mcells);
+        FSI_B_SWITCH(d.batched, L)
+#undef L
+    } else {
 #define L(B)                                                                                                    \
     hipLaunchKernelGGL((k_reorder<B>), grid_for(c.n), dim3(kBlock), 0, st, c, d.perm, d.xstar, d.vel, d.P, d.Q, \
                        d.PL, d.vs, d.bins, d.bin_of, d.bin_rank);
-    FSI_B_SWITCH(d.batched, L)
+        FSI_B_SWITCH(d.batched, L)
 #undef L
-    hipError_t e = scan_exclusive_u32(d.bins, d.bins, kOrderBins, d.sort.partial, st);
-    if (e != hipSuccess) return e;
-    if (d.morton)
-        hipLaunchKernelGGL(k_order, grid_for(c.n), dim3(kBlock), 0, st, c.n, d.bins, d.bin_of, d.bin_rank, d.order);
-    else
-        hipLaunchKernelGGL(k_order_slots, grid_for(c.n), dim3(kBlock), 0, st, c.n, d.order);
+        e = scan_exclusive_u32(d.bins, d.bins, kOrderBins, d.sort.partial, st);
+        if (e != hipSuccess) return e;
+        if (d.morton)
+            hipLaunchKernelGGL(k_order, grid_for(c.n), dim3(kBlock), 0, st, c.n, d.bins, d.bin_of, d.bin_rank,
+                               d.order);
+        else
+            hipLaunchKernelGGL(k_order_slots, grid_for(c.n), dim3(kBlock), 0, st, c.n, d.order);
+    }
     if (tiled_form(d)) {
-        hipLaunchKernelGGL(k_tile_flags, grid_for(c.n), dim3(kBlock), 0, st, c.n, (uint32_t)d.tile_shape, d.order,
-                           d.bin_of, d.tflag);
+        hipLaunchKernelGGL(k_tile_flags, grid_for(c.n), dim3(kBlock), 0, st, c.n, (uint32_t)d.tile_shape, sorder(d),
+                           sbins(d), d.tflag);
         e = scan_exclusive_u32(d.tflag, d.tidx, c.n, d.sort.partial, st);
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL(k_tile_list, grid_for(c.n), dim3(kBlock), 0, st, c.n, d.tflag, d.tidx, d.tiles, d.ntiles);
@@ -1128,11 +1217,11 @@ template <bool B, int BS>
 static void build_list_bs(const StepConst& c, DeviceState& d, const float4* X, const uint32_t* lod, uint32_t iter,
                           bool taps, hipStream_t st) {
     if (taps)
-        hipLaunchKernelGGL((k_build_list<true, B, BS>), grid_bs<BS>(c.n), dim3(BS), 0, st, c, d.order, X, d.Q, lod,
-                           iter, d.cells, d.list_count, d.list, d.cand);
+        hipLaunchKernelGGL((k_build_list<true, B, BS>), grid_bs<BS>(c.n), dim3(BS), 0, st, c, sorder(d), X, d.Q, lod,
+                           iter, gcells(d), d.list_count, d.list, d.cand);
     else
-        hipLaunchKernelGGL((k_build_list<false, B, BS>), grid_bs<BS>(c.n), dim3(BS), 0, st, c, d.order, X, d.Q, lod,
-                           iter, d.cells, d.list_count, d.list, d.cand);
+        hipLaunchKernelGGL((k_build_list<false, B, BS>), grid_bs<BS>(c.n), dim3(BS), 0, st, c, sorder(d), X, d.Q, lod,
+                           iter, gcells(d), d.list_count, d.list, d.cand);
 }
 template <bool B>
 static void build_list_t(const StepConst& c, DeviceState& d, const float4* X, const uint32_t* lod, uint32_t iter,
@@ -1162,11 +1251,11 @@ hipError_t launch_lambda(const StepConst& c, DeviceState& d, uint32_t iter, bool
         hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(kBlock), 0, st, d.ndefer, kDeferWords, 0u);
 #define LM(M, T, B)                                                                                              \
     hipLaunchKernelGGL((k_lambda_tile<M, T, B>), dim3(kTileGrid), dim3(kBlock), 0, st, c, d.tiles, d.ntiles,     \
-                       d.order, d.bin_of, d.P, d.PL, d.rho, d.lod, d.cells, iter, d.cand, d.nbr, d.defer, d.ndefer);
+                       sorder(d), sbins(d), d.P, d.PL, d.rho, d.lod, gcells(d), iter, d.cand, d.nbr, d.defer, d.ndefer);
 #define L(T, B)                                                                                                  \
     if (d.tile_shape == 2) { LM(2, T, B) } else if (d.tile_shape == 1) { LM(1, T, B) } else { LM(0, T, B) }      \
     hipLaunchKernelGGL((k_lambda_deferred<T, B>), dim3(kDeferGrid), dim3(kBlock), 0, st, c, d.defer, d.ndefer, d.P, \
-                       d.PL, d.rho, d.cells, d.cand, d.nbr);
+                       d.PL, d.rho, gcells(d), d.cand, d.nbr);
         if (taps) {
             FSI_B_SWITCH(d.batched, L_T)
         } else {
@@ -1183,11 +1272,11 @@ hipError_t launch_lambda(const StepConst& c, DeviceState& d, uint32_t iter, bool
     }
 #define L(V, B, BS)                                                                                              \
     if (taps)                                                                                                    \
-        hipLaunchKernelGGL((k_lambda<V, true, B, BS>), grid_bs<BS>(c.n), dim3(BS), 0, st, c, d.order, d.P, d.PL, \
-                           d.rho, d.lod, d.cells, iter, d.list_count, d.list, d.cand, d.nbr);                    \
+        hipLaunchKernelGGL((k_lambda<V, true, B, BS>), grid_bs<BS>(c.n), dim3(BS), 0, st, c, sorder(d), d.P, d.PL, \
+                           d.rho, d.lod, gcells(d), iter, d.list_count, d.list, d.cand, d.nbr);                    \
     else                                                                                                         \
-        hipLaunchKernelGGL((k_lambda<V, false, B, BS>), grid_bs<BS>(c.n), dim3(BS), 0, st, c, d.order, d.P, d.PL, \
-                           d.rho, d.lod, d.cells, iter, d.list_count, d.list, d.cand, d.nbr);
+        hipLaunchKernelGGL((k_lambda<V, false, B, BS>), grid_bs<BS>(c.n), dim3(BS), 0, st, c, sorder(d), d.P, d.PL, \
+                           d.rho, d.lod, gcells(d), iter, d.list_count, d.list, d.cand, d.nbr);
     FSI_GATHER_SWITCH(L)
 #undef L
     return hipGetLastError();
@@ -1198,11 +1287,11 @@ hipError_t launch_deltap(const StepConst& c, DeviceState& d, uint32_t iter, bool
         hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(kBlock), 0, st, d.ndefer, kDeferWords, 0u);
 #define LM(M, T, B)                                                                                               \
     hipLaunchKernelGGL((k_deltap_tile<M, T, B>), dim3(kTileGrid), dim3(kBlock), 0, st, c, d.tiles, d.ntiles,      \
-                       d.order, d.bin_of, d.PL, d.P, d.Q, d.lod, d.cells, iter, d.dp_tap, d.defer, d.ndefer);
+                       sorder(d), sbins(d), d.PL, d.P, d.Q, d.lod, gcells(d), iter, d.dp_tap, d.defer, d.ndefer);
 #define L(T, B)                                                                                                   \
     if (d.tile_shape == 2) { LM(2, T, B) } else if (d.tile_shape == 1) { LM(1, T, B) } else { LM(0, T, B) }       \
     hipLaunchKernelGGL((k_deltap_deferred<T, B>), dim3(kDeferGrid), dim3(kBlock), 0, st, c, d.defer, d.ndefer, d.PL, \
-                       d.P, d.Q, d.cells, d.dp_tap);
+                       d.P, d.Q, gcells(d), d.dp_tap);
         if (taps) {
             FSI_B_SWITCH(d.batched, L_T)
         } else {
@@ -1213,16 +1302,16 @@ hipError_t launch_deltap(const StepConst& c, DeviceState& d, uint32_t iter, bool
     } else {
 #define L(V, B, BS)                                                                                             \
     if (taps)                                                                                                   \
-        hipLaunchKernelGGL((k_deltap<V, true, B, BS>), grid_bs<BS>(c.n), dim3(BS), 0, st, c, d.order, d.PL, d.P, \
-                           d.Q, d.lod, d.cells, iter, d.list_count, d.list, d.dp_tap);                          \
+        hipLaunchKernelGGL((k_deltap<V, true, B, BS>), grid_bs<BS>(c.n), dim3(BS), 0, st, c, sorder(d), d.PL, d.P, \
+                           d.Q, d.lod, gcells(d), iter, d.list_count, d.list, d.dp_tap);                          \
     else                                                                                                        \
-        hipLaunchKernelGGL((k_deltap<V, false, B, BS>), grid_bs<BS>(c.n), dim3(BS), 0, st, c, d.order, d.PL,     \
-                           d.P, d.Q, d.lod, d.cells, iter, d.list_count, d.list, d.dp_tap);
+        hipLaunchKernelGGL((k_deltap<V, false, B, BS>), grid_bs<BS>(c.n), dim3(BS), 0, st, c, sorder(d), d.PL,     \
+                           d.P, d.Q, d.lod, gcells(d), iter, d.list_count, d.list, d.dp_tap);
         FSI_GATHER_SWITCH(L)
 #undef L
     }
     if (d.any_sdf) {
-#define L(B) hipLaunchKernelGGL((k_apply_sdf<B>), grid_for(c.n), dim3(kBlock), 0, st, c, d.order, d.P, d.Q, d.lod, iter);
+#define L(B) hipLaunchKernelGGL((k_apply_sdf<B>), grid_for(c.n), dim3(kBlock), 0, st, c, sorder(d), d.P, d.Q, d.lod, iter);
         FSI_B_SWITCH(d.batched, L)
 #undef L
     }
@@ -1237,14 +1326,14 @@ hipError_t launch_integrate(const StepConst& c, DeviceState& d, hipStream_t st) 
 #undef L
         }
 #define L(V, B, BS)                                                                                              \
-    hipLaunchKernelGGL((k_integrate<V, B, BS>), grid_bs<BS>(c.n), dim3(BS), 0, st, c, d.order, d.P, d.vs,       \
-                       d.cells, d.list_count, d.list, d.vdelta, d.curl);
+    hipLaunchKernelGGL((k_integrate<V, B, BS>), grid_bs<BS>(c.n), dim3(BS), 0, st, c, sorder(d), d.P, d.vs,       \
+                       gcells(d), d.list_count, d.list, d.vdelta, d.curl);
         FSI_GATHER_SWITCH(L)
 #undef L
     }
 #define L(B)                                                                                                   \
-    hipLaunchKernelGGL((k_integrate_finish<B>), grid_for(c.n), dim3(kBlock), 0, st, c, d.order, d.P, d.vs, d.vdelta, \
-                       d.perm, d.curl, d.pos, d.vel, d.posS, d.Q);
+    hipLaunchKernelGGL((k_integrate_finish<B>), grid_for(c.n), dim3(kBlock), 0, st, c, sorder(d), d.P, d.vs, d.vdelta, \
+                       sperm(d), d.curl, d.pos, d.vel, d.posS, d.Q);
     FSI_B_SWITCH(d.batched, L)
 #undef L
     return hipGetLastError();
@@ -1257,8 +1346,8 @@ hipError_t launch_vorticity(const StepConst& c, DeviceState& d, hipStream_t st) 
 #undef L
     }
 #define L(V, B, BS)                                                                                              \
-    hipLaunchKernelGGL((k_vorticity<V, B, BS>), grid_bs<BS>(c.n), dim3(BS), 0, st, c, d.order, d.posS, d.curl, \
-                       d.perm, d.vel, d.cells, d.list_count, d.list);
+    hipLaunchKernelGGL((k_vorticity<V, B, BS>), grid_bs<BS>(c.n), dim3(BS), 0, st, c, sorder(d), d.posS, d.curl, \
+                       sperm(d), d.vel, gcells(d), d.list_count, d.list);
     FSI_GATHER_SWITCH(L)
 #undef L
     return hipGetLastError();

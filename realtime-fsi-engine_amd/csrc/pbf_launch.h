@@ -67,8 +67,11 @@ struct DeviceState {
     uint32_t* flags = nullptr; // n: contact flags
     uint32_t* slots = nullptr; // n: exclusive scan of the contact flags
     uint32_t* bins = nullptr;  // kOrderBins: Morton-bin counts, then their exclusive scan
-    uint32_t* bin_of = nullptr;  // sorted slot -> Morton bin
-    uint32_t* bin_rank = nullptr;  // sorted slot -> rank inside its bin
+    uint32_t* bin_of = nullptr;  // sorted slot -> Morton bin (relabelled: its bucket's, at the bucket's first slot)
+    uint32_t* bin_rank = nullptr;  // sorted slot -> rank inside its bin (relabelled: the bucket's offset in it)
+    uint32_t* bin_m = nullptr;   // relabelled: storage slot -> Morton bin of its bucket
+    uint32_t* mperm = nullptr;   // relabelled: storage slot -> particle id
+    int2* mcells = nullptr;      // relabelled: the cell table over storage slots
     uint32_t* order = nullptr; // processing order: thread t handles sorted slot order[t]
     uint32_t* list_count = nullptr;  // by thread: neighbours found by the last lambda pass
     uint32_t* list = nullptr;  // kNbrCap slots per thread, n rounded up to 64 (layout: wave_run in pbf_kernels.hip)
@@ -83,6 +86,8 @@ struct DeviceState {
     int block = 256;           // workgroup size of the gather kernels, 128 or 256 (tuning knob)
     int tile_shape = 0;        // tiled gathers: tiles of 2^tile_shape bins, 0..2 (tuning knob)
     bool morton = true;        // processing order: Morton order of the cells (else sorted slots)
+    bool relabel = false;      // storage order = processing order: whole buckets in Morton order of their cells
+    bool stored_relabel = false;  // the storage order of the last enqueued reorder (for the host readbacks)
     bool tiled = false;        // every domain bins by 2^kTileShift cells (tiled gathers possible)
     bool batched = false;      // several scenes (StepConst::doms / scene_of)
     bool any_sdf = false;      // some domain has an SDF boundary

@@ -60,7 +60,8 @@ def run_single(base, sc, frames):
 @pytest.mark.parametrize("frames,graph,gather,block,tile", [(1, False, 1, 256, 0), (3, True, 1, 256, 0),
                                                              (3, True, 2, 256, 0), (2, False, 0, 128, 0),
                                                              (2, True, 1, 128, 0), (2, False, 2, 256, 1),
-                                                             (3, True, 2, 256, 2)])
+                                                             (3, True, 2, 256, 2), (3, True, 1, 256, -2),
+                                                             (2, False, 2, 256, -1)])
 def test_batch_scenes_equal_single_scene_runs(gpu, frames, graph, gather, block, tile):
     fsi = _fsi()
     base = dam_break_config(n_iter=3, substeps=2)
@@ -68,6 +69,9 @@ def test_batch_scenes_equal_single_scene_runs(gpu, frames, graph, gather, block,
     b = fsi.Batch(base)
     b.set_tuning(fsi.TUNE_GATHER, gather)
     b.set_tuning(fsi.TUNE_BLOCK, block)
+    if tile < 0:  # relabelled storage (processing order 2), tile shape -tile - 1
+        b.set_tuning(fsi.TUNE_ORDER, 2)
+        tile = -tile - 1
     b.set_tuning(fsi.TUNE_TILE, tile)
     b.set_graph(graph)
     b.set_scenes(scenes)

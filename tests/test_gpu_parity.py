@@ -238,13 +238,13 @@ def test_error_paths(gpu):
         s.step(1)
         keys, ids, cells = s.read_grid()
         assert cells.shape == (1 << 10, 2)
-        for knob, value in ((fsi.TUNE_GATHER, 3), (fsi.TUNE_SORT_DIGIT_BITS, 11), (fsi.TUNE_ORDER, 2),
+        for knob, value in ((fsi.TUNE_GATHER, 3), (fsi.TUNE_SORT_DIGIT_BITS, 11), (fsi.TUNE_ORDER, 3),
                             (fsi.TUNE_BLOCK, 64), (fsi.TUNE_BLOCK, 512), (fsi.TUNE_TILE, 3), (fsi.TUNE_TILE, -1),
                             (5, 0)):
             with pytest.raises(fsi.FsiError) as e:
                 s.set_tuning(knob, value)
             assert e.value.code == -1, (knob, value)
-        assert s.knobs() == {"gather": 1, "sort_digit_bits": 10, "order": 1, "block": 256, "tile": 0}
+        assert s.knobs() == {"gather": 1, "sort_digit_bits": 10, "order": 2, "block": 256, "tile": 0}
 
 
 def test_full_size_properties(gpu):
@@ -289,7 +289,8 @@ def test_knobs_and_graph_do_not_change_results(gpu, graph):
     ref.step(2)
     exp = ref.particles()
     knobs = ([(g, b, 1, 256, 0) for g in (0, 1, 2) for b in (6, 8, 10)] + [(g, 8, 0, 256, 0) for g in (0, 1, 2)]
-             + [(g, 8, 1, 128, 0) for g in (0, 1)] + [(2, b, 1, 256, t) for b in (6, 10) for t in (1, 2)])
+             + [(g, 8, 1, 128, 0) for g in (0, 1)] + [(2, b, 1, 256, t) for b in (6, 10) for t in (1, 2)]
+             + [(g, b, 2, 256, 0) for g in (0, 1, 2) for b in (6, 10)] + [(1, 8, 2, 128, 0), (2, 8, 2, 256, 2)])
     for gather, bits, order, block, tile in knobs:
         with fsi.Solver(cfg) as s:
             s.set_particles(parts)
