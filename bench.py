@@ -155,7 +155,7 @@ def timed_runs(step, sync, steps, runs):
     return walls
 
 
-def batched_leg(fsi, steps, warmup, runs=5):
+def batched_leg(fsi, steps, warmup, runs=5, autotune=True, graphs=True, loop=True):
     """Multi-scene handle vs a loop of single-scene handles on the same scenes,
     every handle replaying its frames from a HIP graph under the knobs the
     multi-scene autotuner picked.  Each form runs warmup frames, then `runs`
@@ -169,13 +169,19 @@ def batched_leg(fsi, steps, warmup, runs=5):
 
     b = fsi.Batch(batch_config(fsi))
     b.set_scenes(scenes)
-    b.autotune(1)
+    if autotune:
+        b.autotune(1)
     knobs = b.knobs()
-    b.set_graph(True)
+    b.set_graph(graphs)
     b.step(warmup)
     walls_b = timed_runs(lambda: b.step(1), b.sync, steps, runs)
     got = b.read_particles()
     b.close()
+    wall_b = float(np.median(walls_b))
+    if not loop:
+        return {"scenes": BATCH_SCENES, "particles": total, "iterations": BATCH_ITERS, "substeps": BATCH_SUBSTEPS,
+                "steps": steps, "warmup": warmup, "runs": runs, "value": work / wall_b,
+                "ms_per_step": wall_b / steps * 1e3, "graph": graphs, "loop": "not run"}
 
     solvers = []
     for sc in scenes:
@@ -184,7 +190,7 @@ def batched_leg(fsi, steps, warmup, runs=5):
         sv.set_sdf(sc.sdf, sc.sdf_padding, False)
         for name, k in fsi.KNOBS.items():
             sv.set_tuning(k, knobs[name])
-        sv.set_graph(True)
+        sv.set_graph(graphs)
         solvers.append(sv)
 
     def loop_step():
@@ -228,9 +234,9 @@ def batched_leg(fsi, steps, warmup, runs=5):
         "speedup": wall_l / wall_b,
         "loop_one_stream_value": work / wall_q, "loop_one_stream_ms_per_step": wall_q / steps * 1e3,
         "speedup_vs_one_stream_loop": wall_q / wall_b,
-        "autotuned": {"gather": knobs["gather"], "sort_digit_bits": knobs["sort_digit_bits"],
-                      "block": knobs["block"], "tile": knobs["tile"]},
-        "graph": {"batch": True, "loop": True, "loop_one_stream": True},
+        "autotuned" if autotune else "knobs": {"gather": knobs["gather"], "sort_digit_bits": knobs["sort_digit_bits"],
+                                               "block": knobs["block"], "tile": knobs["tile"], "order": knobs["order"]},
+        "graph": {"batch": graphs, "loop": graphs, "loop_one_stream": graphs},
         "identical_to_loop": bool(same and same_q),
     }
 
@@ -501,8 +507,13 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=8)
     ap.add_argument("--block", type=int, default=None, help="workgroup size of the gathers (128 or 256)")
     ap.add_argument("--tile", type=int, default=None, help="tile shape of the tiled gathers (0, 1, 2)")
-    ap.add_argument("--no-autotune", action="store_true", help="keep the default knobs in the plain leg")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default knobs (plain and batched legs) instead of autotuning them")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-batch-loop", action="store_true",
+                    help="time only the multi-scene handle in the batched leg (no per-scene loops)")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="enqueue the batched leg's frames directly instead of replaying HIP graphs")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--fsi-steps", type=int, default=300)
@@ -595,19 +606,24 @@ def main():
     dom = "lambda"
     dom_avg_ms = tm_dom["ms"][dom] / max(1, tm_dom["launches"][dom])
     achieved = round(ab[dom] / (dom_avg_ms * 1e-3) / 1e9, 1)
-    roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+    roofline = {"bound": "hbm", "kernel": "lambda pass (list builder + lambda walk, one launch pair per iteration)",
+                "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
                 "bytes_per_launch": ab[dom], "avg_launch_us": round(dom_avg_ms * 1e3, 2),
                 "launches": tm_dom["launches"][dom]}
-    # HBM bytes per launch of the dominant kernel: FETCH_SIZE + WRITE_SIZE of
-    # the same workload, calibrated against a copy kernel (profiles/)
-    tfile = os.path.join(ROOT, "profiles", "traffic_c2_1m.json")
+    # HBM bytes per launch of the dominant pass, measured separately: TCC EA
+    # read and write requests by size per dispatch, from rocprofv3 --pmc runs
+    # of this workload (frames 1-3) with the default knobs (profiles/)
+    tfile = os.path.join(ROOT, "profiles", "traffic_pbf_c2_1m.json")
     if os.path.exists(tfile):
         with open(tfile) as f:
-            kern = json.load(f)["kernels"]
-        hits = [v for k, v in kern.items() if k.startswith("k_" + dom + "<")]
-        if hits:
-            roofline["traffic"] = round(hits[0]["bytes_per_launch"])
+            tp = json.load(f)["passes"].get(dom)
+        if tp:
+            roofline["traffic"] = round(tp["bytes_per_launch"])
+            roofline["traffic_of"] = ("median bytes per dispatch of " + " + ".join(tp["kernels"]) +
+                                      ", rocprofv3 --pmc TCC_EA0_RDREQ_{32B,64B,128B}_sum and TCC_EA0_WRREQ(_64B)_sum "
+                                      "over frames 1-3 of this workload (profiles/traffic_pbf_c2_1m.md), "
+                                      "not over the timed window")
 
     out = {
         "metric": "PBF particle-iterations/s at 1M particles (plain/batched) + adjoint/forward ratio",
@@ -624,8 +640,10 @@ def main():
     }
     progress(f"plain leg: {out['ms_per_step']:.3f} ms/frame")
     if not args.no_batched:
-        out["batched"] = batched_leg(fsi, args.batch_steps, 1)
-        progress(f"batched leg: speed-up {out['batched']['speedup']:.2f}")
+        out["batched"] = batched_leg(fsi, args.batch_steps, 1, autotune=not args.no_autotune,
+                                     graphs=not args.no_graphs, loop=not args.no_batch_loop)
+        progress(f"batched leg: {out['batched']['ms_per_step']:.3f} ms/frame, "
+                 f"speed-up {out['batched'].get('speedup', float('nan')):.2f}")
     if not args.no_adjoint:
         out["adjoint"] = adjoint_leg(fsi)
         progress(f"adjoint leg: ratio {out['adjoint']['adjoint_forward_ratio']:.2f}")
