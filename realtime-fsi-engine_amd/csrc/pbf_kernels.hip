@@ -121,22 +121,20 @@ __global__ __launch_bounds__(BS) void k_build_list(StepConst c, const uint32_t* 
     int32_t cx, cy, cz;
     cell_of(x, c.h, cx, cy, cz);
     const BucketWalk bw(cx, cy, cz, domain_of<B>(c, k));
-    // Candidates are read through a buffer resource over Q: 32-bit offsets,
-    // and the loads of a group of four that run past the end of Q return
-    // zeros (the hardware bounds check); such candidates are not valid.
-    const __amdgpu_buffer_rsrc_t qr =
-        __builtin_amdgcn_make_buffer_rsrc((void*)Q, (short)0, (int)(c.n * (uint32_t)sizeof(uint2)), 0x00020000);
-    // the records of slots s..s+3 (two 16-byte loads); an offset past the end
-    // of Q (slot -1 of an empty bucket, or the no-group offset) reads zeros
+    // the records of slots s..s+3 (two 16-byte loads at byte offset off of
+    // Q, which holds n + 4 records: every group of a bucket lies inside it;
+    // records past the bucket are not valid candidates).  A group no lane
+    // needs reads slots 0..3 (kNoGroup) and is not used.
     struct G4 {
         uint4 a, b;
     };
+    typedef uint4 uint4_a8 __attribute__((aligned(8)));
+    const char* qbytes = reinterpret_cast<const char*>(Q);
     auto loadg = [&](uint32_t off) {
-        const auto u = __builtin_amdgcn_raw_buffer_load_b128(qr, (int)off, 0, 0);
-        const auto v = __builtin_amdgcn_raw_buffer_load_b128(qr, (int)off + 16, 0, 0);
-        return G4{make_uint4(u[0], u[1], u[2], u[3]), make_uint4(v[0], v[1], v[2], v[3])};
+        return G4{*reinterpret_cast<const uint4_a8*>(qbytes + off),
+                  *reinterpret_cast<const uint4_a8*>(qbytes + off + 16u)};
     };
-    constexpr uint32_t kNoGroup = 0xffffffe0u;
+    constexpr uint32_t kNoGroup = 0u;
     uint32_t tail = 0, done = 0, ncand = 0;  // entries appended / written out (a multiple of four)
     // writes entries done..done+3 (the last group may hold stale slots past
     // the count: they are never read as entries)
@@ -166,27 +164,33 @@ __global__ __launch_bounds__(BS) void k_build_list(StepConst c, const uint32_t* 
     // The loads of a bucket's first three groups of four (12 slots, most
     // buckets) are issued together, so a bucket costs one load latency, not
     // one per group; a group no lane of the wave needs is not tested.  The
-    // cell-table entry of a bucket is read two buckets ahead.  An empty bucket
-    // is (-1, -1): length 0.
+    // next bucket's cell-table entry is read after them and lands while this
+    // bucket's candidates are tested: loads complete in issue order, so the
+    // tests can wait for the groups and leave the entry in flight.  An empty
+    // bucket is (-1, -1): length 0.
     int2 se = cells[bw.bucket(0)];
-    int2 nx = cells[bw.bucket(1)];
 #pragma unroll 1
     for (int q = 0; q < 27; ++q) {
-        int2 nn = make_int2(-1, -1);
-        if (q < 25) nn = cells[bw.bucket(q + 2)];
         const uint32_t s0 = (uint32_t)se.x;
         const int32_t len = se.y - se.x;  // (-1, -1): 0
         const uint32_t o = s0 * (uint32_t)sizeof(uint2);
         const G4 g0 = loadg(0 < len ? o : kNoGroup);
         const G4 g1 = loadg(4 < len ? o + 32u : kNoGroup);
         const G4 g2 = loadg(8 < len ? o + 64u : kNoGroup);
-        if (__any(0 < len)) visit4(s0, len, g0);
+        // a compiler barrier keeps the entry's load after the groups'; it is
+        // issued on every iteration (the last re-reads bucket 26, unused) so
+        // that every path has the same loads in flight
+        asm volatile("" ::: "memory");
+        se = cells[bw.bucket(q < 26 ? q + 1 : 26)];
+        visit4(s0, len, g0);  // nearly every bucket has a lane that needs it
         if (__any(4 < len)) visit4(s0 + 4u, len - 4, g1);
         if (__any(8 < len)) visit4(s0 + 8u, len - 8, g2);
 #pragma unroll 1
         for (int32_t r = 12; r < len; r += 4) visit4(s0 + (uint32_t)r, len - r, loadg(o + (uint32_t)r * 8u));
-        se = nx;
-        nx = nn;
+        // the groups count as used here: the registers of a group no lane
+        // needed are not reused next iteration while its load is in flight
+        // (that reuse would make the next iteration wait for every load)
+        asm volatile("" ::"v"(g2.b.w));
     }
 #pragma unroll 1
     while (tail > done) copy4();
