@@ -142,6 +142,16 @@ def batch_config(fsi, box=(1.0, 1.0, 1.0), hash_size=1 << 17):
                            bounds_min=(0.0, 0.0, 0.0), bounds_max=box, hash_size=hash_size)
 
 
+def trial_summary(trials):
+    """Fastest autotune trial per gather form (ms of its timed frames), and how many settings were timed."""
+    best = {}
+    for t in trials:
+        g = str(t["gather"])
+        if g not in best or t["ms"] < best[g]["ms"]:
+            best[g] = {k: (round(v, 3) if k == "ms" else v) for k, v in t.items()}
+    return {"settings_timed": len(trials), "fastest_by_gather_form": best}
+
+
 def timed_runs(step, sync, steps, runs):
     """Wall times of `runs` consecutive windows of `steps` frames each."""
     walls = []
@@ -169,8 +179,10 @@ def batched_leg(fsi, steps, warmup, runs=5, autotune=True, graphs=True, loop=Tru
 
     b = fsi.Batch(batch_config(fsi))
     b.set_scenes(scenes)
+    tuned = None
     if autotune:
         b.autotune(1)
+        tuned = trial_summary(b.autotune_trials())
     knobs = b.knobs()
     b.set_graph(graphs)
     b.step(warmup)
@@ -236,6 +248,7 @@ def batched_leg(fsi, steps, warmup, runs=5, autotune=True, graphs=True, loop=Tru
         "speedup_vs_one_stream_loop": wall_q / wall_b,
         "autotuned" if autotune else "knobs": {"gather": knobs["gather"], "sort_digit_bits": knobs["sort_digit_bits"],
                                                "block": knobs["block"], "tile": knobs["tile"], "order": knobs["order"]},
+        "autotune_trials": tuned,
         "graph": {"batch": graphs, "loop": graphs, "loop_one_stream": graphs},
         "identical_to_loop": bool(same and same_q),
     }
@@ -555,13 +568,17 @@ def main():
     s.set_particles(parts)
     s.set_sdf(sdf, 0.025, False)
     fixed = {"gather": args.gather, "order": args.order, "block": args.block, "tile": args.tile}
+    tuned = None
     if all(v is None for v in fixed.values()) and not args.no_autotune:
         s.autotune(2)  # times every knob setting from this state, which it restores
+        tuned = trial_summary(s.autotune_trials())
     for name, v in fixed.items():
         if v is not None:
             s.set_tuning(fsi.KNOBS[name], v)
     knobs = s.knobs()
     knobs["autotuned"] = all(v is None for v in fixed.values()) and not args.no_autotune
+    if tuned:
+        knobs["autotune_trials"] = tuned
     s.step(args.warmup)
     s.sync()
     # timed region: events bracket the launches of the dominant pass (lambda)

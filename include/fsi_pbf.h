@@ -309,6 +309,18 @@ int fsi_pbf_get_tuning(fsi_pbf* h, int knob, int* value);
  * (src/fluid/PBFluids.cpp:330-332,353); no counterpart. */
 int fsi_pbf_autotune(fsi_pbf* h, uint32_t frames, int* gather, int* digit_bits);
 
+/* One timed setting of the last fsi_pbf_autotune / fsi_batch_autotune call:
+ * the knobs (FSI_TUNE_* values) and the milliseconds its frames took.
+ * No counterpart in the reference. */
+typedef struct fsi_tune_trial {
+    int32_t gather, sort_digit_bits, order, block, tile;
+    float ms;
+} fsi_tune_trial;
+/* The settings the last autotune timed, in the order it timed them (a
+ * form-2 setting appears only where the scene's box could be binned by
+ * tiles): up to cap records into out, their total count in *n. */
+int fsi_pbf_read_autotune(fsi_pbf* h, fsi_tune_trial* out, uint32_t cap, uint32_t* n);
+
 /* Replay each frame from a HIP graph captured on first use (re-captured when
  * parameters, buffers or knobs change).  Per-pass timing is not recorded in
  * this mode.  No counterpart in the reference. */
@@ -367,6 +379,7 @@ int fsi_batch_set_graph(fsi_batch* b, int enable);
 int fsi_batch_set_tuning(fsi_batch* b, int knob, int value);
 int fsi_batch_get_tuning(fsi_batch* b, int knob, int* value);
 int fsi_batch_autotune(fsi_batch* b, uint32_t frames, int* gather, int* digit_bits);
+int fsi_batch_read_autotune(fsi_batch* b, fsi_tune_trial* out, uint32_t cap, uint32_t* n);
 int fsi_batch_set_timing(fsi_batch* b, int enable);
 int fsi_batch_set_timing_passes(fsi_batch* b, uint32_t mask);
 int fsi_batch_timing_reset(fsi_batch* b);

@@ -341,6 +341,11 @@ int fsi_batch_autotune(fsi_batch* b, uint32_t frames, int* gather, int* digit_bi
     return core_autotune(b->core, rebin, frame_plan(b), frames, gather, digit_bits);
 }
 
+int fsi_batch_read_autotune(fsi_batch* b, fsi_tune_trial* out, uint32_t cap, uint32_t* n) {
+    if (!b) return fail(FSI_ERR_INVALID_ARG, "handle is NULL");
+    return core_read_autotune(b->core, out, cap, n);
+}
+
 int fsi_batch_set_timing(fsi_batch* b, int enable) {
     if (!b) return fail(FSI_ERR_INVALID_ARG, "handle is NULL");
     b->core.timing = enable != 0;

@@ -644,6 +644,11 @@ int fsi_pbf_autotune(fsi_pbf* h, uint32_t frames, int* gather, int* digit_bits) 
     return core_autotune(h->core, rebin, frame_plan(h), frames, gather, digit_bits);
 }
 
+int fsi_pbf_read_autotune(fsi_pbf* h, fsi_tune_trial* out, uint32_t cap, uint32_t* n) {
+    if (!h) return fail(FSI_ERR_INVALID_ARG, "handle is NULL");
+    return core_read_autotune(h->core, out, cap, n);
+}
+
 int fsi_pbf_set_graph(fsi_pbf* h, int enable) {
     int rc = bind(h);
     if (rc) return rc;

@@ -83,6 +83,7 @@ struct Core {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     std::vector<unsigned char> graph_key;  // StepConst + plan + knobs the graph was captured with
+    std::vector<fsi_tune_trial> trials;    // the settings the last autotune timed
 };
 
 // Binds the handle's device and creates its stream.
@@ -112,6 +113,8 @@ using Rebin = std::function<int(bool tiles, StepConst& out)>;
 // keeps the fastest, leaves the binning for it, and returns its gather form
 // and digit width (the rest through core_get_tuning).
 int core_autotune(Core& c, const Rebin& rebin, const FramePlan& plan, uint32_t frames, int* gather, int* digit_bits);
+// The trials of the last autotune (up to cap into out; *n = their count).
+int core_read_autotune(const Core& c, fsi_tune_trial* out, uint32_t cap, uint32_t* n);
 
 // Morton binning of one scene's processing order: cover its bounds (plus two
 // cells) with at most 2^axis_bits bins per axis, numbered from bin_base, each

@@ -503,6 +503,7 @@ int core_autotune(Core& c, const Rebin& rebin, const FramePlan& plan, uint32_t f
                 for (int t = 0; t < kTileShapes; ++t) trials.push_back({2, b, c.d.block, t, o});
             }
         float best_ms = 1e30f;
+        c.trials.clear();
         for (const Knobs& t : trials) {
             c.d.variant = t.g;
             c.sort_digit_bits = t.bits;
@@ -524,6 +525,7 @@ int core_autotune(Core& c, const Rebin& rebin, const FramePlan& plan, uint32_t f
             }
             float ms = 0.0f;
             (void)hipEventElapsedTime(&ms, scope.ea, scope.eb);
+            c.trials.push_back(fsi_tune_trial{t.g, t.bits, t.order, t.block, t.tile, ms});
             if (ms < best_ms) {
                 best_ms = ms;
                 best = t;
@@ -543,6 +545,13 @@ int core_autotune(Core& c, const Rebin& rebin, const FramePlan& plan, uint32_t f
     if (gather) *gather = best.g;
     if (digit_bits) *digit_bits = best.bits;
     return rc;
+}
+
+int core_read_autotune(const Core& c, fsi_tune_trial* out, uint32_t cap, uint32_t* n) {
+    if (!n || (cap > 0 && !out)) return fail(FSI_ERR_INVALID_ARG, "NULL argument");
+    *n = (uint32_t)c.trials.size();
+    for (uint32_t i = 0; i < cap && i < *n; ++i) out[i] = c.trials[i];
+    return FSI_OK;
 }
 
 int core_drain_timing(Core& c) {

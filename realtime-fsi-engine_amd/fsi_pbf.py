@@ -27,6 +27,19 @@ FSI_OK = 0
 TUNE_GATHER, TUNE_SORT_DIGIT_BITS, TUNE_ORDER, TUNE_BLOCK, TUNE_TILE = 0, 1, 2, 3, 4
 KNOBS = {"gather": TUNE_GATHER, "sort_digit_bits": TUNE_SORT_DIGIT_BITS, "order": TUNE_ORDER,
          "block": TUNE_BLOCK, "tile": TUNE_TILE}
+
+
+class CTuneTrial(C.Structure):
+    _fields_ = [("gather", C.c_int32), ("sort_digit_bits", C.c_int32), ("order", C.c_int32), ("block", C.c_int32),
+                ("tile", C.c_int32), ("ms", C.c_float)]
+
+
+def _read_trials(fn, h):
+    n = C.c_uint32(0)
+    _check(fn(h, None, 0, C.byref(n)))
+    arr = (CTuneTrial * max(1, n.value))()
+    _check(fn(h, C.cast(arr, C.c_void_p), n.value, C.byref(n)))
+    return [{f: getattr(arr[i], f) for f, _ in CTuneTrial._fields_} for i in range(n.value)]
 PASS_NAMES = ["predict", "sort", "tables", "lambda", "deltap", "integrate", "vorticity", "lod"]
 
 
@@ -199,6 +212,7 @@ def load() -> C.CDLL:
         "fsi_pbf_set_tuning": ([P, C.c_int, C.c_int], C.c_int),
         "fsi_pbf_get_tuning": ([P, C.c_int, C.POINTER(C.c_int)], C.c_int),
         "fsi_pbf_autotune": ([P, C.c_uint32, C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
+        "fsi_pbf_read_autotune": ([P, P, C.c_uint32, u32p], C.c_int),
         "fsi_pbf_set_graph": ([P, C.c_int], C.c_int),
         "fsi_batch_create": ([C.POINTER(CFluidConfig), C.POINTER(P)], C.c_int),
         "fsi_batch_destroy": ([P], C.c_int),
@@ -211,6 +225,7 @@ def load() -> C.CDLL:
         "fsi_batch_set_tuning": ([P, C.c_int, C.c_int], C.c_int),
         "fsi_batch_get_tuning": ([P, C.c_int, C.POINTER(C.c_int)], C.c_int),
         "fsi_batch_autotune": ([P, C.c_uint32, C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
+        "fsi_batch_read_autotune": ([P, P, C.c_uint32, u32p], C.c_int),
         "fsi_batch_set_timing": ([P, C.c_int], C.c_int),
         "fsi_batch_timing_reset": ([P], C.c_int),
         "fsi_batch_read_timing": ([P, C.POINTER(CTiming)], C.c_int),
@@ -681,6 +696,11 @@ class Solver:
         """Every launch-shape knob's current value."""
         return {name: self.get_tuning(k) for name, k in KNOBS.items()}
 
+    def autotune_trials(self) -> list:
+        """The settings the last autotune timed, with their milliseconds."""
+        return _read_trials(self.lib.fsi_pbf_read_autotune if isinstance(self, Solver)
+                            else self.lib.fsi_batch_read_autotune, self.h)
+
     def set_timing(self, on: bool):
         _check(self.lib.fsi_pbf_set_timing(self.h, 1 if on else 0))
 
@@ -835,6 +855,11 @@ class Batch:
     def knobs(self) -> dict:
         """Every launch-shape knob's current value."""
         return {name: self.get_tuning(k) for name, k in KNOBS.items()}
+
+    def autotune_trials(self) -> list:
+        """The settings the last autotune timed, with their milliseconds."""
+        return _read_trials(self.lib.fsi_pbf_read_autotune if isinstance(self, Solver)
+                            else self.lib.fsi_batch_read_autotune, self.h)
 
     def set_timing(self, on: bool):
         _check(self.lib.fsi_batch_set_timing(self.h, 1 if on else 0))

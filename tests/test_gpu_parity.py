@@ -328,6 +328,15 @@ def test_autotune_restores_state(gpu):
         k = s.knobs()
         assert (k["gather"], k["sort_digit_bits"]) == (gather, bits)
         assert gather in (0, 1, 2) and bits in (6, 8, 10) and k["block"] in (128, 256) and k["tile"] in (0, 1, 2)
+        # it timed every setting, the tiled form at each tile shape too (its
+        # binning is redone for it from the default form), and kept the fastest
+        trials = s.autotune_trials()
+        assert len(trials) == 2 * 3 * (2 * 2 + 3), len(trials)
+        assert {(t["order"], t["sort_digit_bits"], t["tile"]) for t in trials if t["gather"] == 2} == \
+            {(o, b, t) for o in (1, 2) for b in (6, 8, 10) for t in (0, 1, 2)}
+        assert all(t["ms"] > 0.0 for t in trials)
+        best = min(trials, key=lambda t: t["ms"])
+        assert (best["gather"], best["sort_digit_bits"], best["order"]) == (gather, bits, k["order"])
         s.step(2)
         assert np.array_equal(s.read_particles(), want), k
 
