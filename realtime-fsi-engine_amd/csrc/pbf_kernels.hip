@@ -442,10 +442,10 @@ __global__ __launch_bounds__(kBlock) void k_relabel(StepConst c, const uint32_t*
     const float4 x = xstar[id];
     P[m] = x;
     Q[m] = quantize(xyz(x), c.qscale);
-    PL[m] = x;
+    if (PL != nullptr) PL[m] = x;  // lambda = 0 (read before the lambda pass writes it only under APBF)
     vs[m] = vel[id];
     mperm[m] = id;
-    bin_m[m] = b;
+    if (bin_m != nullptr) bin_m[m] = b;  // read by the tiled gathers only
     if (k == head) mcells[key] = make_int2((int32_t)m, (int32_t)(m + (uint32_t)(se.y - se.x)));
 }
 
@@ -1184,7 +1184,8 @@ hipError_t launch_reorder(const StepConst& c, DeviceState& d, hipStream_t st) {
         if (e != hipSuccess) return e;
 #define L(B)                                                                                                    \
     hipLaunchKernelGGL((k_relabel<B>), grid_for(c.n), dim3(kBlock), 0, st, c, d.skeys, d.perm, d.xstar, d.vel,  \
-                       d.cells, d.bins, d.bin_of, d.bin_rank, d.P, d.Q, d.PL, d.vs, d.mperm, d.bin_m, d.mcells);
+                       d.cells, d.bins, d.bin_of, d.bin_rank, d.P, d.Q, c.enable_apbf ? d.PL : nullptr, d.vs,   \
+                       d.mperm, tiled_form(d) ? d.bin_m : nullptr, d.mcells);
         FSI_B_SWITCH(d.batched, L)
 #undef L
     } else {
