@@ -66,14 +66,15 @@ __device__ __forceinline__ double qsum(double v) {
     return v;
 }
 
-// Order-independent sums over a transposed row.  The atomic appends of
-// k_transpose fill a row in whatever order they run, so its terms are added
-// in a way that does not depend on the order: every term is rounded to a
-// multiple of 2^(E - 55), E the binary exponent of the row's largest |term|,
-// and the multiples are added as 64-bit integers (exact, associative).  A
-// row holds at most kInCap = 128 terms of at most 2^55 units: no overflow.
-// The rounding error, 2^-56 of the largest term per term, is below that of a
-// sequential FP64 sum.  The same row always gives the same bits.
+// Order-independent sums over a transposed row.  The atomic appends of the
+// list builders (quad_list) fill a row in whatever order they run, so its
+// terms are added in a way that does not depend on the order: every term is
+// rounded to a multiple of 2^(E - 55), E the binary exponent of the row's
+// largest |term|, and the multiples are added as 64-bit integers (exact,
+// associative).  A row holds at most kInCap = 128 terms of at most 2^55
+// units: no overflow.  The rounding error, 2^-56 of the largest term per
+// term, is below that of a sequential FP64 sum.  The same row always gives
+// the same bits.
 struct RowSum {
     int e = 0;  // exponent of the row's largest |term| (0 when all are zero)
     __device__ __forceinline__ void set(double maxabs) {
@@ -220,17 +221,33 @@ __device__ __forceinline__ void block_max(const uint32_t (&v)[C], uint32_t* cons
 // many waves in flight as one lane per slot, and the candidate loads of a
 // quad are contiguous.  inside(xk) is the support test r^2 <= h^2.  Returns
 // the slot's neighbour count (entries past kEdgeCap are not stored).
+//
+// The transposed list is filled in the same pass: every stored pair e =
+// r * n + i is appended to row j of tin (an atomic on the row's counter
+// tcnt[j], which the frame cleared before its lists were built); a row longer
+// than kInCap is refused (flag bit 2).  The rows fill in no fixed order, which
+// the sums over them do not depend on (RowSum).  indeg is raised to the
+// largest row length this lane saw.
 template <class T4, class Inside>
 __device__ __forceinline__ uint32_t quad_list(const T4* __restrict__ X, const int2* __restrict__ cells,
                                               const BucketWalk& bw, uint32_t n, uint32_t i, uint32_t lane,
-                                              uint32_t* __restrict__ nbr, Inside&& inside) {
+                                              uint32_t* __restrict__ nbr, uint32_t* __restrict__ tin,
+                                              uint32_t* __restrict__ tcnt, uint32_t* __restrict__ overflow,
+                                              uint32_t& indeg, Inside&& inside) {
     const uint32_t qbase = __lane_id() & ~(kQ - 1u), below = (1u << lane) - 1u;
     uint32_t m = 0;
     auto place = [&](bool in, uint32_t s) {
         const uint32_t qm = (uint32_t)(__ballot(in) >> qbase) & 0xfu;
         if (in) {
             const uint32_t r = m + (uint32_t)__popc(qm & below);
-            if (r < kEdgeCap) nbr[(size_t)r * n + i] = s;
+            if (r < kEdgeCap) {
+                const uint32_t e = r * n + i;
+                nbr[e] = s;
+                const uint32_t q = atomicAdd(&tcnt[s], 1u);
+                if (q < kInCap) tin[(size_t)q * n + s] = e;
+                else atomicOr(overflow, 2u);  // rare: the list is refused
+                indeg = max(indeg, q + 1u);
+            }
         }
         m += (uint32_t)__popc(qm);
     };
@@ -253,11 +270,12 @@ __device__ __forceinline__ uint32_t quad_list(const T4* __restrict__ X, const in
 }
 
 // Stores the count of a quad-built list (lane 0), flags a list longer than
-// kEdgeCap and keeps the largest count.  Every thread of the block calls it;
-// m is 0 for lanes past the last slot.
-__device__ __forceinline__ void store_count(uint32_t m, bool live, uint32_t i, uint32_t lane,
+// kEdgeCap and keeps the largest count and the largest transposed row
+// (indeg).  Every thread of the block calls it; m and indeg are 0 for lanes
+// past the last slot.
+__device__ __forceinline__ void store_count(uint32_t m, uint32_t indeg, bool live, uint32_t i, uint32_t lane,
                                             uint32_t* __restrict__ cnt, uint32_t* __restrict__ overflow) {
-    block_max<1>({m}, {overflow + kOutMax});
+    block_max<2>({m, indeg}, {overflow + kOutMax, overflow + kInMax});
     if (live && lane == 0u) {
         if (m > kEdgeCap) {
             atomicOr(overflow, 1u);
@@ -269,44 +287,23 @@ __device__ __forceinline__ void store_count(uint32_t m, bool live, uint32_t i, u
 
 __global__ __launch_bounds__(kAB) void k_edges(StepConst c, const float4* __restrict__ X,
                                                const int2* __restrict__ cells, uint32_t* __restrict__ nbr,
-                                               uint32_t* __restrict__ cnt, uint32_t* __restrict__ overflow) {
+                                               uint32_t* __restrict__ cnt, uint32_t* __restrict__ tin,
+                                               uint32_t* __restrict__ tcnt, uint32_t* __restrict__ overflow) {
     const uint32_t g = agid(), i = g / kQ, lane = g % kQ;
     const bool live = i < c.n;
-    uint32_t m = 0;
+    uint32_t m = 0, indeg = 0;
     if (live) {
         const V3 x = xyz(X[i]);
         const float h2 = c.h * c.h;
         int32_t cx, cy, cz;
         cell_of(x, c.h, cx, cy, cz);
-        m = quad_list(X, cells, BucketWalk(cx, cy, cz, c.dom), c.n, i, lane, nbr, [&](const float4& xk) {
-            const V3 r = sub(x, xyz(xk));
-            return dot(r, r) <= h2;
-        });
+        m = quad_list(X, cells, BucketWalk(cx, cy, cz, c.dom), c.n, i, lane, nbr, tin, tcnt, overflow, indeg,
+                      [&](const float4& xk) {
+                          const V3 r = sub(x, xyz(xk));
+                          return dot(r, r) <= h2;
+                      });
     }
-    store_count(m, live, i, lane, cnt, overflow);
-}
-
-__global__ __launch_bounds__(kAB) void k_transpose(uint32_t n, const uint32_t* __restrict__ nbr,
-                                                   const uint32_t* __restrict__ cnt, uint32_t* __restrict__ tin,
-                                                   uint32_t* __restrict__ tcnt, uint32_t* __restrict__ overflow) {
-    const uint32_t g = agid(), i = g / kQ, lane = g % kQ;
-    if (i >= n) return;
-    const uint32_t m = cnt[i];
-    for (uint32_t r = lane; r < m; r += kQ) {
-        const size_t e = (size_t)r * n + i;
-        const uint32_t j = nbr[e];
-        const uint32_t q = atomicAdd(&tcnt[j], 1u);
-        if (q < kInCap) tin[(size_t)q * n + j] = (uint32_t)e;
-        else atomicOr(overflow, 2u);  // rare: the list is refused
-    }
-}
-
-// The largest in-degree of a list (a statistic of the backward); rows longer
-// than kInCap were truncated by k_transpose, which flags it.
-__global__ __launch_bounds__(kAB) void k_tstats(uint32_t n, const uint32_t* __restrict__ tcnt,
-                                                uint32_t* __restrict__ flags) {
-    const uint32_t j = agid();
-    block_max<1>({j < n ? tcnt[j] : 0u}, {flags + kInMax});
+    store_count(m, indeg, live, i, lane, cnt, overflow);
 }
 
 // ---- recording forward ----------------------------------------------------
@@ -651,21 +648,23 @@ __global__ __launch_bounds__(kAB) void k_reorder64(uint32_t n, const uint32_t* _
 // Neighbour list over FP64 positions, in the reference's visit order.
 __global__ __launch_bounds__(kAB) void k_edges64(C64 k, const double4* __restrict__ X, const int2* __restrict__ cells,
                                                  uint32_t* __restrict__ nbr, uint32_t* __restrict__ cnt,
+                                                 uint32_t* __restrict__ tin, uint32_t* __restrict__ tcnt,
                                                  uint32_t* __restrict__ overflow) {
     const uint32_t g = agid(), i = g / kQ, lane = g % kQ;
     const bool live = i < k.n;
-    uint32_t m = 0;
+    uint32_t m = 0, indeg = 0;
     if (live) {
         const D3 xi = d3(X[i]);
         const double h2 = k.h * k.h;
         const int32_t cx = (int32_t)floor(xi.x / k.h), cy = (int32_t)floor(xi.y / k.h),
                       cz = (int32_t)floor(xi.z / k.h);
-        m = quad_list(X, cells, BucketWalk(cx, cy, cz, k.hash_mask, 0u), k.n, i, lane, nbr, [&](const double4& xk) {
-            const D3 r = sub(xi, d3(xk));
-            return dot(r, r) <= h2;
-        });
+        m = quad_list(X, cells, BucketWalk(cx, cy, cz, k.hash_mask, 0u), k.n, i, lane, nbr, tin, tcnt, overflow,
+                      indeg, [&](const double4& xk) {
+                          const D3 r = sub(xi, d3(xk));
+                          return dot(r, r) <= h2;
+                      });
     }
-    store_count(m, live, i, lane, cnt, overflow);
+    store_count(m, indeg, live, i, lane, cnt, overflow);
 }
 
 __global__ __launch_bounds__(kAB) void k_rec_lambda64(C64 k, const double4* __restrict__ P,
@@ -1460,16 +1459,6 @@ hipError_t join_out(fsi_adjoint* a, void* stream) {
     return hipStreamWaitEvent((hipStream_t)stream, a->ev_out, 0);
 }
 
-// The transposed list of L (its in-degree counters were cleared for the
-// frame by zero_counters).
-hipError_t finish_list(fsi_adjoint* a, const EdgeList& L) {
-    hipStream_t st = a->core.stream;
-    const uint32_t n = a->n;
-    hipLaunchKernelGGL(k_transpose, qgrid(n), dim3(kAB), 0, st, n, L.nbr, L.cnt, L.tin, L.tcnt, a->flags);
-    hipLaunchKernelGGL(k_tstats, agrid(n), dim3(kAB), 0, st, n, L.tcnt, a->flags);
-    return hipGetLastError();
-}
-
 // The in-degree counters of every list, before a frame's list builds.
 __global__ __launch_bounds__(kAB) void k_zero_tcnt(uint32_t* __restrict__ base, size_t stride, uint32_t n) {
     const uint32_t i = agid();
@@ -1484,16 +1473,18 @@ hipError_t zero_counters(fsi_adjoint* a) {
     return hipGetLastError();
 }
 
+// The list L over the positions X and its transposed list (whose row
+// counters zero_counters cleared for the frame).
 hipError_t build_list(fsi_adjoint* a, const StepConst& c, const EdgeList& L, const float4* X) {
     hipLaunchKernelGGL(k_edges, qgrid(a->n), dim3(kAB), 0, a->core.stream, c, X, a->core.d.cells, L.nbr, L.cnt,
-                       a->flags);
-    return finish_list(a, L);
+                       L.tin, L.tcnt, a->flags);
+    return hipGetLastError();
 }
 
 hipError_t build_list(fsi_adjoint* a, const C64& k, const EdgeList& L, const double4* X) {
     hipLaunchKernelGGL(k_edges64, qgrid(a->n), dim3(kAB), 0, a->core.stream, k, X, a->core.d.cells, L.nbr, L.cnt,
-                       a->flags);
-    return finish_list(a, L);
+                       L.tin, L.tcnt, a->flags);
+    return hipGetLastError();
 }
 
 const EdgeList& list_of(const fsi_adjoint* a, uint32_t s, uint32_t k) { return a->lists[(size_t)s * (a->I + 1) + k]; }
