@@ -86,21 +86,41 @@ __device__ __forceinline__ const uint4* wave_run(const uint32_t* list, uint32_t 
 
 // Per-lane staging ring of the list builder (entries; a power of two and a
 // multiple of four).
-constexpr uint32_t kLRing = 16;
+constexpr uint32_t kLRing = 16;  // holds three groups and a partly filled one
+// The list builder's bucket table: per lane, the 27 buckets of the walk as
+// start << L | length, L = the leading zero bits of n (at least 5: a start
+// is below n < 2^27), and an empty entry after them.  A bucket longer than
+// 2^L - 1 slots (never at 1M particles: L = 11) sends the particle to the
+// bucket scan of the walks.
+constexpr uint32_t kBucketEntries = 28;
 
 // The neighbour-list builder of the gathers (the first of their two
 // kernels).  Thread t takes the particle in slot order[t], whose position is
-// X: it scans the 27 buckets around it as scan_buckets does and gives every
+// X: it scans the 27 buckets around it in scan_buckets' order and gives every
 // candidate the quantised pre-test (qdist2 <= qpass, see pbf_device.h); the
 // slots that pass are its list, in visit order (the particle's own slot
 // passes, r = 0, and so may a candidate just outside the support: the walks
-// give every entry the exact test).  Slots are staged in a per-lane LDS ring
-// and written to HBM four at a time (the layout above), so every store covers whole
-// lines.  list_count[t] is the number of entries, also when it exceeds
-// kNbrCap (then only the first kNbrCap are kept and the walks scan the
-// buckets instead).  With lod, an APBF-inactive particle builds no list.
-// With TAPS, cand_out[k] counts the candidates other than the particle
-// itself, as the reference visits them.  X and Q hold n slots (n < 2^27).
+// give every entry the exact test).
+//
+// First the lane reads the cell-table entries of its 27 buckets into its
+// column of an LDS table.  Then it streams through them on its own: every
+// iteration tests the next four slots of its current bucket and moves to
+// the next bucket when this one is done, so the lanes of a wave run about as
+// many iterations as their longest bucket sequence needs.  (A walk in step
+// over the 27 positions runs, at each position, as many groups as the
+// longest bucket of the wave's lanes there: the lanes sit in several cells
+// whose buckets differ in length, and at 1M particles that is half as many
+// groups again.)  An iteration loads three groups of four of the lane's
+// stream before it tests any, so a wave waits once per twelve slots.
+//
+// Slots are staged in a per-lane LDS ring and written to HBM four at a time
+// (the layout above), so every store covers whole lines.  list_count[t] is
+// the number of entries, also when it exceeds kNbrCap (then only the first
+// kNbrCap are kept and the walks scan the buckets instead; a bucket longer
+// than the table holds gives kNbrCap + 1 without a list).  With lod, an
+// APBF-inactive particle builds no list.  With TAPS, cand_out[k] counts the
+// candidates other than the particle itself, as the reference visits them.
+// X and Q hold n slots (n < 2^27).
 template <bool TAPS, bool B, int BS>
 __global__ __launch_bounds__(BS) void k_build_list(StepConst c, const uint32_t* __restrict__ order,
                                                        const float4* __restrict__ X, const uint2* __restrict__ Q,
@@ -115,27 +135,46 @@ __global__ __launch_bounds__(BS) void k_build_list(StepConst c, const uint32_t* 
     const V3 x = xyz(X[k]);
     const uint2 qi = quantize(x, c.qscale);
     const int32_t qpass = qpass_of(x, c.qscale);
-    __shared__ uint32_t stage[kLRing * BS];
-    uint32_t* ring = &stage[threadIdx.x];
+    // the staging rings first (LDS offset 0, so that a ring address is an
+    // OR of the entry and lane bits), then the bucket tables
+    __shared__ uint32_t lds[(kLRing + kBucketEntries) * BS];
+    uint32_t* ring = &lds[threadIdx.x];
+    uint32_t* mine = &lds[kLRing * BS + threadIdx.x];
     uint4* run = wave_run(list, t);
     int32_t cx, cy, cz;
     cell_of(x, c.h, cx, cy, cz);
     const BucketWalk bw(cx, cy, cz, domain_of<B>(c, k));
-    // the records of slots s..s+3 (two 16-byte loads at byte offset off of
-    // Q, which holds n + 4 records: every group of a bucket lies inside it;
-    // records past the bucket are not valid candidates).  A group no lane
-    // needs reads slots 0..3 (kNoGroup) and is not used.
+    // the bucket table (an empty bucket is (-1, -1) in the cell table)
+    const uint32_t lbits = (uint32_t)__builtin_clz(c.n), lmax = (1u << lbits) - 1u;  // L and 2^L - 1
+    uint32_t ncand = 0;
+    bool longb = false;
+#pragma unroll 1
+    for (int q0 = 0; q0 < 27; q0 += 9) {
+        int2 se[9];
+#pragma unroll
+        for (int j = 0; j < 9; ++j) se[j] = cells[bw.bucket(q0 + j)];
+#pragma unroll
+        for (int j = 0; j < 9; ++j) {
+            const uint32_t len = (uint32_t)(se[j].y - se[j].x);
+            if (TAPS) ncand += len - ((k - (uint32_t)se[j].x) < len ? 1u : 0u);
+            longb = longb || len > lmax;
+            mine[(q0 + j) * BS] = len != 0u ? ((uint32_t)se[j].x << lbits) | (len & lmax) : 0u;
+        }
+    }
+    mine[27 * BS] = 0u;
+    // the records of slots s..s+3 (two 16-byte loads of Q, which holds n + 4
+    // records: every group of a bucket lies inside it; records past the
+    // bucket are not valid candidates)
     struct G4 {
         uint4 a, b;
     };
     typedef uint4 uint4_a8 __attribute__((aligned(8)));
     const char* qbytes = reinterpret_cast<const char*>(Q);
-    auto loadg = [&](uint32_t off) {
-        return G4{*reinterpret_cast<const uint4_a8*>(qbytes + off),
-                  *reinterpret_cast<const uint4_a8*>(qbytes + off + 16u)};
+    auto loadg = [&](uint32_t s) {
+        const char* p = qbytes + (size_t)s * sizeof(uint2);
+        return G4{*reinterpret_cast<const uint4_a8*>(p), *reinterpret_cast<const uint4_a8*>(p + 16)};
     };
-    constexpr uint32_t kNoGroup = 0u;
-    uint32_t tail = 0, done = 0, ncand = 0;  // entries appended / written out (a multiple of four)
+    uint32_t tail = 0, done = 0;  // entries appended / written out (a multiple of four)
     // writes entries done..done+3 (the last group may hold stale slots past
     // the count: they are never read as entries)
     auto copy4 = [&]() {
@@ -147,54 +186,57 @@ __global__ __launch_bounds__(BS) void k_build_list(StepConst c, const uint32_t* 
     // branch-free append: the slot is always written and the tail advances
     // only for a valid slot that passes the pre-test
     auto visit = [&](uint32_t s, uint2 qs, bool valid) {
-        if (TAPS) ncand += (valid && s != k) ? 1u : 0u;
         ring[(tail & (kLRing - 1u)) * BS] = s;
-        // bitwise, so that the test is not made a branch (which would sink
-        // the group's loads into it)
+        // bitwise, so that the test is not made a branch
         tail += (uint32_t)(qdist2(qi, qs) <= qpass) & (uint32_t)valid;
     };
-    // slots s..s+3 of a bucket that has rem (may be <= 0) slots from s on
-    auto visit4 = [&](uint32_t s, int32_t rem, const G4& g) {
-        if (tail - done > kLRing - 4u) copy4();  // room for four more
-        visit(s, make_uint2(g.a.x, g.a.y), 0 < rem);
-        visit(s + 1, make_uint2(g.a.z, g.a.w), 1 < rem);
-        visit(s + 2, make_uint2(g.b.x, g.b.y), 2 < rem);
-        visit(s + 3, make_uint2(g.b.z, g.b.w), 3 < rem);
+    // the stream: entry p of the table, slots s.. of its bucket (rem left,
+    // may be <= 0); p = 27 is the empty entry after the last bucket.  An
+    // iteration tests three groups of four of the stream, A at (s, rem, p),
+    // B and C after it (each the next four slots of the previous group's
+    // bucket or the first four of the next bucket).  All three are loaded
+    // at the top of the iteration, and the table entries the next
+    // iteration needs are read while they are tested.
+    auto entry = [&](uint32_t q) { return mine[min(q, 27u) * BS]; };
+    // the group after (s, rem, p), given e = the entry after p
+    auto after = [&](uint32_t s0, int32_t r0, uint32_t p0, uint32_t e, uint32_t& s1, int32_t& r1, uint32_t& p1) {
+        const bool same = r0 > 4;
+        s1 = same ? s0 + 4u : e >> lbits;
+        r1 = same ? r0 - 4 : (int32_t)(e & lmax);
+        p1 = same ? p0 : min(p0 + 1u, 27u);
     };
-    // The loads of a bucket's first three groups of four (12 slots, most
-    // buckets) are issued together, so a bucket costs one load latency, not
-    // one per group; a group no lane of the wave needs is not tested.  The
-    // next bucket's cell-table entry is read after them and lands while this
-    // bucket's candidates are tested: loads complete in issue order, so the
-    // tests can wait for the groups and leave the entry in flight.  An empty
-    // bucket is (-1, -1): length 0.
-    int2 se = cells[bw.bucket(0)];
+    uint32_t p = longb ? 27u : 0u;
+    const uint32_t e0 = entry(p);
+    uint32_t s = e0 >> lbits, sB, pB, sC, pC;
+    int32_t rem = (int32_t)(e0 & lmax), remB, remC;
+    after(s, rem, p, entry(p + 1u), sB, remB, pB);
+    after(sB, remB, pB, entry(pB + 1u), sC, remC, pC);
+    auto test4 = [&](const G4& g, uint32_t s0, int32_t r0) {
+        visit(s0, make_uint2(g.a.x, g.a.y), 0 < r0);
+        visit(s0 + 1, make_uint2(g.a.z, g.a.w), 1 < r0);
+        visit(s0 + 2, make_uint2(g.b.x, g.b.y), 2 < r0);
+        visit(s0 + 3, make_uint2(g.b.z, g.b.w), 3 < r0);
+    };
 #pragma unroll 1
-    for (int q = 0; q < 27; ++q) {
-        const uint32_t s0 = (uint32_t)se.x;
-        const int32_t len = se.y - se.x;  // (-1, -1): 0
-        const uint32_t o = s0 * (uint32_t)sizeof(uint2);
-        const G4 g0 = loadg(0 < len ? o : kNoGroup);
-        const G4 g1 = loadg(4 < len ? o + 32u : kNoGroup);
-        const G4 g2 = loadg(8 < len ? o + 64u : kNoGroup);
-        // a compiler barrier keeps the entry's load after the groups'; it is
-        // issued on every iteration (the last re-reads bucket 26, unused) so
-        // that every path has the same loads in flight
-        asm volatile("" ::: "memory");
-        se = cells[bw.bucket(q < 26 ? q + 1 : 26)];
-        visit4(s0, len, g0);  // nearly every bucket has a lane that needs it
-        if (__any(4 < len)) visit4(s0 + 4u, len - 4, g1);
-        if (__any(8 < len)) visit4(s0 + 8u, len - 8, g2);
-#pragma unroll 1
-        for (int32_t r = 12; r < len; r += 4) visit4(s0 + (uint32_t)r, len - r, loadg(o + (uint32_t)r * 8u));
-        // the groups count as used here: the registers of a group no lane
-        // needed are not reused next iteration while its load is in flight
-        // (that reuse would make the next iteration wait for every load)
-        asm volatile("" ::"v"(g2.b.w));
+    while (__any(p < 27u || rem > 0)) {
+        const G4 ga = loadg(rem > 0 ? s : 0u), gb = loadg(remB > 0 ? sB : 0u), gc = loadg(remC > 0 ? sC : 0u);
+        // the entries after pC (the next three groups start at or after it)
+        const uint32_t f1 = entry(pC + 1u), f2 = entry(pC + 2u), f3 = entry(pC + 3u);
+        if (tail - done > kLRing - 12u) copy4();  // room for twelve more
+        if (tail - done > kLRing - 12u) copy4();
+        if (tail - done > kLRing - 12u) copy4();
+        test4(ga, s, rem);
+        test4(gb, sB, remB);
+        test4(gc, sC, remC);
+        // the next A follows C, the next B follows it, the next C follows
+        // that; each takes the entry after its predecessor's (pC + 1, 2, 3)
+        after(sC, remC, pC, f1, s, rem, p);
+        after(s, rem, p, p == pC ? f1 : f2, sB, remB, pB);
+        after(sB, remB, pB, pB == pC ? f1 : (pB == pC + 1u ? f2 : f3), sC, remC, pC);
     }
 #pragma unroll 1
     while (tail > done) copy4();
-    list_count[t] = tail;
+    list_count[t] = longb ? (uint32_t)kNbrCap + 1u : tail;
     if (TAPS) cand_out[k] = ncand;
 }
 
