@@ -173,6 +173,13 @@ __device__ __forceinline__ void row_sum(const uint32_t* __restrict__ tin, uint32
 __device__ __forceinline__ V3 zero3() { return mk(0.0f, 0.0f, 0.0f); }
 __device__ __forceinline__ float4 f4(V3 v, float w = 0.0f) { return make_float4(v.x, v.y, v.z, w); }
 
+// The sorted keys of a substep from its permutation: skeys[k] = keys[perm[k]].
+__global__ __launch_bounds__(kAB) void k_sorted_keys(uint32_t n, const uint32_t* __restrict__ keys,
+                                                     const uint32_t* __restrict__ perm, uint32_t* __restrict__ skeys) {
+    const uint32_t k = blockIdx.x * kAB + threadIdx.x;
+    if (k < n) skeys[k] = keys[perm[k]];
+}
+
 __global__ __launch_bounds__(kAB) void k_zero_u32(uint32_t* __restrict__ p, size_t n) {
     const size_t i = (size_t)blockIdx.x * kAB + threadIdx.x;
     if (i < n) p[i] = 0u;
@@ -1289,7 +1296,10 @@ struct fsi_adjoint {
     bool forward_done = false;
     Tape<float4> t32;
     Tape<double4> t64;
-    uint32_t* perm = nullptr;  // S x n
+    uint32_t* perm = nullptr;  // S x n: the sorted permutations of the FP64 replay
+    // frames x S x n: the sorted permutation of every substep of the FP32
+    // forward, which the replay takes instead of sorting the same keys again
+    uint32_t* fwd_perm = nullptr;
     std::vector<EdgeList> lists;  // S x (I + 1) + 1
     uint32_t* list_mem = nullptr;
     // adjoint work (FP64)
@@ -1323,6 +1333,7 @@ void free_all(fsi_adjoint* a) {
     a->t32.release();
     a->t64.release();
     dev_free(&a->perm);
+    dev_free(&a->fwd_perm);
     dev_free(&a->list_mem);
     for (double4** p : {&a->gpos, &a->gvel, &a->gxold, &a->gP, &a->gQ, &a->gown, &a->gvs, &a->gvs_own, &a->gw,
                         &a->gw_own, &a->ep, &a->ev})
@@ -1408,6 +1419,7 @@ hipError_t alloc_common(fsi_adjoint* a) {
 #define ALLOC(p, cnt) \
     if ((e = dev_alloc(&(p), (cnt))) != hipSuccess) return e;
     ALLOC(a->perm, S * n);
+    ALLOC(a->fwd_perm, (size_t)a->frames * S * n);
     const size_t nl = S * (I + 1) + 1;
     const size_t per = ((size_t)kEdgeCap + kInCap + 2) * n;
     ALLOC(a->list_mem, nl * per);
@@ -1489,24 +1501,25 @@ hipError_t build_list(fsi_adjoint* a, const C64& k, const EdgeList& L, const dou
 
 const EdgeList& list_of(const fsi_adjoint* a, uint32_t s, uint32_t k) { return a->lists[(size_t)s * (a->I + 1) + k]; }
 
-// Replays one frame of the FP32 trajectory from the state in d.pos / d.vel,
-// writing the tape.
-int record_frame(fsi_adjoint* a, const StepConst& c) {
+// Replays frame f of the FP32 trajectory from the state in d.pos / d.vel,
+// writing the tape.  The sort of each substep is the forward's (the same
+// keys of the same states): its permutation is taken from fwd_perm.
+int record_frame(fsi_adjoint* a, const StepConst& c, uint32_t f) {
     DeviceState& d = a->core.d;
     Tape<float4>& t = a->t32;
     hipStream_t st = a->core.stream;
     const size_t n = a->n;
     const uint32_t I = a->I;
-    const int key_bits = log2u(a->cfg.hash_size);
     FSI_HIP_TRY(zero_counters(a));
     for (uint32_t s = 0; s < a->S; ++s) {
         FSI_HIP_TRY(hipMemcpyAsync(t.xold + s * n, d.pos, sizeof(float4) * n, hipMemcpyDeviceToDevice, st));
         FSI_HIP_TRY(launch_predict(c, d, st));
-        FSI_HIP_TRY(radix_sort_pairs(d.keys, d.n, key_bits, a->core.sort_digit_bits, d.sort, &d.skeys, &d.perm, st));
+        FSI_HIP_TRY(hipMemcpyAsync(d.perm, a->fwd_perm + ((size_t)f * a->S + s) * n, sizeof(uint32_t) * n,
+                                   hipMemcpyDeviceToDevice, st));
+        hipLaunchKernelGGL(k_sorted_keys, agrid(a->n), dim3(kAB), 0, st, a->n, d.keys, d.perm, d.skeys);
         FSI_HIP_TRY(launch_cells(c, d, st));
         FSI_HIP_TRY(launch_reorder(c, d, st));
         float4* Ps = t.P + (size_t)s * (I + 1) * n;
-        FSI_HIP_TRY(hipMemcpyAsync(a->perm + s * n, d.perm, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, st));
         FSI_HIP_TRY(hipMemcpyAsync(t.vs + s * n, d.vs, sizeof(float4) * n, hipMemcpyDeviceToDevice, st));
         FSI_HIP_TRY(hipMemcpyAsync(Ps, d.P, sizeof(float4) * n, hipMemcpyDeviceToDevice, st));
         for (uint32_t it = 0; it < I; ++it) {
@@ -1577,9 +1590,11 @@ int record_frame64(fsi_adjoint* a, const StepConst& c, const C64& k) {
 
 // Runs the recorded frame backwards: gpos / gvel (id order) hold the adjoint
 // of the frame's final state on entry and of its initial state on exit.
-// posS / curl: the last substep's positions and curl (sorted slots).
+// perms: the sorted permutation of each substep (S x n); posS / curl: the
+// last substep's positions and curl (sorted slots).
 template <class T4>
-int reverse_frame(fsi_adjoint* a, const C64& k, const Tape<T4>& t, const T4* posS, const T4* curl) {
+int reverse_frame(fsi_adjoint* a, const C64& k, const Tape<T4>& t, const uint32_t* perms, const T4* posS,
+                  const T4* curl) {
     hipStream_t st = a->core.stream;
     const uint32_t n = a->n;
     const size_t ns = n;
@@ -1588,7 +1603,7 @@ int reverse_frame(fsi_adjoint* a, const C64& k, const Tape<T4>& t, const T4* pos
     bool have_gw = false;
     if (k.enable_vort) {
         const EdgeList& L = a->lists.back();
-        const uint32_t* perm = a->perm + (S - 1) * ns;
+        const uint32_t* perm = perms + (S - 1) * ns;
         hipLaunchKernelGGL((k_rev_vort<T4>), gq, b, 0, st, k, posS, curl, t.eta, perm, a->gvel, L.nbr, L.cnt, a->gown,
                            a->gw_own, a->ep, a->es);
         hipLaunchKernelGGL((k_rev_vort_t<T4>), gr, b, 0, st, n, curl, perm, L.tin, L.tcnt, a->ep, a->es, a->gown,
@@ -1596,7 +1611,7 @@ int reverse_frame(fsi_adjoint* a, const C64& k, const Tape<T4>& t, const T4* pos
         have_gw = true;
     }
     for (uint32_t s = S; s-- > 0;) {
-        const uint32_t* perm = a->perm + s * ns;
+        const uint32_t* perm = perms + s * ns;
         const T4* Ps = t.P + (size_t)s * (I + 1) * ns;
         const EdgeList& Li = list_of(a, s, I);
         hipLaunchKernelGGL((k_rev_integrate<T4>), gq, b, 0, st, k, t.xold + s * ns, t.vs + s * ns, Ps + I * ns,
@@ -1705,7 +1720,10 @@ int forward_impl(fsi_adjoint* a, const void* pos0, const void* vel0, bool f64_io
     for (uint32_t f = 0; f < a->frames; ++f) {
         FSI_HIP_TRY(hipMemcpyAsync(t.ck_pos + f * n, d.pos, sizeof(float4) * n, hipMemcpyDeviceToDevice, st));
         FSI_HIP_TRY(hipMemcpyAsync(t.ck_vel + f * n, d.vel, sizeof(float4) * n, hipMemcpyDeviceToDevice, st));
-        if ((rc = core_frame(core, c, plan))) return rc;
+        core.record_perm = a->fwd_perm + (size_t)f * a->S * n;
+        rc = core_frame(core, c, plan);
+        core.record_perm = nullptr;
+        if (rc) return rc;
     }
     FSI_HIP_TRY(hipMemcpyAsync(t.ck_pos + a->frames * n, d.pos, sizeof(float4) * n, hipMemcpyDeviceToDevice, st));
     FSI_HIP_TRY(hipMemcpyAsync(t.ck_vel + a->frames * n, d.vel, sizeof(float4) * n, hipMemcpyDeviceToDevice, st));
@@ -1754,15 +1772,17 @@ int backward_impl(fsi_adjoint* a, const void* grad_pos, const void* grad_vel, bo
             if ((rc = record_frame64(a, c, k))) return rc;
             hipLaunchKernelGGL((k_compare<double4>), g, b, 0, st, a->n, t.pos, t.ck_pos + (f + 1) * n, a->flags + 1);
             hipLaunchKernelGGL((k_compare<double4>), g, b, 0, st, a->n, t.vel, t.ck_vel + (f + 1) * n, a->flags + 1);
-            if ((rc = reverse_frame(a, k, t, (const double4*)t.posS, (const double4*)t.curl))) return rc;
+            if ((rc = reverse_frame(a, k, t, a->perm, (const double4*)t.posS, (const double4*)t.curl))) return rc;
         } else {
             Tape<float4>& t = a->t32;
             FSI_HIP_TRY(hipMemcpyAsync(d.pos, t.ck_pos + f * n, sizeof(float4) * n, hipMemcpyDeviceToDevice, st));
             FSI_HIP_TRY(hipMemcpyAsync(d.vel, t.ck_vel + f * n, sizeof(float4) * n, hipMemcpyDeviceToDevice, st));
-            if ((rc = record_frame(a, c))) return rc;
+            if ((rc = record_frame(a, c, f))) return rc;
             hipLaunchKernelGGL((k_compare<float4>), g, b, 0, st, a->n, d.pos, t.ck_pos + (f + 1) * n, a->flags + 1);
             hipLaunchKernelGGL((k_compare<float4>), g, b, 0, st, a->n, d.vel, t.ck_vel + (f + 1) * n, a->flags + 1);
-            if ((rc = reverse_frame(a, k, t, (const float4*)d.posS, (const float4*)d.curl))) return rc;
+            if ((rc = reverse_frame(a, k, t, a->fwd_perm + (size_t)f * a->S * n, (const float4*)d.posS,
+                                    (const float4*)d.curl)))
+                return rc;
         }
         hipLaunchKernelGGL(k_absmax, g, b, 0, st, a->n, a->gpos, a->gvel, a->norms + 2 * kSpread * (size_t)f);
     }

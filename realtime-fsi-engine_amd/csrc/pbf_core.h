@@ -84,6 +84,9 @@ struct Core {
     hipGraphExec_t exec = nullptr;
     std::vector<unsigned char> graph_key;  // StepConst + plan + knobs the graph was captured with
     std::vector<fsi_tune_trial> trials;    // the settings the last autotune timed
+    // when set, every frame copies the sorted permutation of its substep s to
+    // record_perm + s * n (the adjoint's forward keeps them for its replay)
+    uint32_t* record_perm = nullptr;
 };
 
 // Binds the handle's device and creates its stream.

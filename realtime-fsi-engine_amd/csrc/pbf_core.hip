@@ -370,6 +370,9 @@ int enqueue_frame(Core& c, const StepConst& k, const FramePlan& plan, bool timed
         {
             PassTimer t(c, timed, FSI_PASS_SORT);
             FSI_HIP_TRY(radix_sort_pairs(d.keys, d.n, plan.key_bits, c.sort_digit_bits, d.sort, &d.skeys, &d.perm, st));
+            if (c.record_perm)
+                FSI_HIP_TRY(hipMemcpyAsync(c.record_perm + (size_t)s * d.n, d.perm, sizeof(uint32_t) * d.n,
+                                           hipMemcpyDeviceToDevice, st));
         }
         {
             PassTimer t(c, timed, FSI_PASS_TABLES);
@@ -413,10 +416,12 @@ int core_frame(Core& c, const StepConst& k, const FramePlan& plan) {
         // key: everything the launches of a frame depend on
         const int knobs[8] = {c.d.variant, c.sort_digit_bits, c.taps ? 1 : 0, c.d.any_sdf ? 1 : 0,
                               c.d.morton ? 1 : 0, c.d.relabel ? 1 : 0, c.d.block, c.d.tile_shape};
-        std::vector<unsigned char> key(sizeof(StepConst) + sizeof(FramePlan) + sizeof(knobs));
+        std::vector<unsigned char> key(sizeof(StepConst) + sizeof(FramePlan) + sizeof(knobs) + sizeof(c.record_perm));
         std::memcpy(key.data(), &k, sizeof(StepConst));
         std::memcpy(key.data() + sizeof(StepConst), &plan, sizeof(FramePlan));
         std::memcpy(key.data() + sizeof(StepConst) + sizeof(FramePlan), knobs, sizeof(knobs));
+        std::memcpy(key.data() + sizeof(StepConst) + sizeof(FramePlan) + sizeof(knobs), &c.record_perm,
+                    sizeof(c.record_perm));
         if (!c.graph_valid || key != c.graph_key) {
             core_invalidate_graph(c);
             FSI_HIP_TRY(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
