@@ -1296,9 +1296,8 @@ struct fsi_adjoint {
     bool forward_done = false;
     Tape<float4> t32;
     Tape<double4> t64;
-    uint32_t* perm = nullptr;  // S x n: the sorted permutations of the FP64 replay
-    // frames x S x n: the sorted permutation of every substep of the FP32
-    // forward, which the replay takes instead of sorting the same keys again
+    // frames x S x n: the sorted permutation of every substep of the forward,
+    // which the replay takes instead of sorting the same keys again
     uint32_t* fwd_perm = nullptr;
     std::vector<EdgeList> lists;  // S x (I + 1) + 1
     uint32_t* list_mem = nullptr;
@@ -1332,7 +1331,6 @@ int bind(fsi_adjoint* a) {
 void free_all(fsi_adjoint* a) {
     a->t32.release();
     a->t64.release();
-    dev_free(&a->perm);
     dev_free(&a->fwd_perm);
     dev_free(&a->list_mem);
     for (double4** p : {&a->gpos, &a->gvel, &a->gxold, &a->gP, &a->gQ, &a->gown, &a->gvs, &a->gvs_own, &a->gw,
@@ -1418,7 +1416,6 @@ hipError_t alloc_common(fsi_adjoint* a) {
     hipError_t e = hipSuccess;
 #define ALLOC(p, cnt) \
     if ((e = dev_alloc(&(p), (cnt))) != hipSuccess) return e;
-    ALLOC(a->perm, S * n);
     ALLOC(a->fwd_perm, (size_t)a->frames * S * n);
     const size_t nl = S * (I + 1) + 1;
     const size_t per = ((size_t)kEdgeCap + kInCap + 2) * n;
@@ -1546,9 +1543,11 @@ int record_frame(fsi_adjoint* a, const StepConst& c, uint32_t f) {
     return FSI_OK;
 }
 
-// One frame of the FP64 trajectory from the state in t64.pos / t64.vel,
+// Frame f of the FP64 trajectory from the state in t64.pos / t64.vel,
 // writing the tape (the forward runs this too: the lists are the search).
-int record_frame64(fsi_adjoint* a, const StepConst& c, const C64& k) {
+// The forward sorts each substep and keeps the permutation in fwd_perm; the
+// replay of the backward takes it from there.
+int record_frame64(fsi_adjoint* a, const StepConst& c, const C64& k, uint32_t f, bool forward) {
     DeviceState& d = a->core.d;
     Tape<double4>& t = a->t64;
     hipStream_t st = a->core.stream;
@@ -1560,11 +1559,18 @@ int record_frame64(fsi_adjoint* a, const StepConst& c, const C64& k) {
     for (uint32_t s = 0; s < a->S; ++s) {
         FSI_HIP_TRY(hipMemcpyAsync(t.xold + s * n, t.pos, sizeof(double4) * n, hipMemcpyDeviceToDevice, st));
         hipLaunchKernelGGL(k_predict64, g, b, 0, st, k, t.pos, t.vel, t.xstar, d.keys);
-        FSI_HIP_TRY(radix_sort_pairs(d.keys, d.n, key_bits, a->core.sort_digit_bits, d.sort, &d.skeys, &d.perm, st));
+        uint32_t* fp = a->fwd_perm + ((size_t)f * a->S + s) * n;
+        if (forward) {
+            FSI_HIP_TRY(
+                radix_sort_pairs(d.keys, d.n, key_bits, a->core.sort_digit_bits, d.sort, &d.skeys, &d.perm, st));
+            FSI_HIP_TRY(hipMemcpyAsync(fp, d.perm, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, st));
+        } else {
+            FSI_HIP_TRY(hipMemcpyAsync(d.perm, fp, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, st));
+            hipLaunchKernelGGL(k_sorted_keys, g, b, 0, st, a->n, d.keys, d.perm, d.skeys);
+        }
         FSI_HIP_TRY(launch_cells(c, d, st));
         hipLaunchKernelGGL(k_reorder64, g, b, 0, st, a->n, d.perm, t.xstar, t.vel, t.Pw, t.vsw);
         double4* Ps = t.P + (size_t)s * (I + 1) * n;
-        FSI_HIP_TRY(hipMemcpyAsync(a->perm + s * n, d.perm, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, st));
         FSI_HIP_TRY(hipMemcpyAsync(t.vs + s * n, t.vsw, sizeof(double4) * n, hipMemcpyDeviceToDevice, st));
         FSI_HIP_TRY(hipMemcpyAsync(Ps, t.Pw, sizeof(double4) * n, hipMemcpyDeviceToDevice, st));
         for (uint32_t it = 0; it < I; ++it) {
@@ -1682,7 +1688,7 @@ int forward_impl(fsi_adjoint* a, const void* pos0, const void* vel0, bool f64_io
         for (uint32_t f = 0; f < a->frames; ++f) {
             FSI_HIP_TRY(hipMemcpyAsync(t.ck_pos + f * n, t.pos, sizeof(double4) * n, hipMemcpyDeviceToDevice, st));
             FSI_HIP_TRY(hipMemcpyAsync(t.ck_vel + f * n, t.vel, sizeof(double4) * n, hipMemcpyDeviceToDevice, st));
-            if ((rc = record_frame64(a, c, k))) return rc;
+            if ((rc = record_frame64(a, c, k, f, true))) return rc;
         }
         FSI_HIP_TRY(hipMemcpyAsync(t.ck_pos + a->frames * n, t.pos, sizeof(double4) * n, hipMemcpyDeviceToDevice, st));
         FSI_HIP_TRY(hipMemcpyAsync(t.ck_vel + a->frames * n, t.vel, sizeof(double4) * n, hipMemcpyDeviceToDevice, st));
@@ -1769,10 +1775,12 @@ int backward_impl(fsi_adjoint* a, const void* grad_pos, const void* grad_vel, bo
             Tape<double4>& t = a->t64;
             FSI_HIP_TRY(hipMemcpyAsync(t.pos, t.ck_pos + f * n, sizeof(double4) * n, hipMemcpyDeviceToDevice, st));
             FSI_HIP_TRY(hipMemcpyAsync(t.vel, t.ck_vel + f * n, sizeof(double4) * n, hipMemcpyDeviceToDevice, st));
-            if ((rc = record_frame64(a, c, k))) return rc;
+            if ((rc = record_frame64(a, c, k, f, false))) return rc;
             hipLaunchKernelGGL((k_compare<double4>), g, b, 0, st, a->n, t.pos, t.ck_pos + (f + 1) * n, a->flags + 1);
             hipLaunchKernelGGL((k_compare<double4>), g, b, 0, st, a->n, t.vel, t.ck_vel + (f + 1) * n, a->flags + 1);
-            if ((rc = reverse_frame(a, k, t, a->perm, (const double4*)t.posS, (const double4*)t.curl))) return rc;
+            if ((rc = reverse_frame(a, k, t, a->fwd_perm + (size_t)f * a->S * n, (const double4*)t.posS,
+                                    (const double4*)t.curl)))
+                return rc;
         } else {
             Tape<float4>& t = a->t32;
             FSI_HIP_TRY(hipMemcpyAsync(d.pos, t.ck_pos + f * n, sizeof(float4) * n, hipMemcpyDeviceToDevice, st));
