@@ -32,8 +32,9 @@ median of five windows of --batch-steps frames.
 Third leg (reported under "adjoint"): the differentiable workload, 65,536
 particles (a 64x32x32 lattice block falling freely in the middle of a 6.4^3
 box, initial velocities N(0, 0.1) seeded), 20 frames of 4 iterations x 2 substeps,
-viscosity 0.01, eps 600.  The forward (frames + per-frame checkpoints) and the
-backward (replay of every frame with neighbour lists + reverse passes) of the
+viscosity 0.01, eps 600.  The forward (frames, per-frame checkpoints and each
+substep's sorted permutation) and the backward (replay of every frame with
+neighbour lists + reverse passes) of the
 hand-written adjoint are timed on the same inputs; the ratio is backward /
 forward.  The backward's input is the gradient of a 32^3 density-matching loss.
 
@@ -64,6 +65,7 @@ PKG = os.path.join(ROOT, "realtime-fsi-engine_amd")
 sys.path.insert(0, PKG)
 
 HBM_PEAK_GBS = 8000.0
+VALU_SIMDS, VALU_CLOCK_HZ = 1024, 2.4e9  # MI355X: 256 CUs x 4 SIMDs, peak engine clock
 N_SIDE = (128, 64, 128)
 ITERS, SUBSTEPS, HASH = 6, 2, 1 << 20
 
@@ -641,6 +643,24 @@ def main():
                                       ", rocprofv3 --pmc TCC_EA0_RDREQ_{32B,64B,128B}_sum and TCC_EA0_WRREQ(_64B)_sum "
                                       "over frames 1-3 of this workload (profiles/traffic_pbf_c2_1m.md), "
                                       "not over the timed window")
+
+    # issue rate of the dominant pass, which is VALU-bound: its VALU
+    # instructions per launch (rocprofv3 SQ_INSTS_VALU of this workload,
+    # profiles/counters_pbf_c2_1m.json) over the live launch time, at one wave
+    # instruction per 4 cycles on each of the 1024 SIMDs at 2.4 GHz
+    cfile = os.path.join(ROOT, "profiles", "counters_pbf_c2_1m.json")
+    if os.path.exists(cfile):
+        with open(cfile) as f:
+            cp = json.load(f)["passes"].get(dom)
+        if cp:
+            issue_s = cp["valu_inst_per_launch"] * 4.0 / (VALU_SIMDS * VALU_CLOCK_HZ)
+            roofline["valu_issue"] = {
+                "frac": round(issue_s / (dom_avg_ms * 1e-3), 4),
+                "valu_inst_per_launch": round(cp["valu_inst_per_launch"]),
+                "simds": VALU_SIMDS, "clock_ghz": VALU_CLOCK_HZ / 1e9,
+                "of": "VALU instructions per launch of " + " + ".join(cp["kernels"]) + " from rocprofv3 --pmc "
+                      "SQ_INSTS_VALU over frames 1-3 of this workload (profiles/counters_pbf_c2_1m.md), "
+                      "x 4 cycles / (1024 SIMDs x 2.4 GHz), over the live average launch time"}
 
     out = {
         "metric": "PBF particle-iterations/s at 1M particles (plain/batched) + adjoint/forward ratio",

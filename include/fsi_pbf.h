@@ -289,8 +289,8 @@ enum fsi_pbf_knob {
                                   /* storage, 2 the same with the particle arrays stored in    */
                                   /* that order, whole buckets in Morton order of their cells  */
                                   /* (default); form 2 needs 1 or 2 and runs as form 1 under 0 */
-    FSI_TUNE_BLOCK = 3,           /* workgroup size of the form-0/1 gathers: 128 or 256        */
-                                  /* (default; the reference's local_size_x)                  */
+    FSI_TUNE_BLOCK = 3,           /* workgroup size of the form-0/1 gathers: 128 (default) or */
+                                  /* 256 (the reference's local_size_x)                       */
     FSI_TUNE_TILE = 4,            /* cell-tile shape of form 2: 0 4x4x4 cells (default),      */
                                   /* 1 8x4x4, 2 8x8x4                                          */
     FSI_TUNE_COUNT = 5

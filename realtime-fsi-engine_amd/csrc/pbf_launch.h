@@ -83,7 +83,7 @@ struct DeviceState {
     uint32_t* ndefer = nullptr;  // kDeferWords: the deferred count, then the tile queue heads of the 8 XCDs
     SortScratch sort{};
     int variant = 1;           // gather form (tuning knob, see gather() in pbf_kernels.hip)
-    int block = 256;           // workgroup size of the gather kernels, 128 or 256 (tuning knob)
+    int block = 128;           // workgroup size of the gather kernels, 128 or 256 (tuning knob)
     int tile_shape = 0;        // tiled gathers: tiles of 2^tile_shape bins, 0..2 (tuning knob)
     bool morton = true;        // processing order: Morton order of the cells (else sorted slots)
     bool relabel = false;      // storage order = processing order: whole buckets in Morton order of their cells

@@ -244,7 +244,7 @@ def test_error_paths(gpu):
             with pytest.raises(fsi.FsiError) as e:
                 s.set_tuning(knob, value)
             assert e.value.code == -1, (knob, value)
-        assert s.knobs() == {"gather": 1, "sort_digit_bits": 10, "order": 2, "block": 256, "tile": 0}
+        assert s.knobs() == {"gather": 1, "sort_digit_bits": 10, "order": 2, "block": 128, "tile": 0}
 
 
 def test_full_size_properties(gpu):
