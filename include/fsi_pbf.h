@@ -541,8 +541,9 @@ int fsi_adj_destroy(fsi_adjoint* a);
  * way.  Drops the checkpoints of the last forward. */
 int fsi_adj_set_precision(fsi_adjoint* a, uint32_t bits);
 /* Runs the frames from (pos0, vel0), keeping a checkpoint of every frame's
- * starting state; pos_out / vel_out (may be NULL) receive the final state
- * (rounded to FP32 in the FP64 mode). */
+ * starting state and every substep's sorted permutation (n u32, which the
+ * backward's replay uses instead of sorting again); pos_out / vel_out (may be
+ * NULL) receive the final state (rounded to FP32 in the FP64 mode). */
 int fsi_adj_forward(fsi_adjoint* a, const float* pos0, const float* vel0, float viscosity, float eps, float* pos_out,
                     float* vel_out, void* stream);
 /* Given dL/d(final pos) and dL/d(final vel) (either may be NULL: zero),
