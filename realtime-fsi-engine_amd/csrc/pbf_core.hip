@@ -128,6 +128,34 @@ void set_binning(Domain& d, const float bmin[3], const float bmax[3], float h, u
     d.axis_bits = axis_bits;
 }
 
+// Whether the operands of the pair divisions lie where div_rn_normal returns
+// a / b (pbf_device.h): the spiky gradient's -spiky diff^2 / dist (dist in
+// [1e-5, h], diff = h - dist at least half an ulp of h when positive) and
+// s_corr's w / wq (w = poly6 (h^2 - r^2)^3, h^2 - r^2 at least half an ulp of
+// h^2 when positive).  Numerators must lie in [2^-80, 2^60], denominators in
+// [2^-40, 2^40] and quotients in [2^-100, 2^80]: inside the range on which
+// the test library compares div_rn_normal with a / b, and far inside the one
+// on which the two are the same instructions.
+static bool pair_divisions_in_range(const StepConst& c) {
+    const double alo = std::ldexp(1.0, -80), ahi = std::ldexp(1.0, 60);
+    const double blo = std::ldexp(1.0, -40), bhi = std::ldexp(1.0, 40);
+    const double qlo = std::ldexp(1.0, -100), qhi = std::ldexp(1.0, 80);
+    auto in = [](double v, double lo, double hi) { return std::isfinite(v) && v >= lo && v <= hi; };
+    const double h = c.h;
+    if (!(h > 0.0) || !std::isfinite(h)) return false;
+    const double uh = std::ldexp(1.0, std::ilogb(h) - 24), uh2 = std::ldexp(1.0, std::ilogb(h * h) - 24);
+    const double K = std::fabs((double)c.spiky), amax = K * h * h, amin = K * uh * uh;
+    bool ok = in(amin, alo, ahi) && in(amax, alo, ahi) && in(1e-5, blo, bhi) && in(h, blo, bhi) &&
+              in(amin / h, qlo, qhi) && in(amax / 1e-5, qlo, qhi);
+    if (c.enable_scorr != 0u) {
+        const double P = std::fabs((double)c.poly6), wq = c.wq, h2 = h * h;
+        const double wmax = P * h2 * h2 * h2, wmin = P * uh2 * uh2 * uh2;
+        ok = ok && in(wq, blo, bhi) && in(wmin, alo, ahi) && in(wmax, alo, ahi) && in(wmin / wq, qlo, qhi) &&
+             in(wmax / wq, qlo, qhi);
+    }
+    return ok;
+}
+
 StepConst step_const_common(const fsi_fluid_config& cfg, const fsi_pbf_constants& u, uint32_t n, bool interacting,
                             const float ipos[3], const float cam[3]) {
     StepConst c{};
@@ -163,6 +191,7 @@ StepConst step_const_common(const fsi_fluid_config& cfg, const fsi_pbf_constants
     c.interacting = interacting ? 1u : 0u;
     c.vort_eps = cfg.vorticity_epsilon;
     c.qscale = kQUnitsPerH / u.h;
+    c.fast_div = pair_divisions_in_range(c) ? 1u : 0u;
     return c;
 }
 

@@ -43,6 +43,7 @@ struct StepConst {
     uint32_t interacting;
     float ipos[3], cam[3], vort_eps;
     float qscale;               // kQUnitsPerH / h: quantisation of the candidate pre-test
+    uint32_t fast_div;          // the operands of the pair divisions lie where div_rn_normal is exact
     Domain dom;                 // the single scene of the plain solver
     const Domain* doms;         // batched: one per scene
     const uint16_t* scene_of;   // batched: scene of particle id i (= of sorted slot i)
@@ -192,6 +193,32 @@ __device__ __forceinline__ int32_t qdist2(uint2 a, uint2 b) {
     return __builtin_amdgcn_sdot2(dz, dz, s, true);
 }
 
+// ---- correctly rounded division without the range fix-ups ----
+// This build's a / b (IEEE, -fhip-fp32-correctly-rounded-divide-sqrt) is
+// v_div_scale of both operands, the reciprocal estimate and its Newton step,
+// the quotient and two FMA residual corrections (the last in v_div_fmas),
+// then v_div_fixup.  While a and b are normal, their exponents are less than
+// 96 apart and the quotient is normal, v_div_scale returns both operands
+// unchanged and clears VCC, v_div_fmas is then an FMA and v_div_fixup
+// returns its input: div_rn_normal is the same sequence without those three
+// steps and gives the same bits (the test library of this package compares
+// it with a / b on every numerator of a wide range for many denominators).
+// A zero numerator gives +0 where a / b gives a signed zero; the divisions
+// below feed sums that start at +0, which a zero of either sign leaves +0.
+// StepConst::fast_div holds the host's check that a step's operands stay in
+// the range (pair_divisions_in_range); otherwise the kernels divide.
+__device__ __forceinline__ float rcp_refined(float b) {
+    const float r = __builtin_amdgcn_rcpf(b);
+    return __builtin_fmaf(__builtin_fmaf(-b, r, 1.0f), r, r);
+}
+// a / b given r1 = rcp_refined(b)
+__device__ __forceinline__ float div_rn_by(float a, float b, float r1) {
+    const float q = a * r1;
+    const float q1 = __builtin_fmaf(__builtin_fmaf(-b, q, a), r1, q);
+    return __builtin_fmaf(__builtin_fmaf(-b, q1, a), r1, q1);
+}
+__device__ __forceinline__ float div_rn_normal(float a, float b) { return div_rn_by(a, b, rcp_refined(b)); }
+
 // Spiky gradient (CS_ComputeLambdas.glsl:83-92).  r2 >= 1e-10 here, inside
 // the range of sqrt_rn_normal.
 __device__ __forceinline__ V3 spiky(const StepConst& c, V3 r, float r2) {
@@ -199,7 +226,8 @@ __device__ __forceinline__ V3 spiky(const StepConst& c, V3 r, float r2) {
     if (r2 > h2 || r2 < 1e-10f) return mk(0.0f, 0.0f, 0.0f);
     const float dist = sqrt_rn_normal(r2);
     const float diff = c.h - dist;
-    const float scalar = -c.spiky * diff * diff / dist;
+    const float num = -c.spiky * diff * diff;
+    const float scalar = c.fast_div != 0u ? div_rn_normal(num, dist) : num / dist;
     return scl(r, scalar);
 }
 
@@ -224,7 +252,7 @@ __device__ __forceinline__ float scorr_r2(const StepConst& c, float r2) {
     if (c.enable_scorr == 0u) return 0.0f;
     const float w = poly6_r2(c, r2);
     if (c.wq <= 1e-12f || w <= 0.0f) return 0.0f;
-    const float ratio = w / c.wq;
+    const float ratio = c.fast_div != 0u ? div_rn_by(w, c.wq, rcp_refined(c.wq)) : w / c.wq;
     if (ratio <= 0.0f) return 0.0f;
     return -c.k_corr * powr(ratio, c.n_corr);
 }
