@@ -133,6 +133,26 @@ def test_one_substep_grid_counts_positions(gpu, name, gather):
     s.close()
 
 
+def test_scaled_scene_takes_the_ieee_division(gpu):
+    """A dam break 1000x larger (h = 100): s_corr's numerator w = poly6 (h^2 -
+    r^2)^3 can fall below 2^-80, outside the range the shortened pair division
+    is checked on, so the host keeps the IEEE division for this step; the
+    results still equal the oracle's."""
+    import math
+
+    fsi = _fsi()
+    h = 100.0
+    cfg = dam_break_config(n_iter=4, substeps=2, box=(6400.0, 3200.0, 3200.0), h=h, gravity=(0.0, -9810.0, 0.0))
+    # the premise: the smallest positive w, poly6 (ulp(h^2) / 2)^3, below 2^-80
+    poly6 = 315.0 / (64.0 * math.pi * h ** 9)
+    assert poly6 * (2.0 ** (math.floor(math.log2(h * h)) - 24)) ** 3 < 2.0 ** -80
+    parts = fsi.lattice(16, 16, 16, (0.0, 0.0, 0.0), 50.0, 1.0, 1337)
+    s, ref = run_pair(cfg, parts, frames=2)
+    assert_grid_equal(s, ref)
+    assert_state_close(s, ref, cfg, frames=2, label="h=100")
+    s.close()
+
+
 def test_full_step_two_substeps_vorticity(gpu):
     fsi = _fsi()
     cfg = dam_break_config(n_iter=4, substeps=2)
