@@ -138,7 +138,6 @@ __global__ __launch_bounds__(BS) void k_build_list(StepConst c, const uint32_t* 
     // the staging rings first (LDS offset 0, so that a ring address is an
     // OR of the entry and lane bits), then the bucket tables
     __shared__ uint32_t lds[(kLRing + kBucketEntries) * BS];
-    uint32_t* ring = &lds[threadIdx.x];
     uint32_t* mine = &lds[kLRing * BS + threadIdx.x];
     uint4* run = wave_run(list, t);
     int32_t cx, cy, cz;
@@ -174,68 +173,77 @@ __global__ __launch_bounds__(BS) void k_build_list(StepConst c, const uint32_t* 
         const char* p = qbytes + (size_t)s * sizeof(uint2);
         return G4{*reinterpret_cast<const uint4_a8*>(p), *reinterpret_cast<const uint4_a8*>(p + 16)};
     };
-    uint32_t tail = 0, done = 0;  // entries appended / written out (a multiple of four)
+    // ring positions in bytes: entry e of this lane's ring is at
+    // e * step + lane bytes, so an address is one AND-OR of the position
+    constexpr uint32_t step = (uint32_t)BS * 4u, wrap = (kLRing - 1u) * step;
+    const uint32_t laneb = threadIdx.x * 4u;
+    char* const lbase = reinterpret_cast<char*>(lds);
+    uint32_t tail = 0, done = 0;  // entries appended / written out (a multiple of four), times step
     // writes entries done..done+3 (the last group may hold stale slots past
     // the count: they are never read as entries)
     auto copy4 = [&]() {
-        const uint32_t* r = ring + (done & (kLRing - 1u)) * BS;
+        const uint32_t* r = reinterpret_cast<const uint32_t*>(lbase + ((done & wrap) | laneb));
         const uint4 v = make_uint4(r[0], r[BS], r[2 * BS], r[3 * BS]);
-        if (done < (uint32_t)kNbrCap) run[(done >> 2) * 64u] = v;
-        done += 4;
+        if (done < (uint32_t)kNbrCap * step) run[(done / (4u * step)) * 64u] = v;
+        done += 4u * step;
     };
     // branch-free append: the slot is always written and the tail advances
     // only for a valid slot that passes the pre-test
     auto visit = [&](uint32_t s, uint2 qs, bool valid) {
-        ring[(tail & (kLRing - 1u)) * BS] = s;
-        // bitwise, so that the test is not made a branch
-        tail += (uint32_t)(qdist2(qi, qs) <= qpass) & (uint32_t)valid;
+        *reinterpret_cast<uint32_t*>(lbase + ((tail & wrap) | laneb)) = s;
+        tail += (qdist2(qi, qs) <= qpass) && valid ? step : 0u;
     };
-    // the stream: entry p of the table, slots s.. of its bucket (rem left,
-    // may be <= 0); p = 27 is the empty entry after the last bucket.  An
-    // iteration tests three groups of four of the stream, A at (s, rem, p),
-    // B and C after it (each the next four slots of the previous group's
-    // bucket or the first four of the next bucket).  All three are loaded
-    // at the top of the iteration, and the table entries the next
-    // iteration needs are read while they are tested.
+    // the stream: a group is (start << L | slots left), the table's own
+    // encoding (0 when nothing is left); p is the table entry it came from
+    // and p = 27 the empty entry after the last bucket.  An iteration tests
+    // three groups of four of the stream, A (g, p), B and C after it (each
+    // the next four slots of the previous group's bucket or the first four
+    // of the next bucket).  All three are loaded at the top of the
+    // iteration, and the table entries the next iteration needs are read
+    // while they are tested.
     auto entry = [&](uint32_t q) { return mine[min(q, 27u) * BS]; };
-    // the group after (s, rem, p), given e = the entry after p
-    auto after = [&](uint32_t s0, int32_t r0, uint32_t p0, uint32_t e, uint32_t& s1, int32_t& r1, uint32_t& p1) {
-        const bool same = r0 > 4;
-        s1 = same ? s0 + 4u : e >> lbits;
-        r1 = same ? r0 - 4 : (int32_t)(e & lmax);
+    // four slots on in the same bucket: start + 4, left - 4 (left > 4, so
+    // neither field carries into the other)
+    const uint32_t adv = (4u << lbits) - 4u;
+    // the group after (g0, p0), given e = the entry after p0
+    auto after = [&](uint32_t g0, uint32_t p0, uint32_t e, uint32_t& g1, uint32_t& p1) {
+        const bool same = (g0 & lmax) > 4u;
+        g1 = same ? g0 + adv : e;
         p1 = same ? p0 : min(p0 + 1u, 27u);
     };
     uint32_t p = longb ? 27u : 0u;
-    const uint32_t e0 = entry(p);
-    uint32_t s = e0 >> lbits, sB, pB, sC, pC;
-    int32_t rem = (int32_t)(e0 & lmax), remB, remC;
-    after(s, rem, p, entry(p + 1u), sB, remB, pB);
-    after(sB, remB, pB, entry(pB + 1u), sC, remC, pC);
-    auto test4 = [&](const G4& g, uint32_t s0, int32_t r0) {
-        visit(s0, make_uint2(g.a.x, g.a.y), 0 < r0);
-        visit(s0 + 1, make_uint2(g.a.z, g.a.w), 1 < r0);
-        visit(s0 + 2, make_uint2(g.b.x, g.b.y), 2 < r0);
-        visit(s0 + 3, make_uint2(g.b.z, g.b.w), 3 < r0);
+    uint32_t g = entry(p), gB, pB, gC, pC;
+    after(g, p, entry(p + 1u), gB, pB);
+    after(gB, pB, entry(pB + 1u), gC, pC);
+    // a group with no slots left is 0 and loads slots 0..3 (Q holds n + 4
+    // records); a group with slots loads inside Q as well
+    auto test4 = [&](const G4& q4, uint32_t g0) {
+        const uint32_t s0 = g0 >> lbits, r0 = g0 & lmax;
+        visit(s0, make_uint2(q4.a.x, q4.a.y), 0u < r0);
+        visit(s0 + 1, make_uint2(q4.a.z, q4.a.w), 1u < r0);
+        visit(s0 + 2, make_uint2(q4.b.x, q4.b.y), 2u < r0);
+        visit(s0 + 3, make_uint2(q4.b.z, q4.b.w), 3u < r0);
     };
 #pragma unroll 1
-    while (__any(p < 27u || rem > 0)) {
-        const G4 ga = loadg(rem > 0 ? s : 0u), gb = loadg(remB > 0 ? sB : 0u), gc = loadg(remC > 0 ? sC : 0u);
+    while (__any(p < 27u || g != 0u)) {
+        const G4 ga = loadg(g >> lbits), gb = loadg(gB >> lbits), gc = loadg(gC >> lbits);
         // the entries after pC (the next three groups start at or after it)
         const uint32_t f1 = entry(pC + 1u), f2 = entry(pC + 2u), f3 = entry(pC + 3u);
-        if (tail - done > kLRing - 12u) copy4();  // room for twelve more
-        if (tail - done > kLRing - 12u) copy4();
-        if (tail - done > kLRing - 12u) copy4();
-        test4(ga, s, rem);
-        test4(gb, sB, remB);
-        test4(gc, sC, remC);
+        if (tail - done > (kLRing - 12u) * step) copy4();  // room for twelve more
+        if (tail - done > (kLRing - 12u) * step) copy4();
+        if (tail - done > (kLRing - 12u) * step) copy4();
+        test4(ga, g);
+        test4(gb, gB);
+        test4(gc, gC);
         // the next A follows C, the next B follows it, the next C follows
         // that; each takes the entry after its predecessor's (pC + 1, 2, 3)
-        after(sC, remC, pC, f1, s, rem, p);
-        after(s, rem, p, p == pC ? f1 : f2, sB, remB, pB);
-        after(sB, remB, pB, pB == pC ? f1 : (pB == pC + 1u ? f2 : f3), sC, remC, pC);
+        after(gC, pC, f1, g, p);
+        after(g, p, p == pC ? f1 : f2, gB, pB);
+        after(gB, pB, pB == pC ? f1 : (pB == pC + 1u ? f2 : f3), gC, pC);
     }
 #pragma unroll 1
     while (tail > done) copy4();
+    tail /= step;
     list_count[t] = longb ? (uint32_t)kNbrCap + 1u : tail;
     if (TAPS) cand_out[k] = ncand;
 }
