@@ -27,6 +27,7 @@ namespace {
 struct SceneHost {
     uint32_t offset = 0, count = 0, hash_base = 0, hash_size = 0;
     float* sdf_grid = nullptr;  // device copy (owned)
+    float2* sdf_brange = nullptr;  // its block ranges (owned)
 };
 }  // namespace
 
@@ -54,7 +55,10 @@ int bind(fsi_batch* b) {
 }
 
 void free_scenes(fsi_batch* b) {
-    for (SceneHost& s : b->scenes) dev_free(&s.sdf_grid);
+    for (SceneHost& s : b->scenes) {
+        dev_free(&s.sdf_grid);
+        dev_free(&s.sdf_brange);
+    }
     b->scenes.clear();
     b->doms.clear();
     dev_free(&b->d_doms);
@@ -176,6 +180,8 @@ int fsi_batch_set_scenes(fsi_batch* b, const fsi_particle* ps, uint32_t total, c
             const size_t cnt = (size_t)sc[s].sdf_dims[0] * sc[s].sdf_dims[1] * sc[s].sdf_dims[2];
             FSI_HIP_TRY(dev_alloc(&h.sdf_grid, cnt));
             FSI_HIP_TRY(hipMemcpy(h.sdf_grid, sc[s].sdf_grid, cnt * sizeof(float), hipMemcpyHostToDevice));
+            FSI_HIP_TRY(dev_alloc(&h.sdf_brange, sdf_block_count(sc[s].sdf_dims)));
+            FSI_HIP_TRY(launch_sdf_block_range(h.sdf_grid, sc[s].sdf_dims, h.sdf_brange, b->core.stream));
             d.enable_sdf = 1u;
             d.sdf_mode = sc[s].sdf_containment ? 1u : 0u;
             for (int a = 0; a < 3; ++a) {
@@ -185,6 +191,8 @@ int fsi_batch_set_scenes(fsi_batch* b, const fsi_particle* ps, uint32_t total, c
             d.sdf_cell = sc[s].sdf_cell;
             d.sdf_pad = sc[s].sdf_padding;
             d.sdf_grid = h.sdf_grid;
+            d.sdf_brange = h.sdf_brange;
+            for (int a = 0; a < 3; ++a) d.sdf_bdims[a] = sdf_blocks(sc[s].sdf_dims[a]);
             b->any_sdf = true;
         }
     }

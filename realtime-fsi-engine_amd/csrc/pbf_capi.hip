@@ -28,6 +28,7 @@ struct fsi_pbf {
     float bmin[3] = {0, 0, 0}, bmax[3] = {0, 0, 0}, damping = 0.0f;
     // SDF boundary
     float* sdf_grid = nullptr;
+    float2* sdf_brange = nullptr;  // block ranges of sdf_grid (Domain::sdf_brange)
     int32_t sdf_dims[3] = {0, 0, 0};
     float sdf_origin[3] = {0, 0, 0};
     float sdf_cell = 0.0f;
@@ -76,6 +77,8 @@ StepConst step_const(const fsi_pbf* h) {
     d.sdf_cell = h->sdf_cell;
     d.sdf_pad = h->sdf_padding;
     d.sdf_grid = h->sdf_grid;
+    d.sdf_brange = h->sdf_brange;
+    for (int a = 0; a < 3; ++a) d.sdf_bdims[a] = sdf_blocks(h->sdf_dims[a]);
     set_binning(d, u.bounds_min, u.bounds_max, u.h, 7, 0, h->core.d.variant == 2);
     return c;
 }
@@ -199,6 +202,7 @@ int fsi_pbf_destroy(fsi_pbf* h) {
     dev_free(&h->contact_count);
     dev_free(&h->pred);
     dev_free(&h->sdf_grid);
+    dev_free(&h->sdf_brange);
     core_destroy(h->core);
     delete h;
     return FSI_OK;
@@ -368,6 +372,7 @@ int fsi_pbf_set_sdf(fsi_pbf* h, const float* grid, const int32_t dims[3], const 
     FSI_HIP_TRY(hipStreamSynchronize(h->core.stream));
     if (!grid) {
         dev_free(&h->sdf_grid);
+        dev_free(&h->sdf_brange);
         h->sdf_valid = false;
         return FSI_OK;
     }
@@ -378,8 +383,12 @@ int fsi_pbf_set_sdf(fsi_pbf* h, const float* grid, const int32_t dims[3], const 
     if (count > (size_t)1 << 31) return fail(FSI_ERR_INVALID_ARG, "SDF grid above 2^31 voxels");
     const bool same =
         h->sdf_grid && dims[0] == h->sdf_dims[0] && dims[1] == h->sdf_dims[1] && dims[2] == h->sdf_dims[2];
-    if (!same) FSI_HIP_TRY(dev_alloc(&h->sdf_grid, count));
+    if (!same) {
+        FSI_HIP_TRY(dev_alloc(&h->sdf_grid, count));
+        FSI_HIP_TRY(dev_alloc(&h->sdf_brange, sdf_block_count(dims)));
+    }
     FSI_HIP_TRY(hipMemcpy(h->sdf_grid, grid, count * sizeof(float), hipMemcpyHostToDevice));
+    FSI_HIP_TRY(launch_sdf_block_range(h->sdf_grid, dims, h->sdf_brange, h->core.stream));
     for (int a = 0; a < 3; ++a) {
         h->sdf_dims[a] = dims[a];
         h->sdf_origin[a] = origin[a];
@@ -482,9 +491,11 @@ int fsi_pbf_set_sdf_from_mesh(fsi_pbf* h, fsi_mesh* m) {
     if (!same) {
         FSI_HIP_TRY(hipStreamSynchronize(h->core.stream));
         FSI_HIP_TRY(dev_alloc(&h->sdf_grid, count));
+        FSI_HIP_TRY(dev_alloc(&h->sdf_brange, sdf_block_count(dims)));
     }
     FSI_HIP_TRY(hipStreamWaitEvent(h->core.stream, ready, 0));
     FSI_HIP_TRY(hipMemcpyAsync(h->sdf_grid, grid, count * sizeof(float), hipMemcpyDeviceToDevice, h->core.stream));
+    FSI_HIP_TRY(launch_sdf_block_range(h->sdf_grid, dims, h->sdf_brange, h->core.stream));
     if ((rc = mesh_wait_for(m, h->core.stream))) return rc;
     for (int a = 0; a < 3; ++a) {
         h->sdf_dims[a] = dims[a];

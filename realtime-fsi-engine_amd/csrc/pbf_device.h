@@ -27,7 +27,18 @@ struct Domain {
     int32_t dims[3];
     float sdf_origin[3], sdf_cell, sdf_pad;
     const float* sdf_grid;
+    // (minimum, maximum) of the grid over each block of kSdfBlock^3 voxels
+    // and the voxels one step above it (sdf_bdims blocks per axis; (-inf,
+    // +inf) for a block with a non-finite voxel), or nullptr: lets
+    // sdf_project leave out the trilinear samples of a particle that no
+    // projection can reach
+    const float2* sdf_brange;
+    int32_t sdf_bdims[3];
 };
+
+// Voxels per axis of an SDF block and the block count along an axis of d voxels.
+constexpr int32_t kSdfBlockBits = 2, kSdfBlock = 1 << kSdfBlockBits;
+__host__ __device__ inline int32_t sdf_blocks(int32_t d) { return (d + kSdfBlock - 1) >> kSdfBlockBits; }
 
 // Constants of one step: the reference's FluidConfigUBO
 // (include/fluid/PBFluids.h:21-61) plus the per-dispatch uniforms that
@@ -274,28 +285,37 @@ __device__ __forceinline__ void sdf_taps(float tc, int32_t size, int32_t& i0, in
 
 __device__ __forceinline__ float lerp1(float t0, float t1, float a) { return (1.0f - a) * t0 + a * t1; }
 
-// querySDF (CS_Integrate.glsl:86-89).
-__device__ __forceinline__ float sdf_query(const Domain& c, V3 p) {
+// The eight voxels and weights of a trilinear sample at p.
+struct SdfTaps {
+    int32_t x0, x1, y0, y1, z0, z1;
+    float ax, ay, az;
+};
+__device__ __forceinline__ SdfTaps sdf_locate(const Domain& c, V3 p) {
     const float tx = ((p.x - c.sdf_origin[0]) / c.sdf_cell + 0.5f) / (float)c.dims[0];
     const float ty = ((p.y - c.sdf_origin[1]) / c.sdf_cell + 0.5f) / (float)c.dims[1];
     const float tz = ((p.z - c.sdf_origin[2]) / c.sdf_cell + 0.5f) / (float)c.dims[2];
-    int32_t x0, x1, y0, y1, z0, z1;
-    float ax, ay, az;
-    sdf_taps(tx, c.dims[0], x0, x1, ax);
-    sdf_taps(ty, c.dims[1], y0, y1, ay);
-    sdf_taps(tz, c.dims[2], z0, z1, az);
+    SdfTaps t;
+    sdf_taps(tx, c.dims[0], t.x0, t.x1, t.ax);
+    sdf_taps(ty, c.dims[1], t.y0, t.y1, t.ay);
+    sdf_taps(tz, c.dims[2], t.z0, t.z1, t.az);
+    return t;
+}
+__device__ __forceinline__ float sdf_sample(const Domain& c, const SdfTaps& t) {
     const size_t W = (size_t)c.dims[0], WH = (size_t)c.dims[0] * (size_t)c.dims[1];
     const float* g = c.sdf_grid;
-    const size_t r00 = (size_t)y0 * W + (size_t)z0 * WH, r10 = (size_t)y1 * W + (size_t)z0 * WH;
-    const size_t r01 = (size_t)y0 * W + (size_t)z1 * WH, r11 = (size_t)y1 * W + (size_t)z1 * WH;
-    const float c00 = lerp1(g[r00 + x0], g[r00 + x1], ax);
-    const float c10 = lerp1(g[r10 + x0], g[r10 + x1], ax);
-    const float c01 = lerp1(g[r01 + x0], g[r01 + x1], ax);
-    const float c11 = lerp1(g[r11 + x0], g[r11 + x1], ax);
-    const float c0 = lerp1(c00, c10, ay);
-    const float c1 = lerp1(c01, c11, ay);
-    return lerp1(c0, c1, az);
+    const size_t r00 = (size_t)t.y0 * W + (size_t)t.z0 * WH, r10 = (size_t)t.y1 * W + (size_t)t.z0 * WH;
+    const size_t r01 = (size_t)t.y0 * W + (size_t)t.z1 * WH, r11 = (size_t)t.y1 * W + (size_t)t.z1 * WH;
+    const float c00 = lerp1(g[r00 + t.x0], g[r00 + t.x1], t.ax);
+    const float c10 = lerp1(g[r10 + t.x0], g[r10 + t.x1], t.ax);
+    const float c01 = lerp1(g[r01 + t.x0], g[r01 + t.x1], t.ax);
+    const float c11 = lerp1(g[r11 + t.x0], g[r11 + t.x1], t.ax);
+    const float c0 = lerp1(c00, c10, t.ay);
+    const float c1 = lerp1(c01, c11, t.ay);
+    return lerp1(c0, c1, t.az);
 }
+
+// querySDF (CS_Integrate.glsl:86-89).
+__device__ __forceinline__ float sdf_query(const Domain& c, V3 p) { return sdf_sample(c, sdf_locate(c, p)); }
 
 // sdfGradient (CS_ApplyDeltaP.glsl:58-65).
 __device__ __forceinline__ V3 sdf_gradient(const Domain& c, V3 p) {
@@ -311,10 +331,31 @@ __device__ __forceinline__ V3 sdf_gradient(const Domain& c, V3 p) {
 // The shaders sample the gradient before testing the distance; its six
 // samples only matter when the distance test passes, so they are taken
 // after it (same results: a failed test returns false whatever the gradient).
+//
+// Before the sample, the range [m, M] of the voxels the sample reads (their
+// block's) decides most particles: each lerp (1 - a) t0 + a t1 with a in
+// [0, 1] rounds three times, so three levels of it give m - 9u|m| <= d <=
+// M + 9u|M| (u = 2^-24) for normal values, and lb = m - |m| 2^-19 - 2^-120,
+// ub = M + |M| 2^-19 + 2^-120 (rounded; the last term covers subnormal and
+// flushed intermediates) lie outside that.  lb >= pad (obstacle: no
+// projection unless d < pad) or ub <= -pad (container: none unless
+// d > -pad) means the distance test fails whatever the eight voxels are:
+// the same false as sampling them.
 __device__ __forceinline__ bool sdf_project(const Domain& c, V3& p, V3& nrm) {
     const float pad = c.sdf_pad;
-    const float d = sdf_query(c, p);
     const bool container = c.sdf_mode == 1u;
+    const SdfTaps tp = sdf_locate(c, p);
+    if (c.sdf_brange != nullptr) {
+        const int32_t b = (tp.x0 >> kSdfBlockBits) +
+                          c.sdf_bdims[0] * ((tp.y0 >> kSdfBlockBits) + c.sdf_bdims[1] * (tp.z0 >> kSdfBlockBits));
+        const float2 r = c.sdf_brange[b];
+        if (container) {
+            if (r.y + fabsf(r.y) * 0x1p-19f + 0x1p-120f <= -pad) return false;
+        } else {
+            if (r.x - fabsf(r.x) * 0x1p-19f - 0x1p-120f >= pad) return false;
+        }
+    }
+    const float d = sdf_sample(c, tp);
     if (container ? !(d > -pad) : !(d < pad)) return false;
     const V3 g = sdf_gradient(c, p);
     const float l = len(g);

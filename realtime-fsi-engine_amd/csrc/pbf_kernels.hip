@@ -429,6 +429,32 @@ __global__ __launch_bounds__(kBlock) void k_fill_u32(uint32_t* __restrict__ p, u
     if (i < n) p[i] = v;
 }
 
+// Block ranges of an SDF grid (Domain::sdf_brange): block (bx, by, bz) covers
+// voxels bx*kSdfBlock .. bx*kSdfBlock + kSdfBlock on each axis (clamped to the
+// grid), so that it holds both voxels of every trilinear tap pair whose lower
+// voxel lies in the block.  A non-finite voxel makes the block (-inf, +inf).
+__global__ __launch_bounds__(kBlock) void k_sdf_block_range(const float* __restrict__ grid, int32_t dx, int32_t dy,
+                                                            int32_t dz, int32_t bx, int32_t by, int32_t bz,
+                                                            float2* __restrict__ out) {
+    const uint32_t b = gid();
+    if (b >= (uint32_t)bx * (uint32_t)by * (uint32_t)bz) return;
+    const int32_t ix = (int32_t)(b % (uint32_t)bx), iy = (int32_t)((b / (uint32_t)bx) % (uint32_t)by),
+                  iz = (int32_t)(b / ((uint32_t)bx * (uint32_t)by));
+    const int32_t x0 = ix * kSdfBlock, y0 = iy * kSdfBlock, z0 = iz * kSdfBlock;
+    const int32_t x1 = min(x0 + kSdfBlock, dx - 1), y1 = min(y0 + kSdfBlock, dy - 1), z1 = min(z0 + kSdfBlock, dz - 1);
+    float lo = __builtin_inff(), hi = -__builtin_inff();
+    bool finite = true;
+    for (int32_t z = z0; z <= z1; ++z)
+        for (int32_t y = y0; y <= y1; ++y)
+            for (int32_t x = x0; x <= x1; ++x) {
+                const float v = grid[(size_t)x + (size_t)dx * ((size_t)y + (size_t)dy * (size_t)z)];
+                finite = finite && fabsf(v) <= 3.402823466e38f;
+                lo = fminf(lo, v);
+                hi = fmaxf(hi, v);
+            }
+    out[b] = finite ? make_float2(lo, hi) : make_float2(-__builtin_inff(), __builtin_inff());
+}
+
 // Sorted-slot processing order: thread t takes slot t.
 __global__ __launch_bounds__(kBlock) void k_order_slots(uint32_t n, uint32_t* __restrict__ order) {
     const uint32_t k = gid();
@@ -1259,6 +1285,14 @@ hipError_t launch_reorder(const StepConst& c, DeviceState& d, hipStream_t st) {
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL(k_tile_list, grid_for(c.n), dim3(kBlock), 0, st, c.n, d.tflag, d.tidx, d.tiles, d.ntiles);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_sdf_block_range(const float* grid, const int32_t dims[3], float2* out, hipStream_t st) {
+    const int32_t bx = sdf_blocks(dims[0]), by = sdf_blocks(dims[1]), bz = sdf_blocks(dims[2]);
+    const uint32_t nb = (uint32_t)bx * (uint32_t)by * (uint32_t)bz;
+    hipLaunchKernelGGL(k_sdf_block_range, grid_for(nb), dim3(kBlock), 0, st, grid, dims[0], dims[1], dims[2], bx, by, bz,
+                       out);
     return hipGetLastError();
 }
 

@@ -103,5 +103,11 @@ hipError_t launch_integrate(const StepConst& c, DeviceState& d, hipStream_t st);
 hipError_t launch_vorticity(const StepConst& c, DeviceState& d, hipStream_t st);
 hipError_t launch_contacts(const StepConst& c, DeviceState& d, float band, float4* out, uint32_t* count,
                            hipStream_t st);
+// Block ranges (Domain::sdf_brange) of an SDF grid of dims voxels (x fastest)
+// into out, which holds sdf_block_count(dims) entries.
+hipError_t launch_sdf_block_range(const float* grid, const int32_t dims[3], float2* out, hipStream_t st);
+inline size_t sdf_block_count(const int32_t dims[3]) {
+    return (size_t)sdf_blocks(dims[0]) * (size_t)sdf_blocks(dims[1]) * (size_t)sdf_blocks(dims[2]);
+}
 
 }  // namespace fsi

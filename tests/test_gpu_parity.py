@@ -195,6 +195,27 @@ def test_sdf_obstacle_and_container(gpu):
         s.close()
 
 
+def test_sdf_block_ranges_wide_padding(gpu):
+    """sdf_project decides most particles from the (min, max) of the SDF block
+    their sample reads, without the trilinear sample.  With a padding of four
+    voxels, about one block, particles lie on both sides of that decision in
+    both modes: an obstacle the fluid falls onto and a container the fluid
+    fills, with the sphere's centre inside the block.  The positions must
+    equal the oracle's, which always samples."""
+    fsi = _fsi()
+    cfg = dam_break_config(n_iter=3, substeps=2, box=(3.2, 3.2, 3.2))
+    verts, faces = fsi.icosphere(3, (1.6, 1.0, 1.6), 0.9)
+    sdf = fsi.sdf_build(verts, faces, 0.05)
+    cases = ((False, fsi.lattice(30, 12, 30, (0.85, 2.0, 0.85), 0.05, 0.001, 5)),
+             (True, fsi.lattice(24, 24, 24, (1.0, 0.4, 1.0), 0.05, 0.001, 6)))
+    for containment, parts in cases:
+        s, ref = run_pair(cfg, parts, frames=3, sdf=sdf, sdf_padding=0.2, containment=containment)
+        assert_grid_equal(s, ref)
+        assert_state_close(s, ref, cfg, frames=3, label=f"wide sdf padding containment={containment}")
+        assert np.array_equal(s.read_particles()[:, 0:3], ref.particles()[:, 0:3])
+        s.close()
+
+
 def test_apbf_interaction_padding(gpu):
     fsi = _fsi()
     cfg = dam_break_config(n_iter=3, substeps=2, enable_apbf=True, min_lod=1, max_lod=5, lod_max_dist=2.0)
