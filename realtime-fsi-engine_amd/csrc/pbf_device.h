@@ -341,20 +341,18 @@ __device__ __forceinline__ V3 sdf_gradient(const Domain& c, V3 p) {
 // projection unless d < pad) or ub <= -pad (container: none unless
 // d > -pad) means the distance test fails whatever the eight voxels are:
 // the same false as sampling them.
-__device__ __forceinline__ bool sdf_project(const Domain& c, V3& p, V3& nrm) {
+__device__ __forceinline__ bool sdf_out_of_reach(const Domain& c, const SdfTaps& tp) {
+    if (c.sdf_brange == nullptr) return false;
+    const int32_t b = (tp.x0 >> kSdfBlockBits) +
+                      c.sdf_bdims[0] * ((tp.y0 >> kSdfBlockBits) + c.sdf_bdims[1] * (tp.z0 >> kSdfBlockBits));
+    const float2 r = c.sdf_brange[b];
+    if (c.sdf_mode == 1u) return r.y + fabsf(r.y) * 0x1p-19f + 0x1p-120f <= -c.sdf_pad;
+    return r.x - fabsf(r.x) * 0x1p-19f - 0x1p-120f >= c.sdf_pad;
+}
+// The sampled part: the distance test, the gradient and the move.
+__device__ __forceinline__ bool sdf_project_sampled(const Domain& c, const SdfTaps& tp, V3& p, V3& nrm) {
     const float pad = c.sdf_pad;
     const bool container = c.sdf_mode == 1u;
-    const SdfTaps tp = sdf_locate(c, p);
-    if (c.sdf_brange != nullptr) {
-        const int32_t b = (tp.x0 >> kSdfBlockBits) +
-                          c.sdf_bdims[0] * ((tp.y0 >> kSdfBlockBits) + c.sdf_bdims[1] * (tp.z0 >> kSdfBlockBits));
-        const float2 r = c.sdf_brange[b];
-        if (container) {
-            if (r.y + fabsf(r.y) * 0x1p-19f + 0x1p-120f <= -pad) return false;
-        } else {
-            if (r.x - fabsf(r.x) * 0x1p-19f - 0x1p-120f >= pad) return false;
-        }
-    }
     const float d = sdf_sample(c, tp);
     if (container ? !(d > -pad) : !(d < pad)) return false;
     const V3 g = sdf_gradient(c, p);
@@ -368,6 +366,11 @@ __device__ __forceinline__ bool sdf_project(const Domain& c, V3& p, V3& nrm) {
         p = add(p, scl(nrm, pad - d));
     }
     return true;
+}
+__device__ __forceinline__ bool sdf_project(const Domain& c, V3& p, V3& nrm) {
+    const SdfTaps tp = sdf_locate(c, p);
+    if (sdf_out_of_reach(c, tp)) return false;
+    return sdf_project_sampled(c, tp, p, nrm);
 }
 
 }  // namespace fsi

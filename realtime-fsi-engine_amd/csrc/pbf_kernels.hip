@@ -589,10 +589,12 @@ __global__ __launch_bounds__(BS) void k_lambda(StepConst c, const uint32_t* __re
 }
 
 // CS_ComputeDeltaP.glsl:120-166 fused with the clamp and add of
-// CS_ApplyDeltaP.glsl:80-106: reads (x, lambda) from PL, writes the corrected
-// position to P (the SDF projection follows in k_apply_sdf).  In the list
-// form the neighbours come from the list the lambda pass of this iteration
-// walked (same positions, same order).
+// CS_ApplyDeltaP.glsl:80-106 and, for a single scene, the SDF projection of
+// :108-121 (in the batched form the scene's Domain comes from memory, and its
+// projection would double the registers of the gather: k_apply_sdf follows
+// there).  Reads (x, lambda) from PL, writes the corrected position to P.  In
+// the list form the neighbours come from the list the lambda pass of this
+// iteration walked (same positions, same order).
 template <int V, bool TAPS, bool B, int BS>
 __global__ __launch_bounds__(BS) void k_deltap(StepConst c, const uint32_t* __restrict__ order,
                                                    const float4* __restrict__ PL, float4* __restrict__ P,
@@ -626,14 +628,18 @@ __global__ __launch_bounds__(BS) void k_deltap(StepConst c, const uint32_t* __re
         const float l = sqrtf(l2);
         dp = scl(dp, maxdp / l);
     }
-    const V3 p = add(xi, dp);
+    V3 p = add(xi, dp);
+    if (!B && c.dom.enable_sdf == 1u) {
+        V3 nrm;
+        sdf_project(c.dom, p, nrm);
+    }
     P[k] = make_float4(p.x, p.y, p.z, 0.0f);
     Q[k] = quantize(p, c.qscale);
 }
 
 // The SDF projection of CS_ApplyDeltaP.glsl:108-121 on the corrected
-// positions, in place (a kernel of its own so that the seven trilinear
-// samples do not raise the register count of the delta-p gather).
+// positions, in place, after the batched and the tiled delta-p gathers (the
+// single-scene k_deltap projects itself).
 template <bool B>
 __global__ __launch_bounds__(kBlock) void k_apply_sdf(StepConst c, const uint32_t* __restrict__ order,
                                                       float4* __restrict__ P, uint2* __restrict__ Q,
@@ -1399,7 +1405,7 @@ hipError_t launch_deltap(const StepConst& c, DeviceState& d, uint32_t iter, bool
         FSI_GATHER_SWITCH(L)
 #undef L
     }
-    if (d.any_sdf) {
+    if (d.any_sdf && (d.batched || tiled_form(d))) {
 #define L(B) hipLaunchKernelGGL((k_apply_sdf<B>), grid_for(c.n), dim3(kBlock), 0, st, c, sorder(d), d.P, d.Q, d.lod, iter);
         FSI_B_SWITCH(d.batched, L)
 #undef L
