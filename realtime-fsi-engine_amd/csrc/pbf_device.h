@@ -321,9 +321,17 @@ __device__ __forceinline__ float sdf_query(const Domain& c, V3 p) { return sdf_s
 __device__ __forceinline__ V3 sdf_gradient(const Domain& c, V3 p) {
     const float e = c.sdf_cell;
     const float z = 0.0f;
-    const float gx = sdf_query(c, mk(p.x + e, p.y + z, p.z + z)) - sdf_query(c, mk(p.x - e, p.y - z, p.z - z));
-    const float gy = sdf_query(c, mk(p.x + z, p.y + e, p.z + z)) - sdf_query(c, mk(p.x - z, p.y - e, p.z - z));
-    const float gz = sdf_query(c, mk(p.x + z, p.y + z, p.z + e)) - sdf_query(c, mk(p.x - z, p.y - z, p.z - e));
+    // one axis at a time (not unrolled): the six samples do not all hold
+    // their registers at once
+    float gx = 0.0f, gy = 0.0f, gz = 0.0f;
+#pragma unroll 1
+    for (int a = 0; a < 3; ++a) {
+        const float ex = a == 0 ? e : z, ey = a == 1 ? e : z, ez = a == 2 ? e : z;
+        const float ga = sdf_query(c, mk(p.x + ex, p.y + ey, p.z + ez)) - sdf_query(c, mk(p.x - ex, p.y - ey, p.z - ez));
+        gx = a == 0 ? ga : gx;
+        gy = a == 1 ? ga : gy;
+        gz = a == 2 ? ga : gz;
+    }
     return dvs(mk(gx, gy, gz), 2.0f * e);
 }
 

@@ -658,78 +658,20 @@ __global__ __launch_bounds__(kBlock) void k_apply_sdf(StepConst c, const uint32_
     }
 }
 
-// CS_Integrate.glsl:135-280 in two kernels.  The gather: XSPH sum and curl
-// over the neighbours (CS_Integrate.glsl:160-205); neighbour velocities come
-// from the sorted post-predict snapshot vs (the Jacobi form of the racy read
-// at :171).  It writes the XSPH velocity change and the curl.
-template <int V, bool B, int BS>
-__global__ __launch_bounds__(BS) void k_integrate(StepConst c, const uint32_t* __restrict__ order,
-                                                      const float4* __restrict__ P, const float4* __restrict__ vs,
-                                                      const int2* __restrict__ cells,
-                                                      const uint32_t* __restrict__ list_count,
-                                                      const uint32_t* __restrict__ list, float4* __restrict__ vdelta,
-                                                      float4* __restrict__ curl_out) {
-    const uint32_t t = xcd_gid_bs<BS>();
-    if (t >= c.n) return;
-    const uint32_t k = slot_at(order, t);
-    const V3 old_vel = xyz(vs[k]);
-    const V3 xp = xyz(P[k]);
-    const float inv = c.inv_rho0;
-    const bool do_visc = c.enable_visc == 1u;
-    const bool do_vort = c.enable_vort == 1u;
-    V3 visc = mk(0.0f, 0.0f, 0.0f);
-    V3 curl = mk(0.0f, 0.0f, 0.0f);
-    uint32_t ncand = 0, nnbr = 0;
-    gather<V, false>(xp, c.h, domain_of<B>(c, k), cells, P, vs, c.n, k, t, list_count, list,
-                     [&](uint32_t, float4, V3 r, float r2, float4 vj) {
-        const V3 vij = sub(xyz(vj), old_vel);
-        if (do_visc) {
-            const float w = poly6_r2(c, r2);
-            visc = add(visc, scl(scl(vij, w), inv));
-        }
-        if (do_vort) {
-            const V3 gw = spiky(c, r, r2);
-            curl = add(curl, cross(vij, gw));
-        }
-    }, ncand, nnbr);
-    if (do_visc) {
-        const V3 vd = scl(visc, c.viscosity);
-        vdelta[k] = make_float4(vd.x, vd.y, vd.z, 0.0f);
-    }
-    if (do_vort) {
-        const V3 cw = scl(curl, inv);
-        curl_out[k] = make_float4(cw.x, cw.y, cw.z, 0.0f);
-    }
-}
-
 // The rest of CS_Integrate.glsl:206-280 per particle: AABB clamp, SDF
 // projection, velocity from the displacement plus the XSPH change, speed
 // clamp, wall damping, removal of the normal velocity at an SDF contact; the
 // committed state goes to pos / vel (id order) and posS (sorted).  posS.w
 // carries |curl| (the |omega_j| of CS_VorticityConfinement.glsl:156, the same
 // FP32 length), so that the vorticity gather reads one record per neighbour.
-// A kernel of its own so that the trilinear SDF samples do not raise the
-// register count of the gather.
-template <bool B>
-__global__ __launch_bounds__(kBlock) void k_integrate_finish(StepConst c, const uint32_t* __restrict__ order,
-                                                             const float4* __restrict__ P,
-                                                             const float4* __restrict__ vs,
-                                                             const float4* __restrict__ vdelta,
-                                                             const uint32_t* __restrict__ perm,
-                                                             const float4* __restrict__ curl,
-                                                             float4* __restrict__ pos, float4* __restrict__ vel,
-                                                             float4* __restrict__ posS, uint2* __restrict__ Q) {
-    const uint32_t t = xcd_gid();
-    if (t >= c.n) return;
-    const uint32_t k = slot_at(order, t);
-    const Domain& dom = domain_of<B>(c, k);
+// xp = P[k], vw = vs[k].w, visc_delta and om = |curl| as the gather left them.
+__device__ __forceinline__ void finish_particle(const StepConst& c, const Domain& dom, uint32_t k, V3 xp, float vw,
+                                                V3 visc_delta, float om, const uint32_t* __restrict__ perm,
+                                                float4* __restrict__ pos, float4* __restrict__ vel,
+                                                float4* __restrict__ posS, uint2* __restrict__ Q) {
     const uint32_t id = perm[k];
     const float4 op4 = pos[id];
     const V3 old_pos = xyz(op4);
-    const float vw = vs[k].w;
-    V3 xp = xyz(P[k]);
-    V3 visc_delta = mk(0.0f, 0.0f, 0.0f);
-    if (c.enable_visc == 1u) visc_delta = xyz(vdelta[k]);
     bool cxh = false, cyh = false, czh = false;
     if (xp.x <= dom.bmin[0]) { xp.x = dom.bmin[0]; cxh = true; }
     if (xp.x >= dom.bmax[0]) { xp.x = dom.bmax[0]; cxh = true; }
@@ -754,9 +696,76 @@ __global__ __launch_bounds__(kBlock) void k_integrate_finish(StepConst c, const 
     }
     pos[id] = make_float4(xp.x, xp.y, xp.z, op4.w);
     vel[id] = make_float4(v.x, v.y, v.z, vw);
-    const float om = c.enable_vort == 1u ? len(xyz(curl[k])) : 0.0f;
     posS[k] = make_float4(xp.x, xp.y, xp.z, om);
     Q[k] = quantize(xp, c.qscale);  // the vorticity gather's candidates are the committed positions
+}
+
+// CS_Integrate.glsl:135-280.  The gather: XSPH sum and curl over the
+// neighbours (CS_Integrate.glsl:160-205); neighbour velocities come from the
+// sorted post-predict snapshot vs (the Jacobi form of the racy read at :171).
+// It writes the curl and, for one scene (FIN), goes on with finish_particle;
+// the batched gather writes the XSPH velocity change for k_integrate_finish.
+template <int V, bool B, int BS, bool FIN>
+__global__ __launch_bounds__(BS) void k_integrate(StepConst c, const uint32_t* __restrict__ order,
+                                                      const float4* __restrict__ P, const float4* __restrict__ vs,
+                                                      const int2* __restrict__ cells,
+                                                      const uint32_t* __restrict__ list_count,
+                                                      const uint32_t* __restrict__ list, float4* __restrict__ vdelta,
+                                                      float4* __restrict__ curl_out, const uint32_t* __restrict__ perm,
+                                                      float4* __restrict__ pos, float4* __restrict__ vel,
+                                                      float4* __restrict__ posS, uint2* __restrict__ Q) {
+    const uint32_t t = xcd_gid_bs<BS>();
+    if (t >= c.n) return;
+    const uint32_t k = slot_at(order, t);
+    const float4 vs4 = vs[k];
+    const V3 old_vel = xyz(vs4);
+    const V3 xp = xyz(P[k]);
+    const float inv = c.inv_rho0;
+    const bool do_visc = c.enable_visc == 1u;
+    const bool do_vort = c.enable_vort == 1u;
+    V3 visc = mk(0.0f, 0.0f, 0.0f);
+    V3 curl = mk(0.0f, 0.0f, 0.0f);
+    uint32_t ncand = 0, nnbr = 0;
+    gather<V, false>(xp, c.h, domain_of<B>(c, k), cells, P, vs, c.n, k, t, list_count, list,
+                     [&](uint32_t, float4, V3 r, float r2, float4 vj) {
+        const V3 vij = sub(xyz(vj), old_vel);
+        if (do_visc) {
+            const float w = poly6_r2(c, r2);
+            visc = add(visc, scl(scl(vij, w), inv));
+        }
+        if (do_vort) {
+            const V3 gw = spiky(c, r, r2);
+            curl = add(curl, cross(vij, gw));
+        }
+    }, ncand, nnbr);
+    const V3 vd = do_visc ? scl(visc, c.viscosity) : mk(0.0f, 0.0f, 0.0f);
+    const V3 cw = scl(curl, inv);
+    if (do_vort) curl_out[k] = make_float4(cw.x, cw.y, cw.z, 0.0f);
+    if constexpr (FIN) {
+        finish_particle(c, domain_of<B>(c, k), k, xp, vs4.w, vd, do_vort ? len(cw) : 0.0f, perm, pos, vel, posS, Q);
+    } else {
+        if (do_visc) vdelta[k] = make_float4(vd.x, vd.y, vd.z, 0.0f);
+    }
+}
+
+// finish_particle after a gather that did not run it (the batched gather,
+// whose Domain read per lane would raise its register count, and frames
+// with neither viscosity nor vorticity).
+template <bool B>
+__global__ __launch_bounds__(kBlock) void k_integrate_finish(StepConst c, const uint32_t* __restrict__ order,
+                                                             const float4* __restrict__ P,
+                                                             const float4* __restrict__ vs,
+                                                             const float4* __restrict__ vdelta,
+                                                             const uint32_t* __restrict__ perm,
+                                                             const float4* __restrict__ curl,
+                                                             float4* __restrict__ pos, float4* __restrict__ vel,
+                                                             float4* __restrict__ posS, uint2* __restrict__ Q) {
+    const uint32_t t = xcd_gid();
+    if (t >= c.n) return;
+    const uint32_t k = slot_at(order, t);
+    const V3 visc_delta = c.enable_visc == 1u ? xyz(vdelta[k]) : mk(0.0f, 0.0f, 0.0f);
+    const float om = c.enable_vort == 1u ? len(xyz(curl[k])) : 0.0f;
+    finish_particle(c, domain_of<B>(c, k), k, xyz(P[k]), vs[k].w, visc_delta, om, perm, pos, vel, posS, Q);
 }
 
 // CS_VorticityConfinement.glsl:133-167 (pass 1)
@@ -1420,11 +1429,13 @@ hipError_t launch_integrate(const StepConst& c, DeviceState& d, hipStream_t st) 
             FSI_B_SWITCH(d.batched, L)
 #undef L
         }
+        // one scene: the gather finishes its particles itself
 #define L(V, B, BS)                                                                                              \
-    hipLaunchKernelGGL((k_integrate<V, B, BS>), grid_bs<BS>(c.n), dim3(BS), 0, st, c, sorder(d), d.P, d.vs,       \
-                       gcells(d), d.list_count, d.list, d.vdelta, d.curl);
+    hipLaunchKernelGGL((k_integrate<V, B, BS, !B>), grid_bs<BS>(c.n), dim3(BS), 0, st, c, sorder(d), d.P, d.vs,   \
+                       gcells(d), d.list_count, d.list, d.vdelta, d.curl, sperm(d), d.pos, d.vel, d.posS, d.Q);
         FSI_GATHER_SWITCH(L)
 #undef L
+        if (!d.batched) return hipGetLastError();
     }
 #define L(B)                                                                                                   \
     hipLaunchKernelGGL((k_integrate_finish<B>), grid_for(c.n), dim3(kBlock), 0, st, c, sorder(d), d.P, d.vs, d.vdelta, \
