@@ -242,8 +242,16 @@ __device__ __forceinline__ V3 spiky(const StepConst& c, V3 r, float r2) {
     return scl(r, scalar);
 }
 
-// x^n: binary powering for integer n in [0, 64], powf otherwise.
+// x^n: binary powering for integer n in [0, 64], powf otherwise.  For n = 4
+// (the s_corr exponent of FluidConfig and of every workload here) the
+// powering's products are x^2 = x x and (x^2)(x^2), then 1 x^4, which is
+// x^4 itself: written out, without the loop the exponent drives (a loop of
+// three selects per pair, since n is uniform but not a constant).
 __device__ __forceinline__ float powr(float x, float n) {
+    if (n == 4.0f) {
+        const float x2 = x * x;
+        return x2 * x2;
+    }
     if (n >= 0.0f && n <= 64.0f && n == floorf(n)) {
         uint32_t e = (uint32_t)n;
         float result = 1.0f;
