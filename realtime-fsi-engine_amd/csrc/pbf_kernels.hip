@@ -249,14 +249,17 @@ __global__ __launch_bounds__(BS) void k_build_list(StepConst c, const uint32_t* 
 }
 
 // The walk of a gather (the second kernel): hands thread t's neighbours to
-// body(j, xj, r, r2, vj) in visit order, with xj = X[j], r = xi - xj and, when
-// V is given, vj = V[j] (loaded with xj, ahead of the test).  Every list
-// entry gets the exact test (j != self, r^2 <= h^2; the reference adds exact
-// zeros for the pairs it rejects).  A list that overflowed kNbrCap is not
-// walked: the 27 buckets are scanned again.  Four entries per 16-byte load,
-// and their records through a buffer resource over X (n slots): entries past
-// the count are never used, and a stale one past n reads zeros.
-template <bool TAPS, class Body>
+// body(j, xj, r, r2, vj) in visit order, with xj = X[j], r = xi - xj and, with
+// VEL, vj = V[j] (loaded with xj, ahead of the test).  Every list entry gets
+// the exact test (j != self, r^2 <= h^2; the reference adds exact zeros for
+// the pairs it rejects).  A list that overflowed kNbrCap is not walked: the
+// 27 buckets are scanned again.  Four entries per 16-byte load, and their
+// records through a buffer resource over X (n slots): entries past the count
+// are never used, and a stale one past n reads zeros.  Without VEL an
+// iteration takes eight entries (lambda 209.8 -> 205.8 us, delta-p 63.9 ->
+// 60.0 us at 1M particles); with VEL, whose sixteen records per iteration
+// would take 32 more registers, it takes four.
+template <bool TAPS, bool VEL, class Body>
 __device__ __forceinline__ void gather_list(V3 xi, float h, const Domain& dm, const int2* __restrict__ cells,
                                             const float4* __restrict__ X, const float4* __restrict__ V, uint32_t n,
                                             uint32_t self, uint32_t t, const uint32_t* __restrict__ list_count,
@@ -274,7 +277,11 @@ __device__ __forceinline__ void gather_list(V3 xi, float h, const Domain& dm, co
             return make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]),
                                __uint_as_float(v[3]));
         };
-        const bool withv = V != nullptr;
+        // with VEL the velocity loads stay behind a test of V: the compiler
+        // then issues the four position loads before the four velocity loads
+        // (k_integrate 139 us per launch at 1M particles; 159 us with the
+        // loads interleaved as it schedules them when the test is constant)
+        const bool withv = VEL && V != nullptr;
         auto exact = [&](uint32_t j, float4 xj, float4 vj) {
             const V3 r = sub(xi, xyz(xj));
             const float r2 = dot(r, r);
@@ -283,6 +290,29 @@ __device__ __forceinline__ void gather_list(V3 xi, float h, const Domain& dm, co
                 body(j, xj, r, r2, vj);
             }
         };
+        if constexpr (!VEL) {
+            // eight entries per iteration: two list loads, then eight record
+            // loads in flight together (the second four re-read the last
+            // group when the list ends inside the first)
+            const uint32_t last4 = (cnt - 1u) & ~3u;
+#pragma unroll 1
+            for (uint32_t e = 0; e < cnt; e += 8) {
+                const uint4 j = run[(e >> 2) * 64u];
+                const uint4 jb = run[(min(e + 4u, last4) >> 2) * 64u];
+                const float4 a0 = load(xr, j.x), a1 = load(xr, j.y), a2 = load(xr, j.z), a3 = load(xr, j.w);
+                const float4 b0 = load(xr, jb.x), b1 = load(xr, jb.y), b2 = load(xr, jb.z), b3 = load(xr, jb.w);
+                const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                exact(j.x, a0, z);
+                if (e + 1 < cnt) exact(j.y, a1, z);
+                if (e + 2 < cnt) exact(j.z, a2, z);
+                if (e + 3 < cnt) exact(j.w, a3, z);
+                if (e + 4 < cnt) exact(jb.x, b0, z);
+                if (e + 5 < cnt) exact(jb.y, b1, z);
+                if (e + 6 < cnt) exact(jb.z, b2, z);
+                if (e + 7 < cnt) exact(jb.w, b3, z);
+            }
+            return;
+        }
 #pragma unroll 1
         for (uint32_t e = 0; e < cnt; e += 4) {
             const uint4 j = run[(e >> 2) * 64u];
@@ -317,7 +347,7 @@ __device__ __forceinline__ void gather_list(V3 xi, float h, const Domain& dm, co
 //          expensive arithmetic (square root, divisions) together on the
 //          entries instead of under divergence.
 // Pairs with r^2 > h^2 are skipped: in the reference they add exact zeros.
-template <int V, bool TAPS, class Body>
+template <int V, bool TAPS, bool VEL = false, class Body>
 __device__ __forceinline__ void gather(V3 xi, float h, const Domain& dm, const int2* __restrict__ cells,
                                        const float4* __restrict__ X, const float4* __restrict__ Vel, uint32_t n,
                                        uint32_t self, uint32_t t, const uint32_t* __restrict__ list_count,
@@ -333,7 +363,7 @@ __device__ __forceinline__ void gather(V3 xi, float h, const Domain& dm, const i
             }
         });
     } else {
-        gather_list<TAPS>(xi, h, dm, cells, X, Vel, n, self, t, list_count, list, body, nnbr);
+        gather_list<TAPS, VEL>(xi, h, dm, cells, X, Vel, n, self, t, list_count, list, body, nnbr);
     }
 }
 
@@ -726,7 +756,7 @@ __global__ __launch_bounds__(BS) void k_integrate(StepConst c, const uint32_t* _
     V3 visc = mk(0.0f, 0.0f, 0.0f);
     V3 curl = mk(0.0f, 0.0f, 0.0f);
     uint32_t ncand = 0, nnbr = 0;
-    gather<V, false>(xp, c.h, domain_of<B>(c, k), cells, P, vs, c.n, k, t, list_count, list,
+    gather<V, false, true>(xp, c.h, domain_of<B>(c, k), cells, P, vs, c.n, k, t, list_count, list,
                      [&](uint32_t, float4, V3 r, float r2, float4 vj) {
         const V3 vij = sub(xyz(vj), old_vel);
         if (do_visc) {

@@ -64,10 +64,10 @@ def _single_threaded():
     os.environ["OMP_NUM_THREADS"] = "1"  # before the worker loads the oracle
 
 
-def _central_differences(jobs, workers):
+def _central_differences(jobs, workers, say=print):
     """central_difference over jobs (argument tuples), in `workers` processes
     (spawned, single-threaded: they load only numpy and the oracle); a line
-    per finished job."""
+    per finished job, through say."""
     if workers <= 1:
         return [central_difference(*j) for j in jobs]
     import multiprocessing
@@ -79,11 +79,11 @@ def _central_differences(jobs, workers):
         futs = {ex.submit(central_difference, *j): k for k, j in enumerate(jobs)}
         for done, fut in enumerate(as_completed(futs), 1):
             out[futs[fut]] = fut.result()
-            print(f"central difference {done}/{len(jobs)} done", flush=True)
+            say(f"central difference {done}/{len(jobs)} done")
     return out
 
 
-def gradient_check(cfg, parts, v0, frames, lo, hi, res, ndirs, seed=99, precision=32, workers=1):
+def gradient_check(cfg, parts, v0, frames, lo, hi, res, ndirs, seed=99, precision=32, workers=1, say=print):
     """Adjoint directional derivatives of L = sum (splat(x_final) - target)^2
     against FP64 central differences (step 1e-4) along `ndirs` seeded unit
     directions in (v0, viscosity, eps), then along viscosity alone (step 1e-4
@@ -124,7 +124,7 @@ def gradient_check(cfg, parts, v0, frames, lo, hi, res, ndirs, seed=99, precisio
         adjs.append(gvisc * dvisc + geps * deps)
         jobs.append(base + (np.zeros_like(v64), dvisc, deps, step))
     errs = []
-    for name, adj, fd in zip(names, adjs, _central_differences(jobs, workers)):
+    for name, adj, fd in zip(names, adjs, _central_differences(jobs, workers, say)):
         rel = abs(adj - fd) / max(abs(adj), abs(fd))
         errs.append(rel)
         print(f"{name}: adjoint {adj:+.8e} fd {fd:+.8e} rel {rel:.2e}", flush=True)
@@ -155,7 +155,19 @@ def _workers():
     return max(1, min(16, (os.cpu_count() or 2) - 1))
 
 
-def test_differentiable_workload_65k_gradient_fp32_trajectory(gpu):
+def _say_uncaptured(capsys):
+    """A print that reaches the terminal while the test runs: the central
+    differences of the 65,536-particle workload take minutes of CPU time, and
+    a run that prints nothing for that long looks hung."""
+
+    def say(msg):
+        with capsys.disabled():
+            print(msg, flush=True)
+
+    return say
+
+
+def test_differentiable_workload_65k_gradient_fp32_trajectory(gpu, capsys):
     """65,536 particles, 5 frames x 4 iterations x 2 substeps, 32^3 density
     loss, 16 seeded directions plus viscosity and eps.  The adjoint is exact
     for the FP32 trajectory; FP64 central differences see the FP64
@@ -163,18 +175,19 @@ def test_differentiable_workload_65k_gradient_fp32_trajectory(gpu):
     derivative by ~6e-4 per 1e-7 of noise per frame, so the bar here is 1e-2
     and the median 1e-3."""
     cfg, parts, v0, frames, lo, hi, res = c4_scene(frames=5)
-    errs = gradient_check(cfg, parts, v0, frames, lo, hi, res, 16, workers=_workers())
+    errs = gradient_check(cfg, parts, v0, frames, lo, hi, res, 16, workers=_workers(), say=_say_uncaptured(capsys))
     assert max(errs) <= 1e-2, errs
     assert float(np.median(errs)) <= 1e-3, errs
 
 
-def test_differentiable_workload_65k_20_frames_fp64_trajectory(gpu):
+def test_differentiable_workload_65k_20_frames_fp64_trajectory(gpu, capsys):
     """The differentiable-PBF workload at full length: 65,536 particles, 20
     frames x 4 iterations x 2 substeps, 32^3 density loss; the FP64
     trajectory's adjoint against FP64 central differences (step 1e-4) along
     16 seeded unit directions plus viscosity and eps, each to 1e-3."""
     cfg, parts, v0, frames, lo, hi, res = c4_scene(frames=20)
-    errs = gradient_check(cfg, parts, v0, frames, lo, hi, res, 16, precision=64, workers=_workers())
+    errs = gradient_check(cfg, parts, v0, frames, lo, hi, res, 16, precision=64, workers=_workers(),
+                          say=_say_uncaptured(capsys))
     assert max(errs) <= 1e-3, errs
 
 
