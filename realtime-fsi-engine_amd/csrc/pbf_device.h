@@ -68,6 +68,27 @@ __device__ __forceinline__ const Domain& domain_of(const StepConst& c, uint32_t 
     else return c.dom;
 }
 
+// Runs f(dom) with the Domain of slot i.  Batched, the lanes of a wave take
+// their scenes in turn: each pass serves the lanes whose scene is the first
+// remaining lane's, so the Domain is uniform in it and its fields are read
+// with scalar loads instead of occupying vector registers (a wave holds one
+// scene, or two at a scene boundary).
+template <bool B, class F>
+__device__ __forceinline__ void with_domain(const StepConst& c, uint32_t i, F&& f) {
+    if constexpr (B) {
+        const uint32_t s = c.scene_of[i];
+        for (;;) {
+            const uint32_t s0 = __builtin_amdgcn_readfirstlane(s);
+            if (s == s0) {
+                f(c.doms[s0]);
+                break;
+            }
+        }
+    } else {
+        f(c.dom);
+    }
+}
+
 struct V3 {
     float x, y, z;
 };

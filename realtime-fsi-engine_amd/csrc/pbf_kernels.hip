@@ -659,17 +659,19 @@ __global__ __launch_bounds__(BS) void k_deltap(StepConst c, const uint32_t* __re
         dp = scl(dp, maxdp / l);
     }
     V3 p = add(xi, dp);
-    if (!B && c.dom.enable_sdf == 1u) {
-        V3 nrm;
-        sdf_project(c.dom, p, nrm);
-    }
+    with_domain<B>(c, k, [&](const Domain& dom) {
+        if (dom.enable_sdf == 1u) {
+            V3 nrm;
+            sdf_project(dom, p, nrm);
+        }
+    });
     P[k] = make_float4(p.x, p.y, p.z, 0.0f);
     Q[k] = quantize(p, c.qscale);
 }
 
 // The SDF projection of CS_ApplyDeltaP.glsl:108-121 on the corrected
-// positions, in place, after the batched and the tiled delta-p gathers (the
-// single-scene k_deltap projects itself).
+// positions, in place, after the tiled delta-p gathers (k_deltap projects
+// itself).
 template <bool B>
 __global__ __launch_bounds__(kBlock) void k_apply_sdf(StepConst c, const uint32_t* __restrict__ order,
                                                       float4* __restrict__ P, uint2* __restrict__ Q,
@@ -733,8 +735,8 @@ __device__ __forceinline__ void finish_particle(const StepConst& c, const Domain
 // CS_Integrate.glsl:135-280.  The gather: XSPH sum and curl over the
 // neighbours (CS_Integrate.glsl:160-205); neighbour velocities come from the
 // sorted post-predict snapshot vs (the Jacobi form of the racy read at :171).
-// It writes the curl and, for one scene (FIN), goes on with finish_particle;
-// the batched gather writes the XSPH velocity change for k_integrate_finish.
+// It writes the curl and goes on with finish_particle (FIN), or writes the
+// XSPH velocity change for k_integrate_finish.
 template <int V, bool B, int BS, bool FIN>
 __global__ __launch_bounds__(BS) void k_integrate(StepConst c, const uint32_t* __restrict__ order,
                                                       const float4* __restrict__ P, const float4* __restrict__ vs,
@@ -772,15 +774,16 @@ __global__ __launch_bounds__(BS) void k_integrate(StepConst c, const uint32_t* _
     const V3 cw = scl(curl, inv);
     if (do_vort) curl_out[k] = make_float4(cw.x, cw.y, cw.z, 0.0f);
     if constexpr (FIN) {
-        finish_particle(c, domain_of<B>(c, k), k, xp, vs4.w, vd, do_vort ? len(cw) : 0.0f, perm, pos, vel, posS, Q);
+        const float om = do_vort ? len(cw) : 0.0f;
+        with_domain<B>(c, k, [&](const Domain& dom) {
+            finish_particle(c, dom, k, xp, vs4.w, vd, om, perm, pos, vel, posS, Q);
+        });
     } else {
         if (do_visc) vdelta[k] = make_float4(vd.x, vd.y, vd.z, 0.0f);
     }
 }
 
-// finish_particle after a gather that did not run it (the batched gather,
-// whose Domain read per lane would raise its register count, and frames
-// with neither viscosity nor vorticity).
+// finish_particle on frames with neither viscosity nor vorticity (no gather).
 template <bool B>
 __global__ __launch_bounds__(kBlock) void k_integrate_finish(StepConst c, const uint32_t* __restrict__ order,
                                                              const float4* __restrict__ P,
@@ -795,7 +798,9 @@ __global__ __launch_bounds__(kBlock) void k_integrate_finish(StepConst c, const 
     const uint32_t k = slot_at(order, t);
     const V3 visc_delta = c.enable_visc == 1u ? xyz(vdelta[k]) : mk(0.0f, 0.0f, 0.0f);
     const float om = c.enable_vort == 1u ? len(xyz(curl[k])) : 0.0f;
-    finish_particle(c, domain_of<B>(c, k), k, xyz(P[k]), vs[k].w, visc_delta, om, perm, pos, vel, posS, Q);
+    with_domain<B>(c, k, [&](const Domain& dom) {
+        finish_particle(c, dom, k, xyz(P[k]), vs[k].w, visc_delta, om, perm, pos, vel, posS, Q);
+    });
 }
 
 // CS_VorticityConfinement.glsl:133-167 (pass 1)
@@ -1444,7 +1449,7 @@ hipError_t launch_deltap(const StepConst& c, DeviceState& d, uint32_t iter, bool
         FSI_GATHER_SWITCH(L)
 #undef L
     }
-    if (d.any_sdf && (d.batched || tiled_form(d))) {
+    if (d.any_sdf && tiled_form(d)) {
 #define L(B) hipLaunchKernelGGL((k_apply_sdf<B>), grid_for(c.n), dim3(kBlock), 0, st, c, sorder(d), d.P, d.Q, d.lod, iter);
         FSI_B_SWITCH(d.batched, L)
 #undef L
@@ -1459,13 +1464,13 @@ hipError_t launch_integrate(const StepConst& c, DeviceState& d, hipStream_t st) 
             FSI_B_SWITCH(d.batched, L)
 #undef L
         }
-        // one scene: the gather finishes its particles itself
+        // the gather finishes its particles itself
 #define L(V, B, BS)                                                                                              \
-    hipLaunchKernelGGL((k_integrate<V, B, BS, !B>), grid_bs<BS>(c.n), dim3(BS), 0, st, c, sorder(d), d.P, d.vs,   \
+    hipLaunchKernelGGL((k_integrate<V, B, BS, true>), grid_bs<BS>(c.n), dim3(BS), 0, st, c, sorder(d), d.P, d.vs, \
                        gcells(d), d.list_count, d.list, d.vdelta, d.curl, sperm(d), d.pos, d.vel, d.posS, d.Q);
         FSI_GATHER_SWITCH(L)
 #undef L
-        if (!d.batched) return hipGetLastError();
+        return hipGetLastError();
     }
 #define L(B)                                                                                                   \
     hipLaunchKernelGGL((k_integrate_finish<B>), grid_for(c.n), dim3(kBlock), 0, st, c, sorder(d), d.P, d.vs, d.vdelta, \
