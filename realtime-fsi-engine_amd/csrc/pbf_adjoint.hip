@@ -315,6 +315,27 @@ __global__ __launch_bounds__(kAB) void k_edges(StepConst c, const float4* __rest
 
 // ---- recording forward ----------------------------------------------------
 
+// f(j, X[j]) for the m entries of slot i's list (pair r at nbr[r * n + i]), in
+// list order: four entries, then their four records, are loaded before any
+// is used (one entry per iteration left each load's latency exposed).
+// Entries past m repeat the last one and are not used.
+template <class T4, class F>
+__device__ __forceinline__ void walk4(const uint32_t* __restrict__ nbr, uint32_t n, uint32_t i, uint32_t m,
+                                      const T4* __restrict__ X, F&& f) {
+#pragma unroll 1
+    for (uint32_t r = 0; r < m; r += 4) {
+        uint32_t j[4];
+        T4 x[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) j[u] = nbr[(size_t)min(r + (uint32_t)u, m - 1u) * n + i];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) x[u] = X[j[u]];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (r + (uint32_t)u < m) f(j[u], x[u]);
+    }
+}
+
 // CS_ComputeLambdas.glsl:108-162 over the list; tape: (lambda, rho), (g, D).
 __global__ __launch_bounds__(kAB) void k_rec_lambda(StepConst c, const float4* __restrict__ P,
                                                     const uint32_t* __restrict__ nbr, const uint32_t* __restrict__ cnt,
@@ -327,17 +348,15 @@ __global__ __launch_bounds__(kAB) void k_rec_lambda(StepConst c, const float4* _
     float rho = c.w0_self;
     float sum_g2 = 0.0f;
     V3 gi = zero3();
-    const uint32_t m = cnt[i];
-    for (uint32_t r = 0; r < m; ++r) {
-        const uint32_t j = nbr[(size_t)r * c.n + i];
-        const V3 d = sub(xi, xyz(P[j]));
+    walk4(nbr, c.n, i, cnt[i], P, [&](uint32_t, const float4& xj) {
+        const V3 d = sub(xi, xyz(xj));
         const float r2 = dot(d, d);
         rho = rho + poly6_r2(c, r2);
         const V3 gw = spiky(c, d, r2);
         const float gj2 = dot(gw, gw);
         sum_g2 = sum_g2 + gj2 * (inv * inv);
         gi = add(gi, gw);
-    }
+    });
     gi = scl(gi, inv);
     float C = rho / c.rho0 - 1.0f;
     const float floorc = -c.cohesion;
@@ -362,15 +381,13 @@ __global__ __launch_bounds__(kAB) void k_rec_deltap(StepConst c, const float4* _
     const V3 xi = xyz(xi4);
     const float li = xi4.w;
     V3 dp = zero3();
-    const uint32_t m = cnt[i];
-    for (uint32_t r = 0; r < m; ++r) {
-        const float4 a = PL[nbr[(size_t)r * c.n + i]];
+    walk4(nbr, c.n, i, cnt[i], PL, [&](uint32_t, const float4& a) {
         const V3 d = sub(xi, xyz(a));
         const float r2 = dot(d, d);
         const V3 gw = spiky(c, d, r2);
         const float sc = scorr_r2(c, r2);
         dp = add(dp, scl(gw, li + a.w + sc));
-    }
+    });
     dp = scl(dp, c.inv_rho0);
     dptape[i] = f4(dp);
     const float maxdp = 0.5f * c.h;
@@ -544,7 +561,15 @@ __device__ __forceinline__ D3 d_spiky_jvp(const C64& k, D3 r, double r2, D3 v) {
     const double dphi = k.spiky * (2.0 * hd / d + hd * hd / (d * d));
     return add(scl(v, phi), scl(r, dphi / d * dot(r, v)));
 }
+// x^n by binary powering for integer n in [0, 64]; n = 4 and n = 3 (s_corr's
+// exponent and its derivative's at the configured n = 4) written out as the
+// powering's own products: (x x)(x x), and x (x x) after 1 x = x.
 __device__ __forceinline__ double d_powr(double x, double n) {
+    if (n == 4.0) {
+        const double x2 = x * x;
+        return x2 * x2;
+    }
+    if (n == 3.0) return x * (x * x);
     if (n >= 0.0 && n <= 64.0 && n == floor(n)) {
         uint32_t e = (uint32_t)n;
         double result = 1.0, base = x;
@@ -685,16 +710,15 @@ __global__ __launch_bounds__(kAB) void k_rec_lambda64(C64 k, const double4* __re
     double rho = k.w0_self;
     double sum_g2 = 0.0;
     D3 gi = dz3();
-    const uint32_t m = cnt[i];
-    for (uint32_t r = 0; r < m; ++r) {
-        const D3 d = sub(xi, d3(P[nbr[(size_t)r * k.n + i]]));
+    walk4(nbr, k.n, i, cnt[i], P, [&](uint32_t, const double4& xj) {
+        const D3 d = sub(xi, d3(xj));
         const double r2 = dot(d, d);
         rho = rho + d_poly6(k, r2);
         const D3 gw = d_spiky(k, d, r2);
         const double gj2 = dot(gw, gw);
         sum_g2 = sum_g2 + gj2 * (inv * inv);
         gi = add(gi, gw);
-    }
+    });
     gi = scl(gi, inv);
     double C = rho / k.rho0 - 1.0;
     const double floorc = -k.cohesion;
@@ -717,13 +741,11 @@ __global__ __launch_bounds__(kAB) void k_rec_deltap64(C64 k, const double4* __re
     const D3 xi = d3(xi4);
     const double li = xi4.w;
     D3 dp = dz3();
-    const uint32_t m = cnt[i];
-    for (uint32_t r = 0; r < m; ++r) {
-        const double4 a = PL[nbr[(size_t)r * k.n + i]];
+    walk4(nbr, k.n, i, cnt[i], PL, [&](uint32_t, const double4& a) {
         const D3 d = sub(xi, d3(a));
         const double r2 = dot(d, d);
         dp = add(dp, scl(d_spiky(k, d, r2), li + a.w + d_scorr(k, r2)));
-    }
+    });
     dp = scl(dp, k.inv_rho0);
     dptape[i] = d4(dp);
     const double maxdp = 0.5 * k.h;
