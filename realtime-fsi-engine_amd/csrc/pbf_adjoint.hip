@@ -258,20 +258,25 @@ __device__ __forceinline__ uint32_t quad_list(const T4* __restrict__ X, const in
         }
         m += (uint32_t)__popc(qm);
     };
-    int2 se = cells[bw.bucket(0)];
+    // the cell-table entries of nine buckets are loaded together (one bucket
+    // ahead left a load's latency exposed per bucket: a bucket is mostly one
+    // iteration of the quad)
 #pragma unroll 1
-    for (int q = 0; q < 27; ++q) {
-        int2 nxt = make_int2(-1, -1);
-        if (q < 26) nxt = cells[bw.bucket(q + 1)];
-        // an empty bucket is (-1, -1): no iterations
-        const uint32_t end = (uint32_t)se.y, last = end - 1u;
-        for (uint32_t b = (uint32_t)se.x; b < end; b += 2u * kQ) {
-            const uint32_t sa = b + lane, sb = b + kQ + lane;
-            const T4 xa = X[min(sa, last)], xb = X[min(sb, last)];
-            place(sa <= last && sa != i && inside(xa), sa);
-            place(sb <= last && sb != i && inside(xb), sb);
+    for (int q0 = 0; q0 < 27; q0 += 9) {
+        int2 se[9];
+#pragma unroll
+        for (int j = 0; j < 9; ++j) se[j] = cells[bw.bucket(q0 + j)];
+#pragma unroll
+        for (int j = 0; j < 9; ++j) {
+            // an empty bucket is (-1, -1): no iterations
+            const uint32_t end = (uint32_t)se[j].y, last = end - 1u;
+            for (uint32_t b = (uint32_t)se[j].x; b < end; b += 2u * kQ) {
+                const uint32_t sa = b + lane, sb = b + kQ + lane;
+                const T4 xa = X[min(sa, last)], xb = X[min(sb, last)];
+                place(sa <= last && sa != i && inside(xa), sa);
+                place(sb <= last && sb != i && inside(xb), sb);
+            }
         }
-        se = nxt;
     }
     return m;
 }
